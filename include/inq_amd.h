@@ -1,0 +1,161 @@
+/*
+ * inq_amd.h -- C ABI of the MI355X (gfx950) plane-wave Hpsi / real-time
+ * propagator library.
+ *
+ * The library replaces, for one GPU, the hot path of INQ's
+ *   hamiltonian::ks_hamiltonian   (src/hamiltonian/ks_hamiltonian.hpp:35-228)
+ *   operations::transform         (src/operations/transform.hpp:337-400)
+ *                                 (also reachable as operations::space)
+ *   hamiltonian::projector_all    (src/hamiltonian/projector_all.hpp:23-421)
+ *   operations::exponential*      (src/operations/exponential.hpp:21-69)
+ *   real_time::etrs               (src/real_time/etrs.hpp:22-82)
+ * INQ's host code keeps its own containers and calls these functions with
+ * plain device pointers.
+ *
+ * Conventions
+ *   - Orbitals are complex128 (two interleaved doubles) in INQ's layout
+ *     [x][y][z][band], band index contiguous, `ld` elements between grid
+ *     points (ld >= nst), exactly field_set::matrix() (field_set.hpp:51).
+ *   - Vectors suffixed _cov are covariant components (cell.hpp:179-192).
+ *   - All orbital pointers are DEVICE pointers owned by the caller.
+ *     Tables passed at creation are HOST pointers, copied by the library.
+ *   - Every call is asynchronous on the handle's stream; nothing
+ *     synchronises the device unless the function says so.
+ *   - Return value: 0 on success, non-zero status otherwise; the message is
+ *     available from inqamd_last_error(handle).  The library never aborts.
+ *   - One handle per stream; a handle must not be used from two host threads
+ *     at once without external locking.
+ */
+#ifndef INQ_AMD_H
+#define INQ_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  INQAMD_OK = 0,
+  INQAMD_ERR_ARGUMENT = 1,
+  INQAMD_ERR_HIP = 2,
+  INQAMD_ERR_MEMORY = 3,
+  INQAMD_ERR_UNSUPPORTED = 4
+};
+
+typedef struct inqamd_hamiltonian *inqamd_hamiltonian_t;
+
+/* Everything ks_hamiltonian needs for a frozen (non-interacting) potential:
+ * the grid (basis::real_space, real_space.hpp:30-33), V_KS
+ * (ks_hamiltonian.hpp:45) and the packed projector tables of projector_all
+ * (projector_all.hpp:408-419).  Projector tables may be absent (nproj = 0). */
+typedef struct {
+  int n[3];                /* FFT grid */
+  double lattice[9];       /* a0, a1, a2 in bohr, row-major */
+  const double *vloc;      /* host [N] scalar potential (Ha) */
+  int nproj;               /* number of projector spheres (atoms) */
+  int max_sphere;          /* max_sphere_size_ */
+  int max_nlm;             /* max_nlm_ (<= 16) */
+  const int *points;       /* host [nproj][max_sphere][3] grid coords, -1 padded */
+  const float *positions;  /* host [nproj][max_sphere][3] contravariant r - R_atom (FP32) */
+  const double *matrices;  /* host [nproj][max_nlm][max_sphere] beta_lm(r_i) */
+  const double *coeff;     /* host [nproj][max_nlm] KB coefficients */
+  const int *nlm;          /* host [nproj] projectors per sphere */
+  const int *sphere_size;  /* host [nproj] points per sphere */
+  int max_bands;           /* largest nst any later call will pass */
+} inqamd_system_desc;
+
+/* ---- handle --------------------------------------------------------------
+ * Replaces ks_hamiltonian's constructor (ks_hamiltonian.hpp:79-89) and
+ * projector_all(projectors) (projector_all.hpp:86-94).  `stream` is a
+ * hipStream_t (NULL = default stream).  Allocates every workspace the later
+ * calls need, so they allocate nothing. */
+int inqamd_hamiltonian_create(const inqamd_system_desc *desc, void *stream,
+                              inqamd_hamiltonian_t *out);
+void inqamd_hamiltonian_destroy(inqamd_hamiltonian_t h);
+int inqamd_set_stream(inqamd_hamiltonian_t h, void *stream);
+const char *inqamd_last_error(inqamd_hamiltonian_t h);
+const char *inqamd_version(void);
+
+/* orbital_set::kpoint() (orbital_set.hpp) -- covariant k. */
+int inqamd_set_kpoint(inqamd_hamiltonian_t h, const double k_cov[3]);
+/* ks_hamiltonian::uniform_vector_potential() = A (ks_hamiltonian.hpp:216-221),
+ * as self_consistency::update_hamiltonian sets it (self_consistency.hpp:152-156). */
+int inqamd_set_uniform_vector_potential(inqamd_hamiltonian_t h, const double A_cov[3]);
+/* ks_hamiltonian::scalar_potential() (ks_hamiltonian.hpp:200-202): replace V
+ * from a DEVICE array [N] (stream ordered). */
+int inqamd_set_scalar_potential(inqamd_hamiltonian_t h, const double *vloc_device);
+
+/* ---- Hpsi ------------------------------------------------------------------
+ * ks_hamiltonian::operator()(orbital_set<real_space, complex>)
+ * (ks_hamiltonian.hpp:132-152): hpsi = T(k+A) psi + V psi + V_nl(k+A) psi.
+ * psi and hpsi must not alias. */
+int inqamd_hamiltonian_apply(inqamd_hamiltonian_t h, const void *psi, void *hpsi, int nst,
+                             int ld);
+/* ks_hamiltonian::non_local(orbital_set<real_space, complex>)
+ * (ks_hamiltonian.hpp:104-128): vnlpsi = V_nl(k+A) psi. */
+int inqamd_non_local(inqamd_hamiltonian_t h, const void *psi, void *vnlpsi, int nst, int ld);
+
+/* ---- operations::transform (alias operations::space) ----------------------
+ * to_fourier (transform.hpp:337-367): unnormalised forward 3-D DFT.
+ * to_real (transform.hpp:371-400): backward 3-D DFT, x 1/N if normalize.
+ * in and out may be the same pointer. */
+int inqamd_transform_to_fourier(inqamd_hamiltonian_t h, const void *in, void *out, int nst,
+                                int ld);
+int inqamd_transform_to_real(inqamd_hamiltonian_t h, const void *in, void *out, int nst, int ld,
+                             int normalize);
+
+/* ---- projector_all ----------------------------------------------------------
+ * project (projector_all.hpp:98-216): sphere[p][i][b] = V_nl psi restricted
+ * to sphere p ([nproj][max_sphere][nst] complex, device).
+ * apply (projector_all.hpp:220-232): hpsi(r_pi, b) += e^{-i(k+A).r} sphere.
+ * Scatter order is fixed (no atomics): results are bitwise reproducible. */
+int inqamd_projector_project(inqamd_hamiltonian_t h, const void *psi, void *sphere, int nst,
+                             int ld);
+int inqamd_projector_apply(inqamd_hamiltonian_t h, const void *sphere, void *hpsi, int nst,
+                           int ld);
+
+/* ---- reductions --------------------------------------------------------------
+ * energy::calculate eigenvalues (energy.hpp:74-75 via
+ * overlap_diagonal_normalized, overlap_diagonal.hpp:144-171):
+ * eps[b] = Re<hpsi_b|psi_b> / <psi_b|psi_b>.  eps is a DEVICE array [nst]. */
+int inqamd_band_energies(inqamd_hamiltonian_t h, const void *hpsi, const void *psi,
+                         double *eps, int nst, int ld);
+
+/* ---- propagation --------------------------------------------------------------
+ * operations::shift (shift.hpp:33-44): y += (re + i im) x. */
+int inqamd_shift(inqamd_hamiltonian_t h, double re, double im, const void *x, void *y, int nst,
+                 int ld);
+/* operations::exponential_in_place (exponential.hpp:21-34):
+ * psi <- sum_{n<=order} (z H)^n psi / n!  with the handle's current k+A. */
+int inqamd_exponential_in_place(inqamd_hamiltonian_t h, double zre, double zim, int order,
+                                void *psi, int nst, int ld);
+/* operations::exponential_2_for_1 (exponential.hpp:50-69):
+ * expphi <- T(z1) phi, phi <- T(z2) phi, sharing the H^n phi terms. */
+int inqamd_exponential_2_for_1(inqamd_hamiltonian_t h, double z1re, double z1im, double z2re,
+                               double z2im, int order, void *phi, void *expphi, int nst, int ld);
+/* real_time::etrs for a frozen potential (etrs.hpp:22-82):
+ * psi <- T(i dt/2, H(A_tdt)) T(i dt/2, H(A_t)) psi; leaves A = A_tdt. */
+int inqamd_etrs_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov[3],
+                     const double A_tdt_cov[3], int order, void *psi, int nst, int ld);
+
+/* ---- host-side system builders (no GPU needed) --------------------------------
+ * containing_cube (containing_cube.hpp:20-45). */
+int inqamd_containing_cube(const double lattice[9], const int n[3], const double pos[3],
+                           double radius, int lo[3], int hi[3]);
+/* spherical_grid(basis, center, radius) (spherical_grid.hpp:58-158): grid
+ * coords, FP32 distance and FP32 contravariant relative positions, in the
+ * reference's point order.  Writes at most `capacity` points and stores the
+ * full count in *count; returns INQAMD_ERR_MEMORY if capacity was short. */
+int inqamd_sphere_build(const double lattice[9], const int n[3], const double center[3],
+                        double radius, int capacity, int *coords, float *distance,
+                        float *relpos, int *count);
+/* projector::build for a Gaussian radial function exp(-r^2/sigma^2) and real
+ * harmonics l=0 (nlm=1) or l=0,1 ordered y,z,x (nlm=4)
+ * (projector.hpp:34-50): matrix[lm*ld + i]. */
+int inqamd_gaussian_projector_matrix(const double lattice[9], int npoint, const float *distance,
+                                     const float *relpos, double sigma, int nlm, double *matrix,
+                                     int ld);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,69 @@
+// Batched complex128 3-D FFT over INQ's [x][y][z][band] layout, built from
+// one-axis "pencil" passes.  Each pass stages a tile of `len` points x B
+// contiguous bands in LDS, runs a mixed-radix Stockham FFT there and writes
+// the tile back, optionally fused with the kinetic multiply (fwd, x T(G), bwd
+// in one pass) or with the Hpsi epilogue (x 1/N + V psi).
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace inqamd {
+
+constexpr int FFT_MAX_STAGES = 12;
+constexpr int FFT_MAX_GENERIC_RADIX = 31;  // largest prime factor supported
+constexpr int FFT_BLOCK = 256;
+
+enum fft_mode : int { FFT_PLAIN = 0, FFT_KINETIC = 1, FFT_EPILOGUE = 2 };
+
+struct fft_axis_plan {
+  int len = 0;
+  int nstages = 0;
+  int radix[FFT_MAX_STAGES] = {};
+  cplx *tw = nullptr;  // device: W_len^k = exp(-2 pi i k / len), k < len
+};
+
+// Arguments of one pencil pass (passed by value to the kernel).
+struct fft_pass_args {
+  const cplx *in;
+  long in_ld;
+  cplx *out;
+  long out_ld;
+  int nst;
+  int len;
+  long stride;  // point stride along the transformed axis
+  int c1;       // other axis 1 (fastest varying over workgroups)
+  long s1;
+  int c2;
+  long s2;
+  int nstages;
+  int radix[FFT_MAX_STAGES];
+  const cplx *tw;
+  double scale;
+  int sign;
+  // FFT_KINETIC: element (i, o1, o2) sits at grid coords c[ax]=i, c[ax1]=o1, c[ax2]=o2
+  int n[3];
+  int ax, ax1, ax2;
+  double rlat[9];      // reciprocal lattice rows (include 2 pi)
+  const double *kA;    // device, covariant k + A
+  // FFT_EPILOGUE: out = scale * fft + V[p] * psi[p][b]
+  const double *V;
+  const cplx *psi;
+  long psi_ld;
+};
+
+// Factor len into radices (8, 4, 2, 3, 5, then generic primes <= 31).
+// Returns false if a prime factor exceeds FFT_MAX_GENERIC_RADIX.
+bool fft_factor(int len, std::vector<int> &radices);
+
+// Host twiddle table exp(-2 pi i k/len), evaluated in long double.
+std::vector<cplx> fft_twiddles(int len);
+
+// Band tile chosen for a given nst and axis length.
+int fft_band_tile(int nst, int len);
+
+// Launch one pencil pass of the given mode on `stream`.
+void fft_launch_pass(const fft_pass_args &a, int mode, hipStream_t stream);
+
+}  // namespace inqamd

@@ -1,0 +1,608 @@
+// Hamiltonian handle, Hpsi, transforms, reductions and the Taylor/ETRS
+// propagator, exported through the C ABI of include/inq_amd.h.
+
+#include "internal.hpp"
+#include "inq_amd.h"
+
+#include <cmath>
+#include <complex>
+#include <new>
+
+namespace inqamd {
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void set_vec3_kernel(double *dst, double x, double y, double z) {
+  if (threadIdx.x == 0) {
+    dst[0] = x;
+    dst[1] = y;
+    dst[2] = z;
+  }
+}
+
+// y[p][b] += c * x[p][b]  (operations::shift, shift.hpp:39-43)
+__global__ __launch_bounds__(256) void shift_kernel(cplx *y, long yld, const cplx *x, long xld,
+                                                    int nst, long total, double cre, double cim) {
+  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (long)gridDim.x * blockDim.x) {
+    const long p = g / nst;
+    const int b = (int)(g - p * nst);
+    const cplx xv = x[p * xld + b];
+    cplx yv = y[p * yld + b];
+    yv.x += cre * xv.x - cim * xv.y;
+    yv.y += cre * xv.y + cim * xv.x;
+    y[p * yld + b] = yv;
+  }
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(cplx *y, long yld, const cplx *x, long xld,
+                                                   int nst, long total) {
+  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (long)gridDim.x * blockDim.x) {
+    const long p = g / nst;
+    const int b = (int)(g - p * nst);
+    y[p * yld + b] = x[p * xld + b];
+  }
+}
+
+constexpr int EN_BT = 64;
+constexpr int EN_LANES = 4;
+constexpr long EN_CHUNK = 256;
+
+// Per-chunk partial sums of Re(conj(hpsi) psi) and |psi|^2 for every band
+// (overlap_diagonal_normalized_mult, overlap_diagonal.hpp:118-128).
+__global__ __launch_bounds__(256) void energies_partial_kernel(const cplx *hpsi, const cplx *psi,
+                                                               long ld, int nst, long npoint,
+                                                               double *partial) {
+  __shared__ double red[2][EN_LANES][EN_BT];
+  const int bb = threadIdx.x % EN_BT;
+  const int lane = threadIdx.x / EN_BT;
+  const int b = blockIdx.y * EN_BT + bb;
+  const long p0 = (long)blockIdx.x * EN_CHUNK;
+  const long p1 = p0 + EN_CHUNK < npoint ? p0 + EN_CHUNK : npoint;
+  double v = 0.0, nn = 0.0;
+  if (b < nst) {
+    for (long p = p0 + lane; p < p1; p += EN_LANES) {
+      const cplx h = hpsi[p * ld + b];
+      const cplx s = psi[p * ld + b];
+      v += h.x * s.x + h.y * s.y;
+      nn += s.x * s.x + s.y * s.y;
+    }
+  }
+  red[0][lane][bb] = v;
+  red[1][lane][bb] = nn;
+  __syncthreads();
+  if (lane == 0 && b < nst) {
+    for (int l = 1; l < EN_LANES; l++) {
+      v += red[0][l][bb];
+      nn += red[1][l][bb];
+    }
+    partial[((long)blockIdx.x * nst + b) * 2 + 0] = v;
+    partial[((long)blockIdx.x * nst + b) * 2 + 1] = nn;
+  }
+}
+
+__global__ void energies_final_kernel(const double *partial, int nchunk, int nst, double *eps) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nst) return;
+  double v = 0.0, nn = 0.0;
+  for (int c = 0; c < nchunk; c++) {
+    v += partial[((long)c * nst + b) * 2 + 0];
+    nn += partial[((long)c * nst + b) * 2 + 1];
+  }
+  eps[b] = v / nn;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+static unsigned grid_for(long total) {
+  long blocks = (total + 255) / 256;
+  if (blocks > 65536L * 8) blocks = 65536L * 8;
+  if (blocks < 1) blocks = 1;
+  return (unsigned)blocks;
+}
+
+static void set_slot(handle &h, int slot, const double v[3]) {
+  hipLaunchKernelGGL(set_vec3_kernel, dim3(1), dim3(64), 0, h.stream, h.kA + 3 * slot, v[0], v[1],
+                     v[2]);
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+static void update_current_slot(handle &h) {
+  double kA[3] = {h.k_cov[0] + h.A_cov[0], h.k_cov[1] + h.A_cov[1], h.k_cov[2] + h.A_cov[2]};
+  set_slot(h, KA_CURRENT, kA);
+}
+
+static const long *point_strides(const handle &h, long ps[3]) {
+  ps[0] = (long)h.n[1] * h.n[2];
+  ps[1] = h.n[2];
+  ps[2] = 1;
+  return ps;
+}
+
+// Pencil pass along `axis`; the fastest-varying other axis enumerates the
+// workgroups right after the band tiles (neighbouring memory).
+static fft_pass_args pass_args(const handle &h, int axis, const cplx *in, long in_ld, cplx *out,
+                               long out_ld, int nst, int sign, double scale) {
+  static const int others[3][2] = {{2, 1}, {2, 0}, {1, 0}};
+  long ps[3];
+  point_strides(h, ps);
+  fft_pass_args a{};
+  a.in = in;
+  a.in_ld = in_ld;
+  a.out = out;
+  a.out_ld = out_ld;
+  a.nst = nst;
+  a.len = h.n[axis];
+  a.stride = ps[axis];
+  a.ax = axis;
+  a.ax1 = others[axis][0];
+  a.ax2 = others[axis][1];
+  a.c1 = h.n[a.ax1];
+  a.s1 = ps[a.ax1];
+  a.c2 = h.n[a.ax2];
+  a.s2 = ps[a.ax2];
+  a.nstages = h.axis[axis].nstages;
+  for (int s = 0; s < FFT_MAX_STAGES; s++) a.radix[s] = h.axis[axis].radix[s];
+  a.tw = h.axis[axis].tw;
+  a.scale = scale;
+  a.sign = sign;
+  for (int d = 0; d < 3; d++) a.n[d] = h.n[d];
+  for (int d = 0; d < 9; d++) a.rlat[d] = h.rlat[d];
+  a.kA = h.kA;
+  return a;
+}
+
+// 3-D transform as three pencil passes (x, y, z); the last one applies scale.
+static void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int nst,
+                  int sign, double scale) {
+  fft_launch_pass(pass_args(h, 0, in, in_ld, out, out_ld, nst, sign, 1.0), FFT_PLAIN, h.stream);
+  fft_launch_pass(pass_args(h, 1, out, out_ld, out, out_ld, nst, sign, 1.0), FFT_PLAIN, h.stream);
+  fft_launch_pass(pass_args(h, 2, out, out_ld, out, out_ld, nst, sign, scale), FFT_PLAIN, h.stream);
+}
+
+// ks_hamiltonian::operator() (ks_hamiltonian.hpp:132-152) as
+//   project (P coefficients)                         :136
+//   x-fwd, y-fwd, [z-fwd, x 1/2|G+k+A|^2, z-bwd]     :138-140, :146 (shift)
+//   y-bwd, [x-bwd, x 1/N, + V psi]                   :144-146
+//   apply (ordered scatter of V_nl psi)              :149
+static void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
+                              int slot) {
+  projector_project_coefficients(h, psi, ld, nst, slot);
+  cplx *w = h.work;
+  const long wl = nst;
+  fft_launch_pass(pass_args(h, 0, psi, ld, w, wl, nst, -1, 1.0), FFT_PLAIN, h.stream);
+  fft_launch_pass(pass_args(h, 1, w, wl, w, wl, nst, -1, 1.0), FFT_PLAIN, h.stream);
+  fft_pass_args kz = pass_args(h, 2, w, wl, w, wl, nst, -1, 1.0);
+  kz.kA = h.kA + 3 * slot;
+  fft_launch_pass(kz, FFT_KINETIC, h.stream);
+  fft_launch_pass(pass_args(h, 1, w, wl, w, wl, nst, +1, 1.0), FFT_PLAIN, h.stream);
+  fft_pass_args ep = pass_args(h, 0, w, wl, hpsi, hld, nst, +1, 1.0 / (double)h.npoint);
+  ep.V = h.V;
+  ep.psi = psi;
+  ep.psi_ld = ld;
+  fft_launch_pass(ep, FFT_EPILOGUE, h.stream);
+  projector_apply_coefficients(h, hpsi, hld, nst, slot);
+}
+
+static void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
+                  int nst) {
+  const long total = h.npoint * nst;
+  hipLaunchKernelGGL(shift_kernel, dim3(grid_for(total)), dim3(256), 0, h.stream, y, yld, x, xld,
+                     nst, total, c.real(), c.imag());
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+static void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst) {
+  const long total = h.npoint * nst;
+  hipLaunchKernelGGL(copy_kernel, dim3(grid_for(total)), dim3(256), 0, h.stream, y, yld, x, xld,
+                     nst, total);
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+// exponential_in_place (exponential.hpp:21-34)
+static void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
+                                 int nst, int slot) {
+  cplx *hn = h.hn, *hn2 = h.hn2;
+  const long tl = nst;
+  hamiltonian_apply(h, psi, ld, hn, tl, nst, slot);
+  std::complex<double> coeff = 1.0;
+  for (int iter = 1; iter <= order; iter++) {
+    if (iter > 1) {
+      hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot);
+      std::swap(hn, hn2);
+    }
+    coeff *= z / double(iter);
+    shift(h, coeff, hn, tl, psi, ld, nst);
+  }
+}
+
+// exponential_2_for_1 (exponential.hpp:50-69)
+static void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2,
+                                int order, cplx *phi, cplx *expphi, long ld, int nst, int slot) {
+  copy(h, phi, ld, expphi, ld, nst);
+  cplx *hn = h.hn, *hn2 = h.hn2;
+  const long tl = nst;
+  hamiltonian_apply(h, phi, ld, hn, tl, nst, slot);
+  std::complex<double> c1 = 1.0, c2 = 1.0;
+  for (int iter = 1; iter <= order; iter++) {
+    if (iter > 1) {
+      hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot);
+      std::swap(hn, hn2);
+    }
+    c1 *= z1 / double(iter);
+    c2 *= z2 / double(iter);
+    shift(h, c1, hn, tl, expphi, ld, nst);
+    shift(h, c2, hn, tl, phi, ld, nst);
+  }
+}
+
+static void band_energies(handle &h, const cplx *hpsi, const cplx *psi, long ld, int nst,
+                          double *eps) {
+  const int nchunk = (int)((h.npoint + EN_CHUNK - 1) / EN_CHUNK);
+  dim3 grid(nchunk, (nst + EN_BT - 1) / EN_BT);
+  hipLaunchKernelGGL(energies_partial_kernel, grid, dim3(256), 0, h.stream, hpsi, psi, ld, nst,
+                     h.npoint, h.partial);
+  INQAMD_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(energies_final_kernel, dim3((nst + 63) / 64), dim3(64), 0, h.stream,
+                     (const double *)h.partial, nchunk, nst, eps);
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// create / destroy
+// ---------------------------------------------------------------------------
+static void *dalloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) return nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    throw error{ERR_MEMORY, std::string("hipMalloc(") + std::to_string(bytes) + "): " +
+                                hipGetErrorString(e)};
+  return p;
+}
+
+static void release(handle &h) {
+  for (int a = 0; a < 3; a++)
+    if (h.axis[a].tw) (void)hipFree(h.axis[a].tw), h.axis[a].tw = nullptr;
+  for (void *p : {(void *)h.V, (void *)h.kA, (void *)h.work, (void *)h.hn, (void *)h.hn2,
+                  (void *)h.proj, (void *)h.partial})
+    if (p) (void)hipFree(p);
+  h.V = nullptr;
+  h.kA = nullptr;
+  h.work = h.hn = h.hn2 = h.proj = nullptr;
+  h.partial = nullptr;
+  projector_free(h);
+}
+
+static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
+  for (int a = 0; a < 3; a++) INQAMD_REQUIRE(d.n[a] > 0, "grid sizes must be positive");
+  INQAMD_REQUIRE(d.max_bands > 0, "max_bands must be positive");
+  INQAMD_REQUIRE(d.vloc != nullptr, "vloc is required");
+  INQAMD_REQUIRE(d.max_nlm >= 0 && d.max_nlm <= 16, "max_nlm must be in [0, 16]");
+  INQAMD_REQUIRE(d.nproj >= 0 && d.max_sphere >= 0, "negative projector sizes");
+  if (d.nproj > 0)
+    INQAMD_REQUIRE(d.points && d.positions && d.matrices && d.coeff && d.nlm && d.sphere_size,
+                   "projector tables are incomplete");
+  h.stream = stream;
+  for (int a = 0; a < 3; a++) h.n[a] = d.n[a];
+  h.npoint = (long)d.n[0] * d.n[1] * d.n[2];
+  h.max_bands = d.max_bands;
+  for (int i = 0; i < 9; i++) h.lattice[i] = d.lattice[i];
+  // cell.hpp:46-52
+  const double *a0 = d.lattice, *a1 = d.lattice + 3, *a2 = d.lattice + 6;
+  auto cross = [](const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double c12[3], c20[3], c01[3];
+  cross(a1, a2, c12);
+  cross(a2, a0, c20);
+  cross(a0, a1, c01);
+  h.volume = a0[0] * c12[0] + a0[1] * c12[1] + a0[2] * c12[2];
+  INQAMD_REQUIRE(h.volume > 1e-10, "the lattice volume is not positive");
+  for (int j = 0; j < 3; j++) {
+    h.rlat[0 * 3 + j] = 2.0 * M_PI / h.volume * c12[j];
+    h.rlat[1 * 3 + j] = 2.0 * M_PI / h.volume * c20[j];
+    h.rlat[2 * 3 + j] = 2.0 * M_PI / h.volume * c01[j];
+  }
+  for (int a = 0; a < 3; a++) {
+    std::vector<int> rad;
+    if (!fft_factor(h.n[a], rad))
+      throw error{ERR_UNSUPPORTED, "grid size " + std::to_string(h.n[a]) +
+                                       " has a prime factor above 31"};
+    if (2L * h.n[a] * fft_band_tile(1, h.n[a]) * (long)sizeof(cplx) > 65536)
+      throw error{ERR_UNSUPPORTED, "grid size " + std::to_string(h.n[a]) + " is too long"};
+    h.axis[a].len = h.n[a];
+    h.axis[a].nstages = (int)rad.size();
+    for (size_t s = 0; s < rad.size(); s++) h.axis[a].radix[s] = rad[s];
+    std::vector<cplx> tw = fft_twiddles(h.n[a]);
+    h.axis[a].tw = (cplx *)dalloc(tw.size() * sizeof(cplx));
+    INQAMD_HIP_CHECK(hipMemcpy(h.axis[a].tw, tw.data(), tw.size() * sizeof(cplx),
+                               hipMemcpyHostToDevice));
+  }
+  h.V = (double *)dalloc(h.npoint * sizeof(double));
+  INQAMD_HIP_CHECK(hipMemcpy(h.V, d.vloc, h.npoint * sizeof(double), hipMemcpyHostToDevice));
+  h.kA = (double *)dalloc(KA_NSLOTS * 3 * sizeof(double));
+  std::vector<double> zeros(KA_NSLOTS * 3, 0.0);
+  INQAMD_HIP_CHECK(hipMemcpy(h.kA, zeros.data(), zeros.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+  const size_t orb = (size_t)h.npoint * h.max_bands * sizeof(cplx);
+  h.work = (cplx *)dalloc(orb);
+  h.hn = (cplx *)dalloc(orb);
+  h.hn2 = (cplx *)dalloc(orb);
+  const long nchunk = (h.npoint + EN_CHUNK - 1) / EN_CHUNK;
+  h.npartial = nchunk * h.max_bands * 2;
+  h.partial = (double *)dalloc(h.npartial * sizeof(double));
+  projector_upload(h, d.nproj, d.max_sphere, d.max_nlm, d.points, d.positions, d.matrices,
+                   d.coeff, d.nlm, d.sphere_size);
+  if (h.pt.nproj > 0)
+    h.proj = (cplx *)dalloc((size_t)h.pt.nproj * h.pt.max_nlm * h.max_bands * sizeof(cplx));
+}
+
+}  // namespace inqamd
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace inqamd;
+
+namespace {
+
+template <typename F>
+int guarded(inqamd_hamiltonian *h, F &&f) {
+  try {
+    f();
+    if (h) h->last_error.clear();
+    return INQAMD_OK;
+  } catch (const inqamd::error &e) {
+    if (h) h->last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    if (h) h->last_error = "host allocation failed";
+    return INQAMD_ERR_MEMORY;
+  } catch (const std::exception &e) {
+    if (h) h->last_error = e.what();
+    return INQAMD_ERR_ARGUMENT;
+  }
+}
+
+void check_orbital(const handle &h, const void *p, const char *name) {
+  if (p == nullptr) throw error{ERR_ARGUMENT, std::string(name) + " is null"};
+  if (reinterpret_cast<uintptr_t>(p) % 16 != 0)
+    throw error{ERR_ARGUMENT, std::string(name) + " is not 16-byte aligned"};
+  (void)h;
+}
+
+void check_bands(const handle &h, int nst, int ld, bool needs_workspace) {
+  if (nst <= 0) throw error{ERR_ARGUMENT, "nst must be positive"};
+  if (ld < nst) throw error{ERR_ARGUMENT, "ld must be >= nst"};
+  if (needs_workspace && nst > h.max_bands)
+    throw error{ERR_ARGUMENT, "nst (" + std::to_string(nst) + ") exceeds max_bands (" +
+                                  std::to_string(h.max_bands) + ")"};
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+extern "C" {
+
+const char *inqamd_version(void) { return "inq_amd 0.1 (gfx950)"; }
+
+const char *inqamd_last_error(inqamd_hamiltonian_t h) {
+  if (!h) return g_create_error.c_str();
+  return h->last_error.c_str();
+}
+
+int inqamd_hamiltonian_create(const inqamd_system_desc *desc, void *stream,
+                              inqamd_hamiltonian_t *out) {
+  if (!desc || !out) {
+    g_create_error = "null argument";
+    return INQAMD_ERR_ARGUMENT;
+  }
+  *out = nullptr;
+  inqamd_hamiltonian *h = nullptr;
+  try {
+    h = new inqamd_hamiltonian();
+    create(*h, *desc, (hipStream_t)stream);
+  } catch (const inqamd::error &e) {
+    g_create_error = e.msg;
+    if (h) { release(*h); delete h; }
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_create_error = "host allocation failed";
+    if (h) { release(*h); delete h; }
+    return INQAMD_ERR_MEMORY;
+  }
+  g_create_error.clear();
+  *out = h;
+  return INQAMD_OK;
+}
+
+void inqamd_hamiltonian_destroy(inqamd_hamiltonian_t h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->stream);
+  release(*h);
+  delete h;
+}
+
+int inqamd_set_stream(inqamd_hamiltonian_t h, void *stream) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  h->stream = (hipStream_t)stream;
+  return INQAMD_OK;
+}
+
+int inqamd_set_kpoint(inqamd_hamiltonian_t h, const double k_cov[3]) {
+  if (!h || !k_cov) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    for (int d = 0; d < 3; d++) h->k_cov[d] = k_cov[d];
+    update_current_slot(*h);
+  });
+}
+
+int inqamd_set_uniform_vector_potential(inqamd_hamiltonian_t h, const double A_cov[3]) {
+  if (!h || !A_cov) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    for (int d = 0; d < 3; d++) h->A_cov[d] = A_cov[d];
+    update_current_slot(*h);
+  });
+}
+
+int inqamd_set_scalar_potential(inqamd_hamiltonian_t h, const double *vloc_device) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    INQAMD_REQUIRE(vloc_device != nullptr, "vloc is null");
+    INQAMD_HIP_CHECK(hipMemcpyAsync(h->V, vloc_device, h->npoint * sizeof(double),
+                                    hipMemcpyDeviceToDevice, h->stream));
+  });
+}
+
+int inqamd_hamiltonian_apply(inqamd_hamiltonian_t h, const void *psi, void *hpsi, int nst,
+                             int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_orbital(*h, hpsi, "hpsi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(psi != hpsi, "psi and hpsi must not alias");
+    hamiltonian_apply(*h, (const cplx *)psi, ld, (cplx *)hpsi, ld, nst, KA_CURRENT);
+  });
+}
+
+int inqamd_non_local(inqamd_hamiltonian_t h, const void *psi, void *vnlpsi, int nst, int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_orbital(*h, vnlpsi, "vnlpsi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(psi != vnlpsi, "psi and vnlpsi must not alias");
+    // vnlphi.fill(0) then project + apply (ks_hamiltonian.hpp:119-126)
+    INQAMD_HIP_CHECK(hipMemsetAsync(vnlpsi, 0, (size_t)h->npoint * ld * sizeof(cplx), h->stream));
+    projector_project_coefficients(*h, (const cplx *)psi, ld, nst, KA_CURRENT);
+    projector_apply_coefficients(*h, (cplx *)vnlpsi, ld, nst, KA_CURRENT);
+  });
+}
+
+int inqamd_transform_to_fourier(inqamd_hamiltonian_t h, const void *in, void *out, int nst,
+                                int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, in, "in");
+    check_orbital(*h, out, "out");
+    check_bands(*h, nst, ld, false);
+    fft3d(*h, (const cplx *)in, ld, (cplx *)out, ld, nst, -1, 1.0);
+  });
+}
+
+int inqamd_transform_to_real(inqamd_hamiltonian_t h, const void *in, void *out, int nst, int ld,
+                             int normalize) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, in, "in");
+    check_orbital(*h, out, "out");
+    check_bands(*h, nst, ld, false);
+    fft3d(*h, (const cplx *)in, ld, (cplx *)out, ld, nst, +1,
+          normalize ? 1.0 / (double)h->npoint : 1.0);
+  });
+}
+
+int inqamd_projector_project(inqamd_hamiltonian_t h, const void *psi, void *sphere, int nst,
+                             int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_orbital(*h, sphere, "sphere");
+    check_bands(*h, nst, ld, true);
+    projector_project_coefficients(*h, (const cplx *)psi, ld, nst, KA_CURRENT);
+    projector_expand(*h, (cplx *)sphere, nst);
+  });
+}
+
+int inqamd_projector_apply(inqamd_hamiltonian_t h, const void *sphere, void *hpsi, int nst,
+                           int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, sphere, "sphere");
+    check_orbital(*h, hpsi, "hpsi");
+    check_bands(*h, nst, ld, false);
+    projector_apply_sphere(*h, (const cplx *)sphere, (cplx *)hpsi, ld, nst, KA_CURRENT);
+  });
+}
+
+int inqamd_band_energies(inqamd_hamiltonian_t h, const void *hpsi, const void *psi, double *eps,
+                         int nst, int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, hpsi, "hpsi");
+    check_orbital(*h, psi, "psi");
+    INQAMD_REQUIRE(eps != nullptr, "eps is null");
+    check_bands(*h, nst, ld, true);
+    band_energies(*h, (const cplx *)hpsi, (const cplx *)psi, ld, nst, eps);
+  });
+}
+
+int inqamd_shift(inqamd_hamiltonian_t h, double re, double im, const void *x, void *y, int nst,
+                 int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, x, "x");
+    check_orbital(*h, y, "y");
+    check_bands(*h, nst, ld, false);
+    shift(*h, std::complex<double>(re, im), (const cplx *)x, ld, (cplx *)y, ld, nst);
+  });
+}
+
+int inqamd_exponential_in_place(inqamd_hamiltonian_t h, double zre, double zim, int order,
+                                void *psi, int nst, int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(order >= 1, "order must be >= 1");
+    exponential_in_place(*h, std::complex<double>(zre, zim), order, (cplx *)psi, ld, nst,
+                         KA_CURRENT);
+  });
+}
+
+int inqamd_exponential_2_for_1(inqamd_hamiltonian_t h, double z1re, double z1im, double z2re,
+                               double z2im, int order, void *phi, void *expphi, int nst, int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, phi, "phi");
+    check_orbital(*h, expphi, "expphi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(order >= 1, "order must be >= 1");
+    INQAMD_REQUIRE(phi != expphi, "phi and expphi must not alias");
+    exponential_2_for_1(*h, std::complex<double>(z1re, z1im), std::complex<double>(z2re, z2im),
+                        order, (cplx *)phi, (cplx *)expphi, ld, nst, KA_CURRENT);
+  });
+}
+
+int inqamd_etrs_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov[3],
+                     const double A_tdt_cov[3], int order, void *psi, int nst, int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(A_t_cov && A_tdt_cov, "vector potentials are required");
+    INQAMD_REQUIRE(order >= 1, "order must be >= 1");
+    double kt[3], ktdt[3];
+    for (int d = 0; d < 3; d++) {
+      kt[d] = h->k_cov[d] + A_t_cov[d];
+      ktdt[d] = h->k_cov[d] + A_tdt_cov[d];
+    }
+    set_slot(*h, KA_T, kt);
+    set_slot(*h, KA_TDT, ktdt);
+    const std::complex<double> z(0.0, dt / 2.0);  // etrs.hpp:36, :67
+    exponential_in_place(*h, z, order, (cplx *)psi, ld, nst, KA_T);
+    exponential_in_place(*h, z, order, (cplx *)psi, ld, nst, KA_TDT);
+    for (int d = 0; d < 3; d++) h->A_cov[d] = A_tdt_cov[d];
+    update_current_slot(*h);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+"""INQ-shaped host API over the C ABI, on torch ROCm tensors.
+
+Orbitals are complex128 device tensors laid out like INQ's orbital_set:
+shape [nx, ny, nz, nst] (or [N, nst]), band index contiguous.
+
+    ham = ks_hamiltonian(system, max_bands)      # ks_hamiltonian.hpp:79-89
+    hphi = ham(phi)                               # ks_hamiltonian.hpp:132-152
+    ham.uniform_vector_potential = A_cov          # ks_hamiltonian.hpp:216-221
+    vnl = ham.non_local(phi)                      # ks_hamiltonian.hpp:104-128
+    sphere = ham.projectors_all().project(phi)    # projector_all.hpp:98-216
+    ham.projectors_all().apply(sphere, vnl)       # projector_all.hpp:220-232
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _orbital_args(phi):
+    if not isinstance(phi, torch.Tensor):
+        raise TypeError("orbitals must be torch tensors")
+    if phi.dtype != torch.complex128:
+        raise TypeError("orbitals must be complex128")
+    if not phi.is_cuda:
+        raise ValueError("orbitals must live on the GPU")
+    if not phi.is_contiguous():
+        raise ValueError("orbitals must be contiguous ([points][bands], band fastest)")
+    nst = phi.shape[-1]
+    return ctypes.c_void_p(phi.data_ptr()), nst, nst
+
+
+class _Handle:
+    """Owns one inqamd_hamiltonian_t."""
+
+    def __init__(self, n, lattice, vloc, tables, max_bands):
+        L = _lib.lib()
+        self._lib = L
+        d = _lib.SystemDesc()
+        d.n = (ctypes.c_int * 3)(*[int(x) for x in n])
+        d.lattice = (ctypes.c_double * 9)(*np.ravel(np.asarray(lattice, float)))
+        v = np.ascontiguousarray(np.ravel(vloc), np.float64)
+        keep = [v]
+        d.vloc = v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        d.max_bands = int(max_bands)
+        if tables and tables.get("nproj", 0) > 0:
+            arrs = {
+                "points": (np.int32, ctypes.c_int), "positions": (np.float32, ctypes.c_float),
+                "matrices": (np.float64, ctypes.c_double), "coeff": (np.float64, ctypes.c_double),
+                "nlm": (np.int32, ctypes.c_int), "sphere_size": (np.int32, ctypes.c_int),
+            }
+            for name, (dt, ct) in arrs.items():
+                a = np.ascontiguousarray(tables[name], dt)
+                keep.append(a)
+                setattr(d, name, a.ctypes.data_as(ctypes.POINTER(ct)))
+            d.nproj = int(tables["nproj"])
+            d.max_sphere = int(tables["max_sphere"])
+            d.max_nlm = int(tables["max_nlm"])
+        h = ctypes.c_void_p()
+        rc = L.inqamd_hamiltonian_create(ctypes.byref(d), _stream_ptr(), ctypes.byref(h))
+        _lib.check(rc, None)
+        self.h = h
+        self.n = tuple(int(x) for x in n)
+        self.max_bands = int(max_bands)
+        del keep
+
+    def call(self, name, *args):
+        L = self._lib
+        _lib.check(L.inqamd_set_stream(self.h, _stream_ptr()), self.h)
+        _lib.check(getattr(L, name)(self.h, *args), self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.inqamd_hamiltonian_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class projector_all:
+    """projector_all view of a ks_hamiltonian (projector_all.hpp:23-421)."""
+
+    def __init__(self, ham):
+        self._ham = ham
+        t = ham.tables
+        self.nprojs = int(t.get("nproj", 0)) if t else 0
+        self.max_sphere_size = int(t.get("max_sphere", 0)) if t else 0
+        self.max_nlm = int(t.get("max_nlm", 0)) if t else 0
+
+    def project(self, phi):
+        p, nst, ld = _orbital_args(phi)
+        out = torch.zeros((self.nprojs, self.max_sphere_size, nst), dtype=torch.complex128,
+                          device=phi.device)
+        if self.nprojs:
+            self._ham._h.call("inqamd_projector_project", p, ctypes.c_void_p(out.data_ptr()), nst, ld)
+        return out
+
+    def apply(self, sphere, vnlphi):
+        p, nst, ld = _orbital_args(vnlphi)
+        if self.nprojs:
+            s = sphere.contiguous()
+            self._ham._h.call("inqamd_projector_apply", ctypes.c_void_p(s.data_ptr()), p, nst, ld)
+        return vnlphi
+
+
+class ks_hamiltonian:
+    """Frozen-potential Kohn-Sham Hamiltonian on one GPU."""
+
+    def __init__(self, system, max_bands, kpoint=(0.0, 0.0, 0.0)):
+        self.system = system
+        self.tables = system.tables
+        self.n = tuple(system.n)
+        self._h = _Handle(system.n, system.lattice, system.vloc, system.tables, max_bands)
+        self._A = np.zeros(3)
+        self._k = np.zeros(3)
+        self.kpoint = kpoint
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def max_bands(self):
+        return self._h.max_bands
+
+    @property
+    def uniform_vector_potential(self):
+        return self._A.copy()
+
+    @uniform_vector_potential.setter
+    def uniform_vector_potential(self, A_cov):
+        self._A = np.asarray(A_cov, float).reshape(3).copy()
+        self._h.call("inqamd_set_uniform_vector_potential", _lib.dvec(self._A))
+
+    @property
+    def kpoint(self):
+        return self._k.copy()
+
+    @kpoint.setter
+    def kpoint(self, k_cov):
+        self._k = np.asarray(k_cov, float).reshape(3).copy()
+        self._h.call("inqamd_set_kpoint", _lib.dvec(self._k))
+
+    def set_scalar_potential(self, vloc):
+        v = vloc.to(dtype=torch.float64).contiguous()
+        self._h.call("inqamd_set_scalar_potential", ctypes.c_void_p(v.data_ptr()))
+        self._vkeep = v
+
+    def __call__(self, phi, out=None):
+        p, nst, ld = _orbital_args(phi)
+        if out is None:
+            out = torch.empty_like(phi)
+        q, _, _ = _orbital_args(out)
+        self._h.call("inqamd_hamiltonian_apply", p, q, nst, ld)
+        return out
+
+    def non_local(self, phi):
+        p, nst, ld = _orbital_args(phi)
+        out = torch.empty_like(phi)
+        self._h.call("inqamd_non_local", p, ctypes.c_void_p(out.data_ptr()), nst, ld)
+        return out
+
+    def projectors_all(self):
+        return projector_all(self)
+
+    def band_energies(self, hphi, phi):
+        """energy::calculate eigenvalues (energy.hpp:74-75)."""
+        p, nst, ld = _orbital_args(phi)
+        q, _, _ = _orbital_args(hphi)
+        eps = torch.empty(nst, dtype=torch.float64, device=phi.device)
+        self._h.call("inqamd_band_energies", q, p, ctypes.c_void_p(eps.data_ptr()), nst, ld)
+        return eps
+
+    def close(self):
+        self._h.close()

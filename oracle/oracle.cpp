@@ -1,0 +1,558 @@
+/*
+ * oracle.cpp -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * A plain, serial-per-band FP64 restatement of INQ's CPU branches for the
+ * Hpsi / Taylor / ETRS path.  Every function cites the reference lines it
+ * follows.  Its FFT is a recursive decimation-in-time mixed-radix transform
+ * with an O(p^2) combination step for any prime factor p, and twiddles
+ * evaluated in long double -- deliberately a different algorithm from the
+ * HIP library's Stockham passes so that the two do not share mistakes.
+ */
+#include "oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+#include <algorithm>
+#include <memory>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using cplx = std::complex<double>;
+
+/* ---------------------------------------------------------------------------
+ * Cell metric (src/systems/cell.hpp:36-53, 146-224)
+ * ------------------------------------------------------------------------ */
+struct vec3 {
+  double v[3];
+  double &operator[](int i) { return v[i]; }
+  double operator[](int i) const { return v[i]; }
+};
+
+static vec3 operator+(vec3 a, vec3 b) { return {{a[0] + b[0], a[1] + b[1], a[2] + b[2]}}; }
+static vec3 operator-(vec3 a, vec3 b) { return {{a[0] - b[0], a[1] - b[1], a[2] - b[2]}}; }
+static vec3 operator*(double s, vec3 a) { return {{s * a[0], s * a[1], s * a[2]}}; }
+static double dot(vec3 a, vec3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static vec3 cross(vec3 a, vec3 b) {
+  return {{a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}};
+}
+
+struct cell {
+  vec3 lat[3];
+  vec3 rlat[3];
+  double volume;
+
+  explicit cell(const double *lattice) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) lat[i][j] = lattice[3 * i + j];
+    volume = dot(lat[0], cross(lat[1], lat[2]));
+    rlat[0] = (2.0 * M_PI / volume) * cross(lat[1], lat[2]);
+    rlat[1] = (2.0 * M_PI / volume) * cross(lat[2], lat[0]);
+    rlat[2] = (2.0 * M_PI / volume) * cross(lat[0], lat[1]);
+  }
+  /* cell.hpp:215-217 */
+  vec3 contra_to_cart(vec3 v) const { return (v[0] * lat[0] + v[1] * lat[1]) + v[2] * lat[2]; }
+  /* cell.hpp:220-222 */
+  vec3 cov_to_cart(vec3 v) const {
+    vec3 s = (v[0] * rlat[0] + v[1] * rlat[1]) + v[2] * rlat[2];
+    return {{s[0] / (2.0 * M_PI), s[1] / (2.0 * M_PI), s[2] / (2.0 * M_PI)}};
+  }
+  /* cell.hpp:205-207 */
+  vec3 cart_to_contra(vec3 w) const {
+    return {{dot(rlat[0], w) / (2.0 * M_PI), dot(rlat[1], w) / (2.0 * M_PI),
+             dot(rlat[2], w) / (2.0 * M_PI)}};
+  }
+  /* cell.hpp:185-187 */
+  vec3 cart_to_cov(vec3 w) const { return {{dot(lat[0], w), dot(lat[1], w), dot(lat[2], w)}}; }
+  double norm_cov(vec3 v) const { vec3 c = cov_to_cart(v); return dot(c, c); }
+  vec3 cov_to_contra(vec3 v) const { return cart_to_contra(cov_to_cart(v)); }
+};
+
+/* grid.hpp:111-117 */
+static int symmetric(int i, int n) { return i >= (n + 1) / 2 ? i - n : i; }
+
+/* ---------------------------------------------------------------------------
+ * 1-D FFT: recursive DIT, smallest-prime-first, O(p^2) combination.
+ * ------------------------------------------------------------------------ */
+struct twiddles {
+  int n;
+  std::vector<cplx> w; /* w[k] = exp(-2 pi i k / n) */
+};
+
+static const twiddles &get_twiddles(int n) {
+  static std::mutex mtx;
+  static std::map<int, std::unique_ptr<twiddles>> cache;
+  std::lock_guard<std::mutex> lock(mtx);
+  auto it = cache.find(n);
+  if (it != cache.end()) return *it->second;
+  auto tw = std::make_unique<twiddles>();
+  tw->n = n;
+  tw->w.resize(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < n; k++) {
+    long double ang = -2.0L * pi * (long double)k / (long double)n;
+    tw->w[k] = cplx((double)cosl(ang), (double)sinl(ang));
+  }
+  auto &ref = *tw;
+  cache[n] = std::move(tw);
+  return ref;
+}
+
+static int smallest_factor(int n) {
+  for (int p : {2, 3, 5}) if (n % p == 0) return p;
+  for (int p = 7; p * p <= n; p += 2) if (n % p == 0) return p;
+  return n;
+}
+
+/* DFT of length nsub of in[0], in[istride], ... into out[0..nsub).
+ * tw_step = N / nsub so that exp(-2 pi i j / nsub) = w[j * tw_step]. */
+static void fft_rec(const cplx *in, long istride, cplx *out, int nsub, int tw_step,
+                    const std::vector<cplx> &w, int N, bool inverse, cplx *scratch) {
+  if (nsub == 1) { out[0] = in[0]; return; }
+  int p = smallest_factor(nsub);
+  int m = nsub / p;
+  for (int q = 0; q < p; q++)
+    fft_rec(in + q * istride, istride * p, out + q * m, m, tw_step * p, w, N, inverse, scratch);
+  auto W = [&](long idx) { cplx c = w[idx % N]; return inverse ? std::conj(c) : c; };
+  for (int k = 0; k < m; k++) {
+    for (int q = 0; q < p; q++) scratch[q] = out[q * m + k] * W((long)q * k * tw_step);
+    for (int s = 0; s < p; s++) {
+      cplx acc = 0.0;
+      for (int q = 0; q < p; q++) acc += scratch[q] * W((long)((q * s) % p) * m * tw_step);
+      scratch[p + s] = acc;
+    }
+    for (int s = 0; s < p; s++) out[k + s * m] = scratch[p + s];
+  }
+}
+
+static void fft1d(int n, const cplx *in, cplx *out, int sign, cplx *scratch) {
+  const twiddles &tw = get_twiddles(n);
+  fft_rec(in, 1, out, n, 1, tw.w, n, sign > 0, scratch);
+}
+
+static int max_prime(int n) {
+  int best = 1;
+  while (n > 1) { int p = smallest_factor(n); best = std::max(best, p); n /= p; }
+  return best;
+}
+
+/* Batched 3-D DFT over [x][y][z][b] (src/operations/transform.hpp:132, 265). */
+static void fft3d(const int n[3], int nst, int ld, const cplx *in, cplx *out, int sign,
+                  bool normalize) {
+  const long N = (long)n[0] * n[1] * n[2];
+  /* dense working copy [point][band] */
+  std::vector<cplx> work((size_t)N * nst);
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) work[p * nst + b] = in[p * ld + b];
+  const long stride[3] = {(long)n[1] * n[2], (long)n[2], 1};
+  for (int axis = 0; axis < 3; axis++) {
+    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    const int len = n[axis];
+    const long nlines = (long)n[a1] * n[a2] * nst;
+    int pmax = max_prime(len);
+#pragma omp parallel
+    {
+      std::vector<cplx> line(len), res(len), scratch(2 * pmax + 2);
+#pragma omp for schedule(static)
+      for (long l = 0; l < nlines; l++) {
+        int b = (int)(l % nst);
+        long rest = l / nst;
+        int i1 = (int)(rest % n[a1]);
+        int i2 = (int)(rest / n[a1]);
+        long base = i1 * stride[a1] + i2 * stride[a2];
+        for (int i = 0; i < len; i++) line[i] = work[(base + i * stride[axis]) * nst + b];
+        fft1d(len, line.data(), res.data(), sign, scratch.data());
+        for (int i = 0; i < len; i++) work[(base + i * stride[axis]) * nst + b] = res[i];
+      }
+    }
+  }
+  const double factor = 1.0 / (double)N;
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) {
+      cplx v = work[p * nst + b];
+      /* transform.hpp:327-330: ar[ip] = factor*ar[ip] */
+      out[p * ld + b] = normalize ? factor * v : v;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Hamiltonian pieces
+ * ------------------------------------------------------------------------ */
+static long npoints(const orc_system *s) { return (long)s->n[0] * s->n[1] * s->n[2]; }
+
+/* projector_all::project (src/hamiltonian/projector_all.hpp:98-216), CPU
+ * branches (:117-125, :172-180). */
+static void project(const orc_system *s, const double kA_cov[3], int nst, int ld,
+                    const cplx *psi, cplx *sphere) {
+  cell c(s->lattice);
+  const long maxK = s->max_sphere;
+  const double dV = c.volume / (double)npoints(s); /* real_space.hpp:150-152 */
+#pragma omp parallel for schedule(dynamic)
+  for (int ip = 0; ip < s->nproj; ip++) {
+    cplx *sph = sphere + (long)ip * maxK * nst;
+    /* gather with phase polar(1, dot(k, pos)) (:104-115) */
+    for (long i = 0; i < maxK; i++) {
+      const int *poi = s->points + ((long)ip * maxK + i) * 3;
+      if (poi[0] >= 0) {
+        const float *pos = s->positions + ((long)ip * maxK + i) * 3;
+        double arg = kA_cov[0] * (double)pos[0] + kA_cov[1] * (double)pos[1] +
+                     kA_cov[2] * (double)pos[2];
+        cplx phase = std::polar(1.0, arg);
+        long gp = ((long)poi[0] * s->n[1] + poi[1]) * s->n[2] + poi[2];
+        for (int b = 0; b < nst; b++) sph[i * nst + b] = phase * psi[gp * ld + b];
+      } else {
+        for (int b = 0; b < nst; b++) sph[i * nst + b] = 0.0;
+      }
+    }
+    if (s->sphere_size[ip] == 0 || s->nlm[ip] == 0) continue; /* locally_empty_ */
+    const double *M = s->matrices + (long)ip * s->max_nlm * maxK;
+    /* GEMM1: P[lm][b] = dV sum_i M[lm][i] sph[i][b] (:117-125) */
+    std::vector<cplx> P((size_t)s->max_nlm * nst, 0.0);
+    for (int lm = 0; lm < s->max_nlm; lm++)
+      for (long i = 0; i < maxK; i++) {
+        double m = M[lm * maxK + i];
+        for (int b = 0; b < nst; b++) P[lm * nst + b] += m * sph[i * nst + b];
+      }
+    for (int lm = 0; lm < s->max_nlm; lm++)
+      for (int b = 0; b < nst; b++) P[lm * nst + b] *= dV;
+    /* scale by the KB coefficients (:158-165) */
+    for (int lm = 0; lm < s->max_nlm; lm++)
+      for (int b = 0; b < nst; b++) P[lm * nst + b] *= s->coeff[(long)ip * s->max_nlm + lm];
+    /* GEMM2: sph[i][b] = sum_lm M[lm][i] P[lm][b] (:172-180) */
+    for (long i = 0; i < maxK; i++)
+      for (int b = 0; b < nst; b++) {
+        cplx acc = 0.0;
+        for (int lm = 0; lm < s->max_nlm; lm++) acc += M[lm * maxK + i] * P[lm * nst + b];
+        sph[i * nst + b] = acc;
+      }
+  }
+}
+
+/* projector_all::apply (src/hamiltonian/projector_all.hpp:220-232), with the
+ * scatter in fixed (projector, point) order instead of atomics. */
+static void apply(const orc_system *s, const double kA_cov[3], int nst, int ld,
+                  const cplx *sphere, cplx *hpsi) {
+  const long maxK = s->max_sphere;
+  for (int ip = 0; ip < s->nproj; ip++) {
+    if (s->sphere_size[ip] == 0 || s->nlm[ip] == 0) continue;
+    const cplx *sph = sphere + (long)ip * maxK * nst;
+    for (long i = 0; i < maxK; i++) {
+      const int *poi = s->points + ((long)ip * maxK + i) * 3;
+      if (poi[0] < 0) continue;
+      const float *pos = s->positions + ((long)ip * maxK + i) * 3;
+      double arg = kA_cov[0] * (double)pos[0] + kA_cov[1] * (double)pos[1] +
+                   kA_cov[2] * (double)pos[2];
+      cplx phase = std::polar(1.0, -arg);
+      long gp = ((long)poi[0] * s->n[1] + poi[1]) * s->n[2] + poi[2];
+      for (int b = 0; b < nst; b++) hpsi[gp * ld + b] += phase * sph[i * nst + b];
+    }
+  }
+}
+
+/* laplacian(ff, -0.5, -2 (k+A)^contra) G-space branch
+ * (src/operations/laplacian.hpp:70-79), in place on a dense [N][nst] array. */
+static void laplacian_kA(const orc_system *s, const double kA_cov[3], int nst, cplx *ff) {
+  cell c(s->lattice);
+  vec3 kA{{kA_cov[0], kA_cov[1], kA_cov[2]}};
+  vec3 gc = c.cov_to_contra(kA);
+  gc = -2.0 * gc;
+  const double factor = -0.5;
+#pragma omp parallel for schedule(static)
+  for (int ix = 0; ix < s->n[0]; ix++)
+    for (int iy = 0; iy < s->n[1]; iy++)
+      for (int iz = 0; iz < s->n[2]; iz++) {
+        /* fourier_space.hpp:66-73: G covariant = 2 pi * symmetric index */
+        vec3 g{{2.0 * M_PI * symmetric(ix, s->n[0]), 2.0 * M_PI * symmetric(iy, s->n[1]),
+                2.0 * M_PI * symmetric(iz, s->n[2])}};
+        double g2 = c.norm_cov(g);
+        double lapl = factor * (-g2 + (gc[0] * g[0] + gc[1] * g[1] + gc[2] * g[2]));
+        long p = ((long)ix * s->n[1] + iy) * s->n[2] + iz;
+        for (int b = 0; b < nst; b++) ff[p * nst + b] = lapl * ff[p * nst + b];
+      }
+}
+
+static void hamiltonian(const orc_system *s, const double kA_cov[3], int nst, int ld,
+                        const cplx *psi, cplx *hpsi) {
+  cell c(s->lattice);
+  const long N = npoints(s);
+  const long maxK = s->max_sphere;
+  /* ks_hamiltonian.hpp:136 */
+  std::vector<cplx> sphere((size_t)std::max<long>(1, s->nproj * maxK * nst));
+  if (s->nproj > 0) project(s, kA_cov, nst, ld, psi, sphere.data());
+  /* :138-144 */
+  std::vector<cplx> fs((size_t)N * nst);
+  fft3d(s->n, nst, ld, psi, fs.data(), -1, false);
+  laplacian_kA(s, kA_cov, nst, fs.data());
+  std::vector<cplx> rs((size_t)N * nst);
+  fft3d(s->n, nst, nst, fs.data(), rs.data(), +1, true);
+  /* :146 scalar_potential_add with shift 0.5 |k+A|^2 (scalar_potential.hpp:29-34) */
+  vec3 kA{{kA_cov[0], kA_cov[1], kA_cov[2]}};
+  const double shift = 0.5 * c.norm_cov(kA);
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++)
+      hpsi[p * ld + b] = rs[p * nst + b] + (s->vloc[p] + shift) * psi[p * ld + b];
+  /* :149 */
+  if (s->nproj > 0) apply(s, kA_cov, nst, ld, sphere.data(), hpsi);
+}
+
+/* operations::shift (src/operations/shift.hpp:33-44): phi += factor*shift */
+static void shift_add(long N, int nst, int ld, cplx factor, const cplx *x, cplx *y) {
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) y[p * ld + b] += factor * x[p * ld + b];
+}
+
+/* exponential.hpp:21-34 */
+static void exponential_in_place(const orc_system *s, const double kA[3], cplx z, int order,
+                                 int nst, int ld, cplx *phi) {
+  const long N = npoints(s);
+  std::vector<cplx> hn((size_t)N * ld), tmp((size_t)N * ld);
+  hamiltonian(s, kA, nst, ld, phi, hn.data());
+  cplx coeff = 1.0;
+  for (int iter = 1; iter <= order; iter++) {
+    if (iter > 1) { hamiltonian(s, kA, nst, ld, hn.data(), tmp.data()); std::swap(hn, tmp); }
+    coeff *= z / (double)iter;
+    shift_add(N, nst, ld, coeff, hn.data(), phi);
+  }
+}
+
+} // namespace
+
+/* ===========================================================================
+ * C ABI
+ * ======================================================================== */
+extern "C" {
+
+void orc_set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+int orc_fft3d(const int n[3], int nst, int ld, const double *in, double *out, int sign,
+              int normalize) {
+  if (!n || n[0] <= 0 || n[1] <= 0 || n[2] <= 0 || nst <= 0 || ld < nst) return 1;
+  fft3d(n, nst, ld, reinterpret_cast<const cplx *>(in), reinterpret_cast<cplx *>(out), sign,
+        normalize != 0);
+  return 0;
+}
+
+int orc_fft1d(int n, const double *in, double *out, int sign) {
+  if (n <= 0) return 1;
+  std::vector<cplx> scratch(2 * max_prime(n) + 2), src(n);
+  for (int i = 0; i < n; i++) src[i] = cplx(in[2 * i], in[2 * i + 1]);
+  fft1d(n, src.data(), reinterpret_cast<cplx *>(out), sign, scratch.data());
+  return 0;
+}
+
+int orc_containing_cube(const double lattice[9], const int n[3], const double pos_[3],
+                        double radius, int lo[3], int hi[3]) {
+  cell c(lattice);
+  vec3 pos{{pos_[0], pos_[1], pos_[2]}};
+  for (int idir = 0; idir < 3; idir++) {
+    vec3 rec = c.rlat[idir];
+    double len = std::sqrt(dot(rec, rec));
+    vec3 lointer = pos - (radius / len) * rec;
+    vec3 hiinter = pos + (radius / len) * rec;
+    double dlo = c.cart_to_contra(lointer)[idir];
+    double dhi = c.cart_to_contra(hiinter)[idir];
+    double spacing = 1.0 / n[idir]; /* grid.hpp:37 */
+    lo[idir] = (int)lround(floor(dlo / spacing));
+    hi[idir] = (int)lround(ceil(dhi / spacing)) + 1;
+    int rb = -n[idir] / 2, re = n[idir] / 2 + n[idir] % 2; /* grid.hpp:142-148 */
+    lo[idir] = std::min(std::max(lo[idir], rb), re);
+    hi[idir] = std::min(std::max(hi[idir], rb), re);
+  }
+  return 0;
+}
+
+int orc_sphere_build(const double lattice[9], const int n[3], const double center[3],
+                     double radius, int capacity, int *coords, float *distance, float *relpos,
+                     int *count) {
+  cell c(lattice);
+  /* periodic_replicas (src/ionic/periodic_replicas.hpp:27-55) with
+   * position_in_cell (src/systems/cell.hpp:238-250) */
+  vec3 pos{{center[0], center[1], center[2]}};
+  vec3 crys = c.cart_to_contra(pos);
+  for (int d = 0; d < 3; d++) {
+    crys[d] -= floor(crys[d]);
+    if (crys[d] >= 0.5) crys[d] -= 1.0;
+  }
+  pos = c.contra_to_cart(crys);
+  double rl[3];
+  for (int d = 0; d < 3; d++) rl[d] = std::sqrt(dot(c.lat[d], c.lat[d]));
+  double range = std::sqrt(rl[0] * rl[0] + rl[1] * rl[1] + rl[2] * rl[2]); /* grid.hpp:62-64 */
+  std::vector<vec3> reps;
+  reps.push_back(pos);
+  int nm[3];
+  for (int d = 0; d < 3; d++) nm[d] = 1 + (int)lround(range / std::sqrt(dot(c.lat[d], c.lat[d])));
+  for (int ix = -nm[0]; ix <= nm[0]; ix++)
+    for (int iy = -nm[1]; iy <= nm[1]; iy++)
+      for (int iz = -nm[2]; iz <= nm[2]; iz++) {
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        vec3 r = ((pos + (double)ix * c.lat[0]) + (double)iy * c.lat[1]) + (double)iz * c.lat[2];
+        reps.push_back(r);
+      }
+  /* spherical_grid::initialize (src/basis/spherical_grid.hpp:58-158) */
+  int cnt = 0;
+  const double r2max = radius * radius;
+  for (const vec3 &re : reps) {
+    int lo[3], hi[3];
+    double rev[3] = {re[0], re[1], re[2]};
+    orc_containing_cube(lattice, n, rev, radius, lo, hi);
+    int d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
+    if (d0 * d1 * d2 == 0) continue;
+    for (int ix = 0; ix < d0; ix++)
+      for (int iy = 0; iy < d1; iy++)
+        for (int iz = 0; iz < d2; iz++) {
+          int ii[3] = {lo[0] + ix, lo[1] + iy, lo[2] + iz};
+          for (int d = 0; d < 3; d++) if (ii[d] < 0) ii[d] += n[d]; /* grid.hpp:123-128 */
+          /* real_space.hpp:76-79: symmetric index times 1/n, then to cartesian */
+          vec3 rc{{symmetric(ii[0], n[0]) * (1.0 / n[0]), symmetric(ii[1], n[1]) * (1.0 / n[1]),
+                   symmetric(ii[2], n[2]) * (1.0 / n[2])}};
+          vec3 rpoint = c.contra_to_cart(rc);
+          vec3 dr = rpoint - re;
+          double n2 = dot(dr, dr);
+          if (n2 > r2max) continue;
+          if (cnt < capacity) {
+            coords[3 * cnt + 0] = ii[0];
+            coords[3 * cnt + 1] = ii[1];
+            coords[3 * cnt + 2] = ii[2];
+            distance[cnt] = (float)std::sqrt(n2);
+            vec3 rp = c.cart_to_contra(dr);
+            relpos[3 * cnt + 0] = (float)rp[0];
+            relpos[3 * cnt + 1] = (float)rp[1];
+            relpos[3 * cnt + 2] = (float)rp[2];
+          }
+          cnt++;
+        }
+  }
+  *count = cnt;
+  return cnt > capacity ? 2 : 0;
+}
+
+int orc_gaussian_projector_matrix(const double lattice[9], int npoint, const float *distance,
+                                  const float *relpos, double sigma, int nlm, double *matrix,
+                                  int ld) {
+  if (nlm != 1 && nlm != 4) return 1;
+  cell c(lattice);
+  const double y00 = 1.0 / std::sqrt(4.0 * M_PI);
+  const double y1 = std::sqrt(3.0 / (4.0 * M_PI));
+  for (int i = 0; i < npoint; i++) {
+    double d = (double)distance[i];
+    double f = std::exp(-d * d / (sigma * sigma));
+    vec3 rc{{(double)relpos[3 * i], (double)relpos[3 * i + 1], (double)relpos[3 * i + 2]}};
+    vec3 x = c.contra_to_cart(rc);
+    double r = std::sqrt(dot(x, x));
+    matrix[0 * (long)ld + i] = f * y00;
+    if (nlm == 4) {
+      double inv = r > 0.0 ? 1.0 / r : 0.0;
+      matrix[1 * (long)ld + i] = f * y1 * x[1] * inv; /* m = -1 : y */
+      matrix[2 * (long)ld + i] = f * y1 * x[2] * inv; /* m =  0 : z */
+      matrix[3 * (long)ld + i] = f * y1 * x[0] * inv; /* m = +1 : x */
+    }
+  }
+  return 0;
+}
+
+int orc_hamiltonian_apply(const orc_system *sys, const double kA_cov[3], int nst, int ld,
+                          const double *psi, double *hpsi) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  hamiltonian(sys, kA_cov, nst, ld, reinterpret_cast<const cplx *>(psi),
+              reinterpret_cast<cplx *>(hpsi));
+  return 0;
+}
+
+int orc_projector_project(const orc_system *sys, const double kA_cov[3], int nst, int ld,
+                          const double *psi, double *sphere) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  project(sys, kA_cov, nst, ld, reinterpret_cast<const cplx *>(psi),
+          reinterpret_cast<cplx *>(sphere));
+  return 0;
+}
+
+int orc_projector_apply(const orc_system *sys, const double kA_cov[3], int nst, int ld,
+                        const double *sphere, double *hpsi) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  apply(sys, kA_cov, nst, ld, reinterpret_cast<const cplx *>(sphere),
+        reinterpret_cast<cplx *>(hpsi));
+  return 0;
+}
+
+int orc_band_energies(int npoint, int nst, int ld, const double *hpsi_, const double *psi_,
+                      double *eps) {
+  const cplx *hpsi = reinterpret_cast<const cplx *>(hpsi_);
+  const cplx *psi = reinterpret_cast<const cplx *>(psi_);
+  for (int b = 0; b < nst; b++) {
+    cplx value = 0.0;
+    cplx norm = 0.0;
+    for (long p = 0; p < npoint; p++) {
+      /* overlap_diagonal.hpp:124-126: {conj(mat1)*mat2, conj(mat2)*mat2} */
+      value += std::conj(hpsi[p * ld + b]) * psi[p * ld + b];
+      norm += std::conj(psi[p * ld + b]) * psi[p * ld + b];
+    }
+    eps[b] = std::real(value / norm); /* :167 with real_part */
+  }
+  return 0;
+}
+
+int orc_exponential_in_place(const orc_system *sys, const double kA_cov[3], double zre,
+                             double zim, int order, int nst, int ld, double *psi) {
+  if (!sys || nst <= 0 || ld < nst || order < 0) return 1;
+  exponential_in_place(sys, kA_cov, cplx(zre, zim), order, nst, ld,
+                       reinterpret_cast<cplx *>(psi));
+  return 0;
+}
+
+int orc_exponential_2_for_1(const orc_system *sys, const double kA_cov[3], double z1re,
+                            double z1im, double z2re, double z2im, int order, int nst, int ld,
+                            double *phi_, double *expphi_) {
+  if (!sys || nst <= 0 || ld < nst || order < 0) return 1;
+  const long N = npoints(sys);
+  cplx *phi = reinterpret_cast<cplx *>(phi_);
+  cplx *expphi = reinterpret_cast<cplx *>(expphi_);
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) expphi[p * ld + b] = phi[p * ld + b];
+  std::vector<cplx> hn((size_t)N * ld), tmp((size_t)N * ld);
+  hamiltonian(sys, kA_cov, nst, ld, phi, hn.data());
+  cplx c1 = 1.0, c2 = 1.0;
+  for (int iter = 1; iter <= order; iter++) {
+    if (iter > 1) { hamiltonian(sys, kA_cov, nst, ld, hn.data(), tmp.data()); std::swap(hn, tmp); }
+    c1 *= cplx(z1re, z1im) / (double)iter;
+    c2 *= cplx(z2re, z2im) / (double)iter;
+    shift_add(N, nst, ld, c1, hn.data(), expphi);
+    shift_add(N, nst, ld, c2, hn.data(), phi);
+  }
+  return 0;
+}
+
+int orc_etrs_step(const orc_system *sys, const double kA_t[3], const double kA_tdt[3], double dt,
+                  int order, int nst, int ld, double *psi_) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  cplx *psi = reinterpret_cast<cplx *>(psi_);
+  /* etrs.hpp:36: halfstep = T(i dt/2, H(t)) psi, (full step only feeds the
+   * density, which a frozen potential ignores); etrs.hpp:57-67: restore the
+   * half step and apply T(i dt/2, H(t+dt)). */
+  exponential_in_place(sys, kA_t, cplx(0.0, dt / 2.0), order, nst, ld, psi);
+  exponential_in_place(sys, kA_tdt, cplx(0.0, dt / 2.0), order, nst, ld, psi);
+  return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,124 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of INQ's CPU path for the plane-wave Hpsi and the
+ * real-time Taylor/ETRS step.  It is the checker the HIP library is compared
+ * against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it; the product library never links or calls it.
+ *
+ * Conventions (all follow INQ):
+ *   - orbitals are complex128 (interleaved re/im doubles) in the layout
+ *     [x][y][z][band] with the band index contiguous, leading dimension `ld`
+ *     (src/basis/field_set.hpp:51, :188-194);
+ *   - to_fourier is the unnormalised forward DFT (sign -1), to_real is the
+ *     backward DFT (sign +1) followed by x 1/N (src/operations/transform.hpp:
+ *     132, 265, 325-331);
+ *   - G and r indices use to_symmetric_range (src/basis/grid.hpp:111-117);
+ *   - vectors tagged _cov are covariant (src/systems/cell.hpp:146-224).
+ *
+ * Parity pins (each function's checks live in tests/):
+ *   orc_fft3d           numpy.fft golden vectors, brute-force DFT, round trip,
+ *                       Gaussian closed form (transform.hpp:490-545)
+ *   orc_containing_cube reference lo/hi known answers (containing_cube.hpp:77-117)
+ *   orc_sphere_build    reference sphere counts 257/13758 and volume ratios
+ *                       (spherical_grid.hpp:272-396)
+ *   orc_hamiltonian_apply
+ *                       reference harmonic-oscillator answer 0.0051420503
+ *                       (ks_hamiltonian.hpp:356-397), plane-wave eigenvalues
+ *                       with k+A, laplacian plane waves (laplacian.hpp:104-297),
+ *                       dense brute-force V_nl, Hermiticity
+ *   orc_band_energies   plane-wave eigenvalues, Rayleigh quotient closed form
+ *   orc_exponential_*   diagonal closed forms in the plane-wave basis
+ *                       (exponential.hpp:94-233 pattern), scipy expm
+ *   orc_etrs_step       composition of two Taylor-4 half steps; norm drift
+ */
+#ifndef INQ_AMD_ORACLE_H
+#define INQ_AMD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A frozen-potential Kohn-Sham system as INQ's ks_hamiltonian holds it. */
+typedef struct {
+  int n[3];               /* grid sizes */
+  double lattice[9];      /* a0, a1, a2 (cartesian, bohr), row-major */
+  const double *vloc;     /* [N] local potential, Ha */
+  int nproj;              /* projector_all::nprojs_ */
+  int max_sphere;         /* projector_all::max_sphere_size_ */
+  int max_nlm;            /* projector_all::max_nlm_ */
+  const int *points;      /* [nproj][max_sphere][3], -1 padded */
+  const float *positions; /* [nproj][max_sphere][3] contravariant, FP32 */
+  const double *matrices; /* [nproj][max_nlm][max_sphere] */
+  const double *coeff;    /* [nproj][max_nlm] */
+  const int *nlm;         /* [nproj] */
+  const int *sphere_size; /* [nproj] */
+} orc_system;
+
+/* Batched 3-D DFT over the [x][y][z][band] layout (in and out may alias).
+ * sign = -1 forward, +1 backward; normalize multiplies by 1/N afterwards. */
+int orc_fft3d(const int n[3], int nst, int ld, const double *in, double *out,
+              int sign, int normalize);
+
+/* Plain 1-D DFT of one contiguous complex vector (used by tests). */
+int orc_fft1d(int n, const double *in, double *out, int sign);
+
+/* containing_cube (src/basis/containing_cube.hpp:20-45). */
+int orc_containing_cube(const double lattice[9], const int n[3],
+                        const double pos[3], double radius, int lo[3], int hi[3]);
+
+/* spherical_grid::initialize (src/basis/spherical_grid.hpp:58-158).  Writes at
+ * most `capacity` points; returns the number of points in *count (which may
+ * exceed capacity, then call again with a larger buffer). */
+int orc_sphere_build(const double lattice[9], const int n[3], const double center[3],
+                     double radius, int capacity, int *coords, float *distance,
+                     float *relpos, int *count);
+
+/* Gaussian-projector table M[lm][i] = exp(-d^2/sigma^2) Y_lm(r_i) for l=0,1
+ * (real harmonics ordered y, z, x for l=1), evaluated from the FP32 sphere
+ * geometry as projector::build does (src/hamiltonian/projector.hpp:44-50). */
+int orc_gaussian_projector_matrix(const double lattice[9], int npoint,
+                                  const float *distance, const float *relpos,
+                                  double sigma, int nlm, double *matrix, int ld);
+
+/* ks_hamiltonian::operator()(orbital_set<real_space, complex>)
+ * (src/hamiltonian/ks_hamiltonian.hpp:132-152). kA_cov = k + A (covariant). */
+int orc_hamiltonian_apply(const orc_system *sys, const double kA_cov[3], int nst,
+                          int ld, const double *psi, double *hpsi);
+
+/* projector_all::project / apply (src/hamiltonian/projector_all.hpp:98-232).
+ * sphere is [nproj][max_sphere][nst] complex. */
+int orc_projector_project(const orc_system *sys, const double kA_cov[3], int nst,
+                          int ld, const double *psi, double *sphere);
+int orc_projector_apply(const orc_system *sys, const double kA_cov[3], int nst,
+                        int ld, const double *sphere, double *hpsi);
+
+/* energy::calculate eigenvalue part: eps_b = Re<Hpsi_b|psi_b>/<psi_b|psi_b>
+ * (src/hamiltonian/energy.hpp:74-75, overlap_diagonal.hpp:144-171). */
+int orc_band_energies(int npoint, int nst, int ld, const double *hpsi,
+                      const double *psi, double *eps);
+
+/* operations::exponential_in_place (src/operations/exponential.hpp:21-34). */
+int orc_exponential_in_place(const orc_system *sys, const double kA_cov[3],
+                             double zre, double zim, int order, int nst, int ld,
+                             double *psi);
+
+/* operations::exponential_2_for_1 (src/operations/exponential.hpp:50-69):
+ * phi <- T(z2) phi, expphi <- T(z1) phi (old phi). */
+int orc_exponential_2_for_1(const orc_system *sys, const double kA_cov[3],
+                            double z1re, double z1im, double z2re, double z2im,
+                            int order, int nst, int ld, double *phi, double *expphi);
+
+/* real_time::etrs with a frozen potential (src/real_time/etrs.hpp:22-82):
+ * psi(t+dt) = T4(i dt/2, H(t+dt)) T4(i dt/2, H(t)) psi(t). */
+int orc_etrs_step(const orc_system *sys, const double kA_t[3], const double kA_tdt[3],
+                  double dt, int order, int nst, int ld, double *psi);
+
+/* Set the number of OpenMP threads the oracle uses (<=0: library default). */
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,271 @@
+"""ctypes wrapper of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
+import this module.  It loads ``oracle/_build/liboracle.so`` (built by
+``oracle/Makefile``) and exposes numpy-level helpers.  Arrays are numpy
+complex128 in INQ's [x][y][z][band] layout.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_c_flt_p = ctypes.POINTER(ctypes.c_float)
+
+
+class OrcSystem(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("lattice", ctypes.c_double * 9),
+        ("vloc", _c_dbl_p),
+        ("nproj", ctypes.c_int),
+        ("max_sphere", ctypes.c_int),
+        ("max_nlm", ctypes.c_int),
+        ("points", _c_int_p),
+        ("positions", _c_flt_p),
+        ("matrices", _c_dbl_p),
+        ("coeff", _c_dbl_p),
+        ("nlm", _c_int_p),
+        ("sphere_size", _c_int_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {_LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_fft3d.argtypes = [_c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.orc_fft1d.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.orc_containing_cube.argtypes = [_c_dbl_p, _c_int_p, _c_dbl_p, ctypes.c_double,
+                                          _c_int_p, _c_int_p]
+        L.orc_sphere_build.argtypes = [_c_dbl_p, _c_int_p, _c_dbl_p, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _c_int_p]
+        L.orc_gaussian_projector_matrix.argtypes = [_c_dbl_p, ctypes.c_int, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_int]
+        sysp = ctypes.POINTER(OrcSystem)
+        L.orc_hamiltonian_apply.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_projector_project.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_projector_apply.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_band_energies.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_exponential_in_place.argtypes = [sysp, _c_dbl_p, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_void_p]
+        L.orc_exponential_2_for_1.argtypes = [sysp, _c_dbl_p, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_void_p]
+        L.orc_etrs_step.argtypes = [sysp, _c_dbl_p, _c_dbl_p, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _ivec(v):
+    return (ctypes.c_int * len(v))(*[int(x) for x in v])
+
+
+def _dvec(v):
+    return (ctypes.c_double * len(v))(*[float(x) for x in v])
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with status {rc}")
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def _as_points_bands(a):
+    """View an [nx][ny][nz][nst] (or [N][nst]) complex array as contiguous."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+def fft3d(a, sign=-1, normalize=False):
+    """Batched 3-D DFT over the first three axes of a [nx][ny][nz][nst] array."""
+    a = _as_points_bands(a)
+    n = a.shape[:3]
+    nst = a.shape[3]
+    out = np.empty_like(a)
+    _check(lib().orc_fft3d(_ivec(n), nst, nst, _ptr(a), _ptr(out), int(sign), int(normalize)),
+           "fft3d")
+    return out
+
+
+def to_fourier(a):
+    return fft3d(a, -1, False)
+
+
+def to_real(a, normalize=True):
+    return fft3d(a, +1, normalize)
+
+
+def fft1d(v, sign=-1):
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    out = np.empty_like(v)
+    _check(lib().orc_fft1d(v.size, _ptr(v), _ptr(out), int(sign)), "fft1d")
+    return out
+
+
+def containing_cube(lattice, n, pos, radius):
+    lo = (ctypes.c_int * 3)()
+    hi = (ctypes.c_int * 3)()
+    _check(lib().orc_containing_cube(_dvec(np.ravel(lattice)), _ivec(n), _dvec(pos), float(radius),
+                                     lo, hi), "containing_cube")
+    return list(lo), list(hi)
+
+
+def sphere_build(lattice, n, center, radius):
+    """Returns (coords int32 [K,3], distance float32 [K], relpos float32 [K,3])."""
+    cap = 4096
+    while True:
+        coords = np.zeros((cap, 3), np.int32)
+        dist = np.zeros(cap, np.float32)
+        rel = np.zeros((cap, 3), np.float32)
+        cnt = ctypes.c_int(0)
+        rc = lib().orc_sphere_build(_dvec(np.ravel(lattice)), _ivec(n), _dvec(center), float(radius),
+                                    cap, _ptr(coords), _ptr(dist), _ptr(rel), ctypes.byref(cnt))
+        if rc == 2:
+            cap = cnt.value
+            continue
+        _check(rc, "sphere_build")
+        k = cnt.value
+        return coords[:k].copy(), dist[:k].copy(), rel[:k].copy()
+
+
+def gaussian_projector_matrix(lattice, distance, relpos, sigma, nlm=4):
+    k = distance.size
+    m = np.zeros((nlm, k), np.float64)
+    d = np.ascontiguousarray(distance, np.float32)
+    r = np.ascontiguousarray(relpos, np.float32)
+    _check(lib().orc_gaussian_projector_matrix(_dvec(np.ravel(lattice)), k, _ptr(d), _ptr(r),
+                                               float(sigma), nlm, _ptr(m), k), "projector matrix")
+    return m
+
+
+class System:
+    """Holds numpy arrays alive and the OrcSystem struct pointing at them."""
+
+    def __init__(self, n, lattice, vloc, tables=None):
+        self.n = tuple(int(x) for x in n)
+        self.lattice = np.ascontiguousarray(np.reshape(lattice, (3, 3)), np.float64)
+        self.vloc = np.ascontiguousarray(np.ravel(vloc), np.float64)
+        s = OrcSystem()
+        s.n = (ctypes.c_int * 3)(*self.n)
+        s.lattice = (ctypes.c_double * 9)(*self.lattice.ravel())
+        s.vloc = self.vloc.ctypes.data_as(_c_dbl_p)
+        if tables is None or tables["nproj"] == 0:
+            self.keep = []
+            s.nproj = 0
+            s.max_sphere = 0
+            s.max_nlm = 0
+        else:
+            pts = np.ascontiguousarray(tables["points"], np.int32)
+            pos = np.ascontiguousarray(tables["positions"], np.float32)
+            mat = np.ascontiguousarray(tables["matrices"], np.float64)
+            coe = np.ascontiguousarray(tables["coeff"], np.float64)
+            nlm = np.ascontiguousarray(tables["nlm"], np.int32)
+            ssz = np.ascontiguousarray(tables["sphere_size"], np.int32)
+            self.keep = [pts, pos, mat, coe, nlm, ssz]
+            s.nproj = int(tables["nproj"])
+            s.max_sphere = int(tables["max_sphere"])
+            s.max_nlm = int(tables["max_nlm"])
+            s.points = pts.ctypes.data_as(_c_int_p)
+            s.positions = pos.ctypes.data_as(_c_flt_p)
+            s.matrices = mat.ctypes.data_as(_c_dbl_p)
+            s.coeff = coe.ctypes.data_as(_c_dbl_p)
+            s.nlm = nlm.ctypes.data_as(_c_int_p)
+            s.sphere_size = ssz.ctypes.data_as(_c_int_p)
+        self.s = s
+
+    @property
+    def npoints(self):
+        return self.n[0] * self.n[1] * self.n[2]
+
+    def hamiltonian(self, psi, kA=(0.0, 0.0, 0.0)):
+        psi = np.ascontiguousarray(psi, np.complex128)
+        nst = psi.shape[-1]
+        out = np.empty_like(psi)
+        _check(lib().orc_hamiltonian_apply(ctypes.byref(self.s), _dvec(kA), nst, nst, _ptr(psi),
+                                           _ptr(out)), "hamiltonian_apply")
+        return out
+
+    def project(self, psi, kA=(0.0, 0.0, 0.0)):
+        psi = np.ascontiguousarray(psi, np.complex128)
+        nst = psi.shape[-1]
+        out = np.zeros((self.s.nproj, self.s.max_sphere, nst), np.complex128)
+        _check(lib().orc_projector_project(ctypes.byref(self.s), _dvec(kA), nst, nst, _ptr(psi),
+                                           _ptr(out)), "projector_project")
+        return out
+
+    def apply(self, sphere, hpsi, kA=(0.0, 0.0, 0.0)):
+        sphere = np.ascontiguousarray(sphere, np.complex128)
+        hpsi = np.array(hpsi, dtype=np.complex128, order="C", copy=True)
+        nst = hpsi.shape[-1]
+        _check(lib().orc_projector_apply(ctypes.byref(self.s), _dvec(kA), nst, nst, _ptr(sphere),
+                                         _ptr(hpsi)), "projector_apply")
+        return hpsi
+
+    def exponential_in_place(self, psi, z, order=4, kA=(0.0, 0.0, 0.0)):
+        psi = np.array(psi, dtype=np.complex128, order="C", copy=True)
+        nst = psi.shape[-1]
+        _check(lib().orc_exponential_in_place(ctypes.byref(self.s), _dvec(kA), float(np.real(z)),
+                                              float(np.imag(z)), int(order), nst, nst, _ptr(psi)),
+               "exponential_in_place")
+        return psi
+
+    def exponential_2_for_1(self, psi, z1, z2, order=4, kA=(0.0, 0.0, 0.0)):
+        phi = np.array(psi, dtype=np.complex128, order="C", copy=True)
+        exp = np.empty_like(phi)
+        nst = phi.shape[-1]
+        _check(lib().orc_exponential_2_for_1(ctypes.byref(self.s), _dvec(kA), float(np.real(z1)),
+                                             float(np.imag(z1)), float(np.real(z2)),
+                                             float(np.imag(z2)), int(order), nst, nst, _ptr(phi),
+                                             _ptr(exp)), "exponential_2_for_1")
+        return phi, exp
+
+    def etrs_step(self, psi, dt, kA_t, kA_tdt, order=4):
+        psi = np.array(psi, dtype=np.complex128, order="C", copy=True)
+        nst = psi.shape[-1]
+        _check(lib().orc_etrs_step(ctypes.byref(self.s), _dvec(kA_t), _dvec(kA_tdt), float(dt),
+                                   int(order), nst, nst, _ptr(psi)), "etrs_step")
+        return psi
+
+
+def band_energies(hpsi, psi):
+    hpsi = np.ascontiguousarray(hpsi, np.complex128)
+    psi = np.ascontiguousarray(psi, np.complex128)
+    nst = psi.shape[-1]
+    npoint = psi.size // nst
+    eps = np.zeros(nst, np.float64)
+    _check(lib().orc_band_energies(npoint, nst, nst, _ptr(hpsi), _ptr(psi), _ptr(eps)),
+           "band_energies")
+    return eps

@@ -1,0 +1,219 @@
+"""GPU parity of the HIP library against the CPU oracle (through the C ABI).
+
+Tolerance: 1e-12 relative (||gpu - oracle||_2 / ||oracle||_2, globally and per
+band) for Hpsi, transforms, projector outputs and band energies, FP64.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import inq_amd
+from inq_amd import operations, systems
+from helpers import band_relerr, grid_positions, relerr
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a, np.complex128)).to(device)
+
+
+def to_host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    s = systems.diamond_system(1)
+    psi = systems.random_orbitals(s, 16)
+    return s, psi, O.System(s.n, s.lattice, s.vloc, s.tables)
+
+
+@pytest.mark.parametrize("n", [(32, 32, 32), (24, 30, 36), (17, 10, 12), (8, 9, 5)])
+@pytest.mark.parametrize("nst", [1, 5, 16, 37])
+def test_transform_matches_oracle(device, n, nst):
+    say("transform", n, nst)
+    rng = np.random.default_rng(sum(n) + nst)
+    a = rng.standard_normal(n + (nst,)) + 1j * rng.standard_normal(n + (nst,))
+    plan = operations.fourier_plan(n, np.diag([5.0, 6.0, 7.0]), max_bands=nst)
+    da = to_dev(a, device)
+    f = operations.transform.to_fourier(da, plan)
+    assert relerr(to_host(f), O.to_fourier(a)) < TOL
+    r = operations.space.to_real(f, plan)
+    assert relerr(to_host(r), a) < TOL
+    # in place
+    operations.transform.to_fourier(da, plan, out=da)
+    assert relerr(to_host(da), O.to_fourier(a)) < TOL
+
+
+@pytest.mark.parametrize("kA", [(0.0, 0.0, 0.0), (0.35, -0.21, 0.5)])
+def test_hpsi_config0_matches_oracle(device, cfg0, kA):
+    s, psi, osys = cfg0
+    say("hpsi config0 kA", kA)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    ham.uniform_vector_potential = kA
+    hphi = to_host(ham(to_dev(psi, device)))
+    ref = osys.hamiltonian(psi, kA)
+    err, berr = relerr(hphi, ref), band_relerr(hphi, ref)
+    say("hpsi rel err", err, "per-band", berr)
+    assert err < TOL and berr < TOL
+    eps = ham.band_energies(to_dev(hphi, device), to_dev(psi, device)).cpu().numpy()
+    eref = O.band_energies(ref, psi)
+    assert np.abs(eps - eref).max() < TOL * np.abs(eref).max()
+    ham.close()
+
+
+def test_hpsi_leading_dimension_and_band_slices(device, cfg0):
+    """ld > nst: apply H to the first 5 bands of a 16-band buffer."""
+    s, psi, osys = cfg0
+    say("hpsi ld slice")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    d = to_dev(psi, device)
+    out = torch.zeros_like(d)
+    ham.handle.call("inqamd_hamiltonian_apply", ctypes.c_void_p(d.data_ptr()),
+                    ctypes.c_void_p(out.data_ptr()), 5, 16)
+    got = to_host(out)
+    ref = osys.hamiltonian(np.ascontiguousarray(psi[..., :5]))
+    assert relerr(got[..., :5], ref) < TOL
+    assert np.abs(got[..., 5:]).max() == 0.0
+    ham.close()
+
+
+def test_projector_project_apply_and_non_local(device, cfg0):
+    s, psi, osys = cfg0
+    kA = (0.1, 0.2, -0.15)
+    say("projector", kA)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    ham.uniform_vector_potential = kA
+    d = to_dev(psi, device)
+    sph = ham.projectors_all().project(d)
+    sref = osys.project(psi, kA)
+    assert relerr(to_host(sph), sref) < TOL
+    base = np.random.default_rng(0).standard_normal(psi.shape) + 0j
+    acc = ham.projectors_all().apply(sph, to_dev(base, device))
+    assert relerr(to_host(acc), osys.apply(sref, base, kA)) < TOL
+    vnl = to_host(ham.non_local(d))
+    assert relerr(vnl, osys.apply(sref, np.zeros_like(psi), kA)) < TOL
+    ham.close()
+
+
+def test_hpsi_bitwise_deterministic(device, cfg0):
+    s, psi, _ = cfg0
+    say("determinism")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    d = to_dev(psi, device)
+    a = to_host(ham(d))
+    b = to_host(ham(d))
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    ham.close()
+
+
+def test_harmonic_oscillator_known_answer_gpu(device):
+    """ks_hamiltonian.hpp:356-397 on the GPU: mean |H phi - 1.5 w phi| = 0.0051420503."""
+    say("harmonic oscillator")
+    lat = np.eye(3) * 10.0
+    n = (20, 20, 20)
+    r = grid_positions(lat, n)
+    r2 = (r * r).sum(-1)
+    s = systems.System(n=n, lattice=lat, vloc=2.0 * r2, atoms=np.zeros((0, 3)),
+                       tables={"nproj": 0})
+    phi = np.repeat(np.exp(-2.0 * r2)[..., None], 6, axis=-1).astype(complex)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=6)
+    h = to_host(ham(to_dev(phi, device)))
+    assert np.abs(h - 3.0 * phi).mean() == pytest.approx(0.0051420503, abs=1e-10)
+    ham.close()
+
+
+@pytest.mark.parametrize("order", [1, 4])
+def test_exponential_in_place_matches_oracle(device, cfg0, order):
+    s, psi, osys = cfg0
+    kA = (0.0, 0.0, 0.07)
+    say("exponential order", order)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    ham.uniform_vector_potential = kA
+    d = to_dev(psi[..., :6], device)
+    operations.exponential_in_place(ham, 0.025j, d, order)
+    ref = osys.exponential_in_place(psi[..., :6], 0.025j, order, kA)
+    assert relerr(to_host(d), ref) < TOL
+    ham.close()
+
+
+def test_exponential_2_for_1_matches_oracle(device, cfg0):
+    s, psi, osys = cfg0
+    say("exponential_2_for_1")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    d = to_dev(psi[..., :4], device)
+    e = operations.exponential_2_for_1(ham, 0.025j, 0.05j, d, 4)
+    rphi, rexp = osys.exponential_2_for_1(psi[..., :4], 0.025j, 0.05j, 4)
+    assert relerr(to_host(d), rphi) < TOL
+    assert relerr(to_host(e), rexp) < TOL
+    ham.close()
+
+
+def test_etrs_steps_match_oracle(device, cfg0):
+    s, psi, osys = cfg0
+    say("etrs")
+    lat = s.lattice
+    pert = inq_amd.real_time.fourier_series_vector_potential([0.03, -0.02, 0.01], 0.1)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    d = to_dev(psi[..., :4], device)
+    ref = psi[..., :4].copy()
+    dt = 0.05
+    for step in range(3):
+        t = step * dt
+        inq_amd.real_time.etrs(ham, d, t, dt, pert)
+        At = lat @ pert.uniform_vector_potential(t)
+        Atdt = lat @ pert.uniform_vector_potential(t + dt)
+        ref = osys.etrs_step(ref, dt, At, Atdt)
+    err = relerr(to_host(d), ref)
+    say("etrs rel err after 3 steps", err)
+    assert err < TOL
+    ham.close()
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    s = systems.diamond_system(2)
+    psi = systems.random_orbitals(s, 256)
+    return s, psi
+
+
+def test_hpsi_config1_full_size(device, cfg1):
+    """64^3 x 256 bands, 64 atoms: oracle parity on a band sample (bands are
+    independent under H), plus size-independent properties on all bands:
+    linearity and Hermiticity."""
+    s, psi = cfg1
+    say("hpsi config1 full size")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=256)
+    d = to_dev(psi, device)
+    h = ham(d)
+    sample = [0, 1, 127, 255]
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    ref = osys.hamiltonian(np.ascontiguousarray(psi[..., sample]))
+    got = to_host(h)[..., sample]
+    err = relerr(got, ref)
+    say("config1 sample rel err", err)
+    assert err < TOL and band_relerr(got, ref) < TOL
+    # linearity: H(a psi + b chi) = a H psi + b H chi, with chi = psi rolled by one band
+    a, b = 0.7 - 0.2j, -1.3 + 0.4j
+    chi = torch.roll(d, 1, dims=-1)
+    lhs = ham(a * d + b * chi)
+    rhs = a * h + b * torch.roll(h, 1, dims=-1)
+    assert (torch.linalg.vector_norm(lhs - rhs) / torch.linalg.vector_norm(rhs)).item() < TOL
+    # Hermiticity: <psi_i|H psi_j> = conj(<psi_j|H psi_i>) over all bands
+    m = (d.reshape(-1, 256).conj().T.contiguous() @ h.reshape(-1, 256)) * (s.volume / s.npoints)
+    herm = (torch.linalg.matrix_norm(m - m.conj().T) / torch.linalg.matrix_norm(m)).item()
+    say("config1 hermiticity", herm)
+    assert herm < TOL
+    ham.close()
