@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run 3 plain device copies of this size (calibrates rocprof "
+                         "FETCH_SIZE/WRITE_SIZE against a kernel of known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the bands shard with no communication inside "
@@ -84,6 +87,13 @@ def main():
     d_psi = torch.from_numpy(psi).to(dev)
     d_out = torch.empty_like(d_psi)
     ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    if args.copy_bytes > 0:
+        src = torch.empty(args.copy_bytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         ham(d_psi, out=d_out)
     torch.cuda.synchronize()

@@ -66,4 +66,13 @@ int fft_band_tile(int nst, int len);
 // Launch one pencil pass of the given mode on `stream`.
 void fft_launch_pass(const fft_pass_args &a, int mode, hipStream_t stream);
 
+// Register-resident pass (fft_reg.hip): plans with radices in {2,3,4,5,8},
+// len <= 128.  `variant` selects the (band tile, threads) shape.
+bool fft_reg_supported(const fft_pass_args &a);
+void fft_reg_launch(const fft_pass_args &a, int mode, int variant, hipStream_t stream);
+
+// Plan-specialised pass (fft_plan.hip) for the tabulated lengths and band
+// tiles B in {8, 16}; returns false when (len, B) has no instantiation.
+bool fft_plan_launch(const fft_pass_args &a, int mode, int B, hipStream_t stream);
+
 }  // namespace inqamd

@@ -5,6 +5,7 @@
 #include "inq_amd.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <complex>
 #include <new>
 
@@ -168,22 +169,47 @@ static void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld,
 //   x-fwd, y-fwd, [z-fwd, x 1/2|G+k+A|^2, z-bwd]     :138-140, :146 (shift)
 //   y-bwd, [x-bwd, x 1/N, + V psi]                   :144-146
 //   apply (ordered scatter of V_nl psi)              :149
-static void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
-                              int slot) {
-  projector_project_coefficients(h, psi, ld, nst, slot);
+// The five passes of T + V on bands [b0, b0 + nb) of psi, through a dense
+// [N][nb] intermediate.
+static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
+                         int slot) {
   cplx *w = h.work;
-  const long wl = nst;
-  fft_launch_pass(pass_args(h, 0, psi, ld, w, wl, nst, -1, 1.0), FFT_PLAIN, h.stream);
-  fft_launch_pass(pass_args(h, 1, w, wl, w, wl, nst, -1, 1.0), FFT_PLAIN, h.stream);
-  fft_pass_args kz = pass_args(h, 2, w, wl, w, wl, nst, -1, 1.0);
+  const long wl = nb;
+  fft_launch_pass(pass_args(h, 0, psi, ld, w, wl, nb, -1, 1.0), FFT_PLAIN, h.stream);
+  fft_launch_pass(pass_args(h, 1, w, wl, w, wl, nb, -1, 1.0), FFT_PLAIN, h.stream);
+  fft_pass_args kz = pass_args(h, 2, w, wl, w, wl, nb, -1, 1.0);
   kz.kA = h.kA + 3 * slot;
   fft_launch_pass(kz, FFT_KINETIC, h.stream);
-  fft_launch_pass(pass_args(h, 1, w, wl, w, wl, nst, +1, 1.0), FFT_PLAIN, h.stream);
-  fft_pass_args ep = pass_args(h, 0, w, wl, hpsi, hld, nst, +1, 1.0 / (double)h.npoint);
+  fft_launch_pass(pass_args(h, 1, w, wl, w, wl, nb, +1, 1.0), FFT_PLAIN, h.stream);
+  fft_pass_args ep = pass_args(h, 0, w, wl, hpsi, hld, nb, +1, 1.0 / (double)h.npoint);
   ep.V = h.V;
   ep.psi = psi;
   ep.psi_ld = ld;
   fft_launch_pass(ep, FFT_EPILOGUE, h.stream);
+}
+
+// Bands per chunk of the local passes (0: all bands at once).  A chunk's
+// intermediate array can stay cache-resident between passes.
+static int band_chunk(const handle &h, int nst) {
+  static int forced = [] {
+    const char *e = getenv("INQAMD_BAND_CHUNK");
+    return e ? atoi(e) : -1;
+  }();
+  if (forced >= 0) return forced == 0 ? nst : forced;
+  // about 128 MiB of intermediate per chunk (half the 256 MB Infinity Cache)
+  long c = (128L << 20) / (h.npoint * (long)sizeof(cplx));
+  c = c < 8 ? 8 : c / 8 * 8;
+  return c < nst ? (int)c : nst;
+}
+
+static void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
+                              int slot) {
+  projector_project_coefficients(h, psi, ld, nst, slot);
+  const int chunk = band_chunk(h, nst);
+  for (int b0 = 0; b0 < nst; b0 += chunk) {
+    const int nb = nst - b0 < chunk ? nst - b0 : chunk;
+    local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot);
+  }
   projector_apply_coefficients(h, hpsi, hld, nst, slot);
 }
 

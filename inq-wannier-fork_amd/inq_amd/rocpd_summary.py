@@ -1,0 +1,88 @@
+"""Summarise rocprofv3 result databases (rocpd SQLite) as plain text.
+
+    python -m inq_amd.rocpd_summary trace.db [--fetch fetch.db --write write.db]
+        [--calib-bytes BYTES] [--per N] [--label TEXT]
+
+--fetch/--write are databases of `rocprofv3 --pmc FETCH_SIZE` and
+`--pmc WRITE_SIZE` runs (values in KiB per dispatch).  The counters are
+scaled by what they read for the largest __amd_rocclr_copyBuffer dispatch,
+a plain copy of --calib-bytes bytes (read and written once).  --per divides
+totals by the number of Hpsi calls in the run.
+"""
+import argparse
+import collections
+import sqlite3
+
+
+def kernel_stats(db):
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, total_calls, total_duration, average, percentage "
+                       "from top_kernels").fetchall()
+    con.close()
+    return rows
+
+
+def pmc(db, counter):
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, grid_size, value, duration from counters_collection "
+                       "where counter_name = ?", (counter,)).fetchall()
+    con.close()
+    agg = collections.defaultdict(list)
+    for name, grid, val, dur in rows:
+        agg[(name, grid)].append((val, dur))
+    return agg
+
+
+def short(name, width=72):
+    return name if len(name) <= width else name[: width - 3] + "..."
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("trace")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--calib-bytes", type=float, default=float(1 << 30))
+    ap.add_argument("--per", type=int, default=1, help="Hpsi calls in the traced run")
+    ap.add_argument("--label", default="")
+    args = ap.parse_args()
+
+    print(f"# rocprofv3 kernel summary {args.label}".rstrip())
+    print(f"{'kernel':74s} {'calls':>6s} {'avg_us':>9s} {'total_us':>10s} {'%':>6s}")
+    for name, calls, total, avg, pct in kernel_stats(args.trace):
+        # top_kernels reports microseconds
+        print(f"{short(name):74s} {calls:6d} {avg:9.2f} {total:10.1f} {pct:6.2f}")
+
+    if args.fetch and args.write:
+        f = pmc(args.fetch, "FETCH_SIZE")
+        w = pmc(args.write, "WRITE_SIZE")
+
+        def calib(agg):
+            cands = [(g, v) for (n, g), v in agg.items() if "copyBuffer" in n]
+            g, v = max(cands, key=lambda x: x[0])
+            return args.calib_bytes / (sum(x[0] for x in v) / len(v) * 1024.0)
+
+        cf, cw = calib(f), calib(w)
+        print()
+        print(f"# HBM-side bytes per dispatch (FETCH_SIZE x {cf:.3f}, WRITE_SIZE x {cw:.3f}; "
+              f"factors from a {args.calib_bytes / 2**20:.0f} MiB copyBuffer)")
+        print(f"{'kernel':74s} {'grid':>10s} {'read_MB':>9s} {'write_MB':>9s} {'avg_us':>8s} {'GB/s':>8s}")
+        tot_r = tot_w = 0.0
+        for key in sorted(f, key=lambda k: -sum(x[1] for x in f[k])):
+            name, grid = key
+            if "copyBuffer" in name or "vectorized_elementwise" in name or key not in w:
+                continue
+            rd = sum(x[0] for x in f[key]) / len(f[key]) * 1024.0 * cf
+            wr = sum(x[0] for x in w[key]) / len(w[key]) * 1024.0 * cw
+            dur = sum(x[1] for x in f[key]) / len(f[key])
+            n = len(f[key])
+            tot_r += rd * n
+            tot_w += wr * n
+            print(f"{short(name):74s} {grid:10d} {rd / 1e6:9.1f} {wr / 1e6:9.1f} {dur / 1e3:8.1f} "
+                  f"{(rd + wr) / max(dur, 1):8.0f}")
+        print(f"# total per Hpsi: read {tot_r / args.per / 1e9:.3f} GB, write "
+              f"{tot_w / args.per / 1e9:.3f} GB, traffic {(tot_r + tot_w) / args.per / 1e9:.3f} GB")
+
+
+if __name__ == "__main__":
+    main()
