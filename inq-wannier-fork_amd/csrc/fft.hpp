@@ -75,4 +75,25 @@ void fft_reg_launch(const fft_pass_args &a, int mode, int variant, hipStream_t s
 // tiles B in {8, 16}; returns false when (len, B) has no instantiation.
 bool fft_plan_launch(const fft_pass_args &a, int mode, int B, hipStream_t stream);
 
+// Arguments of the three-sweep local Hamiltonian (fft3.hip) on a band chunk
+// of a cubic L^3 grid.
+struct fft3_args {
+  const cplx *psi;   // chunk's first band, INQ layout, leading dimension ld
+  long ld;
+  cplx *hpsi;        // output, leading dimension hld
+  long hld;
+  cplx *W;           // band-major intermediate [nb][L][L][L]
+  int nb;
+  const double *V;   // [N]
+  const double *kA;  // device covariant k + A
+  double rlat[9];
+  const cplx *tw;    // W_L^k table
+  double scale;      // 1/N
+};
+
+// True when the three-sweep kernels exist for length L and fit in max_lds.
+bool fft3_supported(int L, size_t max_lds);
+// hpsi = IFFT(1/2|G+k+A|^2 FFT(psi)) / N + V psi for the chunk's bands.
+void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream);
+
 }  // namespace inqamd

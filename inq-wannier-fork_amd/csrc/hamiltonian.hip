@@ -173,6 +173,22 @@ static void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld,
 // [N][nb] intermediate.
 static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
                          int slot) {
+  if (h.three_sweep) {
+    fft3_args a;
+    a.psi = psi;
+    a.ld = ld;
+    a.hpsi = hpsi;
+    a.hld = hld;
+    a.W = h.work;
+    a.nb = nb;
+    a.V = h.V;
+    a.kA = h.kA + 3 * slot;
+    for (int d = 0; d < 9; d++) a.rlat[d] = h.rlat[d];
+    a.tw = h.axis[0].tw;
+    a.scale = 1.0 / (double)h.npoint;
+    fft3_local_hamiltonian(a, h.n[0], h.stream);
+    return;
+  }
   cplx *w = h.work;
   const long wl = nb;
   fft_launch_pass(pass_args(h, 0, psi, ld, w, wl, nb, -1, 1.0), FFT_PLAIN, h.stream);
@@ -349,6 +365,18 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
     h.axis[a].tw = (cplx *)dalloc(tw.size() * sizeof(cplx));
     INQAMD_HIP_CHECK(hipMemcpy(h.axis[a].tw, tw.data(), tw.size() * sizeof(cplx),
                                hipMemcpyHostToDevice));
+  }
+  {
+    int dev = 0, lds = 0;
+    INQAMD_HIP_CHECK(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess ||
+        lds <= 0)
+      INQAMD_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    h.max_lds = (size_t)lds;
+    const char *path = getenv("INQAMD_HPSI_PATH");
+    const bool pencil_forced = path && std::string(path) == "pencil";
+    h.three_sweep = !pencil_forced && h.n[0] == h.n[1] && h.n[1] == h.n[2] &&
+                    fft3_supported(h.n[0], h.max_lds);
   }
   h.V = (double *)dalloc(h.npoint * sizeof(double));
   INQAMD_HIP_CHECK(hipMemcpy(h.V, d.vloc, h.npoint * sizeof(double), hipMemcpyHostToDevice));

@@ -42,6 +42,8 @@ struct inqamd_hamiltonian {
   int max_bands = 0;
 
   inqamd::fft_axis_plan axis[3];
+  size_t max_lds = 65536;       // opt-in LDS per workgroup on this device
+  bool three_sweep = false;     // cubic grid with fft3 kernels available
 
   double *V = nullptr;          // [N]
   double *kA = nullptr;         // [KA_NSLOTS][3]
