@@ -74,6 +74,11 @@ void inqamd_hamiltonian_destroy(inqamd_hamiltonian_t h);
 int inqamd_set_stream(inqamd_hamiltonian_t h, void *stream);
 const char *inqamd_last_error(inqamd_hamiltonian_t h);
 const char *inqamd_version(void);
+/* Self-test of the FP64 matrix-core lane layout the projector kernels use:
+ * writes D = A B (16 x 16, row-major) for A[i][k] = 1 + i + 16k,
+ * B[k][j] = 2 + 3j - 5k into the DEVICE array out[256].  No reference
+ * counterpart; a diagnostic for the projector contraction. */
+int inqamd_selftest_mfma(double *out_device, void *stream);
 
 /* orbital_set::kpoint() (orbital_set.hpp) -- covariant k. */
 int inqamd_set_kpoint(inqamd_hamiltonian_t h, const double k_cov[3]);
@@ -136,6 +141,33 @@ int inqamd_exponential_2_for_1(inqamd_hamiltonian_t h, double z1re, double z1im,
  * psi <- T(i dt/2, H(A_tdt)) T(i dt/2, H(A_t)) psi; leaves A = A_tdt. */
 int inqamd_etrs_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov[3],
                      const double A_tdt_cov[3], int order, void *psi, int nst, int ld);
+
+/* ---- graph-captured ETRS --------------------------------------------------------
+ * One frozen-potential ETRS step on a fixed orbital buffer, captured once as a
+ * hipGraph (no allocation, no host synchronisation at replay).  The vector
+ * potential is read from device memory at replay:
+ *  - inqamd_etrs_graph_launch(g, A_t, A_tdt) writes k + A(t) and k + A(t+dt)
+ *    into the device slots on the handle's stream, then launches the graph;
+ *  - with a Fourier-series A(t) = direction * sum_{j=1..n} c_j sin(j w t)
+ *    (Cartesian) the graph evaluates A at t = step*dt and t + dt itself and
+ *    advances a device step counter, so A_t/A_tdt may be NULL at launch.
+ * Replaces the per-step host loop of real_time::propagate (propagate.hpp:93-98)
+ * around etrs (etrs.hpp:22-82). */
+typedef struct inqamd_graph *inqamd_graph_t;
+typedef struct {
+  int n;                    /* number of coefficients (<= 64) */
+  const double *coeff;      /* host [n] */
+  double omega;             /* base angular frequency */
+  double direction[3];      /* Cartesian polarisation */
+} inqamd_vector_potential_series;
+
+int inqamd_etrs_graph_create(inqamd_hamiltonian_t h, void *psi, int nst, int ld, double dt,
+                             int order, const inqamd_vector_potential_series *series,
+                             inqamd_graph_t *out);
+int inqamd_etrs_graph_launch(inqamd_graph_t g, const double A_t_cov[3], const double A_tdt_cov[3]);
+/* Step counter of a series graph (time = step * dt); stream ordered. */
+int inqamd_etrs_graph_set_step(inqamd_graph_t g, long step);
+void inqamd_graph_destroy(inqamd_graph_t g);
 
 /* ---- host-side system builders (no GPU needed) --------------------------------
  * containing_cube (containing_cube.hpp:20-45). */

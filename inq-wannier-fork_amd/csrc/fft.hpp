@@ -89,10 +89,24 @@ struct fft3_args {
   double rlat[9];
   const cplx *tw;    // W_L^k table
   double scale;      // 1/N
+  // projector apply fused into the last sweep (own == nullptr: not fused):
+  // hpsi(p, b) += sum_{j < cnt} Qe[e0 + j][b] with own[p] = (e0 << 4) | cnt
+  const int *own;
+  const cplx *Qe;        // [nent][ldQ], offset to the chunk's first band
+  long ldQ;
+  // Taylor accumulations acc += c * hpsi (p == nullptr: unused), chunk-offset
+  cplx *acc1;
+  long acc1_ld;
+  double c1re, c1im;
+  cplx *acc2;
+  long acc2_ld;
+  double c2re, c2im;
 };
 
 // True when the three-sweep kernels exist for length L and fit in max_lds.
 bool fft3_supported(int L, size_t max_lds);
+// Sets the kernels' LDS attributes for length L (once, before any launch).
+void fft3_prepare(int L);
 // hpsi = IFFT(1/2|G+k+A|^2 FFT(psi)) / N + V psi for the chunk's bands.
 void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream);
 

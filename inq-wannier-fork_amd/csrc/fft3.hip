@@ -11,6 +11,9 @@
 //             hpsi = FFT/N + V psi in INQ layout (psi, V prefetched into
 //             registers at kernel start).
 //
+// The projector's V_nl psi (precomputed per owned point) and the Taylor
+// accumulations acc += c * hpsi are added in the last sweep's write.
+//
 // Traffic per band: 2 x 16N (z_fwd) + 2 x 16N (xy_kin) + 3 x 16N (z_bwd_epi)
 // = 112 N bytes, against 176 N for five pencil passes; run per band chunk
 // so W stays resident in the Infinity Cache between sweeps.
@@ -25,6 +28,7 @@ constexpr int Z_TY = 8;                 // y values per z tile
 constexpr int Z_TB = 8;                 // bands per z tile
 constexpr int Z_COLS = Z_TY * Z_TB;     // FFT columns per z tile
 constexpr int Z_NT = 512;
+static_assert(Z_NT % Z_TB == 0, "thread -> band mapping of the epilogue");
 
 constexpr int plane_threads(int L) {
   int t = (L * L / 16 + 63) / 64 * 64;
@@ -139,18 +143,21 @@ __global__ __launch_bounds__(Z_NT) void z_bwd_epi_kernel(const fft3_args a) {
   const int x = (int)(t / nyt);
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
 
-  // prefetch V psi of this thread's outputs (band fastest)
+  // prefetch V psi (and the ownership entries) of this thread's outputs
   cplx pre[NE];
+  int own[NE];
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     const int e = threadIdx.x + k * Z_NT;
     const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
     pre[k] = mk(0.0, 0.0);
+    own[k] = -1;
     if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
       const long p = ((long)x * L + y0 + y) * L + z;
       const cplx ps = a.psi[p * a.ld + cb0 + b];
       const double vp = a.V[p];
       pre[k] = mk(vp * ps.x, vp * ps.y);
+      if (a.own) own[k] = a.own[p];
     }
   }
 
@@ -163,16 +170,43 @@ __global__ __launch_bounds__(Z_NT) void z_bwd_epi_kernel(const fft3_args a) {
   auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + c] = v; };
   run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, +1, false, gl, ll, ls, ls, true);
 
+  // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
+  // points, its ownership entries summed in fixed (sphere, point) order
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    if (own[k] >= 0) {
+      const int e = threadIdx.x + k * Z_NT;
+      const int cb = cb0 + e % Z_TB;
+      const int e0 = own[k] >> 4, cnt = own[k] & 15;
+      for (int j = 0; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
+    }
+  }
+
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     const int e = threadIdx.x + k * Z_NT;
     const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
     if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
       const long p = ((long)x * L + y0 + y) * L + z;
+      const int cb = cb0 + b;
       cplx v = cscale(a.scale, lds[z * Z_COLS + b * Z_TY + y]);
       v.x += pre[k].x;
       v.y += pre[k].y;
-      a.hpsi[p * a.hld + cb0 + b] = v;
+      a.hpsi[p * a.hld + cb] = v;
+      if (a.acc1) {
+        cplx *d = a.acc1 + p * a.acc1_ld + cb;
+        cplx o = *d;
+        o.x += a.c1re * v.x - a.c1im * v.y;
+        o.y += a.c1re * v.y + a.c1im * v.x;
+        *d = o;
+      }
+      if (a.acc2) {
+        cplx *d = a.acc2 + p * a.acc2_ld + cb;
+        cplx o = *d;
+        o.x += a.c2re * v.x - a.c2im * v.y;
+        o.y += a.c2re * v.y + a.c2im * v.x;
+        *d = o;
+      }
     }
   }
 }
@@ -188,19 +222,20 @@ static void ensure_lds(K kernel, size_t bytes) {
 }
 
 template <int R0, int R1, int R2>
+static void prepare3() {
+  constexpr int L = R0 * R1 * R2;
+  ensure_lds(z_fwd_kernel<R0, R1, R2>, (size_t)L * Z_COLS * sizeof(cplx));
+  ensure_lds(xy_kin_kernel<R0, R1, R2>, (size_t)L * (L + 1) * sizeof(cplx));
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2>, (size_t)L * Z_COLS * sizeof(cplx));
+}
+
+template <int R0, int R1, int R2>
 static void launch3(const fft3_args &a, hipStream_t stream) {
   constexpr int L = R0 * R1 * R2;
   const long nyt = (L + Z_TY - 1) / Z_TY;
   const long nbt = (a.nb + Z_TB - 1) / Z_TB;
   const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
   const size_t plds = (size_t)L * (L + 1) * sizeof(cplx);
-  static bool attrs = false;
-  if (!attrs) {
-    ensure_lds(z_fwd_kernel<R0, R1, R2>, zlds);
-    ensure_lds(xy_kin_kernel<R0, R1, R2>, plds);
-    ensure_lds(z_bwd_epi_kernel<R0, R1, R2>, zlds);
-    attrs = true;
-  }
   hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2>), dim3((unsigned)(L * nyt * nbt)), dim3(Z_NT), zlds,
                      stream, a);
   INQAMD_HIP_CHECK(hipGetLastError());
@@ -233,6 +268,16 @@ bool fft3_supported(int L, size_t max_lds) {
       return (size_t)L * (L + 1) * sizeof(cplx) <= max_lds &&
              (size_t)L * Z_COLS * sizeof(cplx) <= max_lds;
     default: return false;
+  }
+}
+
+void fft3_prepare(int L) {
+  switch (L) {
+#define INQAMD_FFT3_CASE(LL, R0, R1, R2) \
+  case LL: prepare3<R0, R1, R2>(); return;
+    INQAMD_FFT3_TABLE(INQAMD_FFT3_CASE)
+#undef INQAMD_FFT3_CASE
+    default: return;
   }
 }
 

@@ -47,6 +47,39 @@ __global__ __launch_bounds__(256) void copy_kernel(cplx *y, long yld, const cplx
   }
 }
 
+// A(t) = direction * sum_j c_j sin(j w t) at t = step*dt and t + dt
+// (etrs.hpp:54: the Hamiltonian is updated to time + dt), covariant, plus k;
+// then step += 1.
+struct series_args {
+  int n;
+  double c[64];
+  double omega;
+  double dt;
+  double dir_cov[3];
+  double k_cov[3];
+};
+
+__global__ void advance_series_kernel(const series_args s, double *step, double *kA) {
+  if (threadIdx.x != 0) return;
+  const double t0 = step[0] * s.dt;
+  const double t1 = t0 + s.dt;
+  double f0 = 0.0, f1 = 0.0;
+  for (int j = 0; j < s.n; j++) {
+    f0 += s.c[j] * sin((j + 1) * s.omega * t0);
+    f1 += s.c[j] * sin((j + 1) * s.omega * t1);
+  }
+  for (int d = 0; d < 3; d++) {
+    kA[3 * 1 + d] = s.k_cov[d] + s.dir_cov[d] * f0;  // KA_T
+    kA[3 * 2 + d] = s.k_cov[d] + s.dir_cov[d] * f1;  // KA_TDT
+    kA[3 * 0 + d] = s.k_cov[d] + s.dir_cov[d] * f1;  // KA_CURRENT: H(t+dt) after the step
+  }
+  step[0] += 1.0;
+}
+
+__global__ void set_scalar_kernel(double *dst, double v) {
+  if (threadIdx.x == 0) dst[0] = v;
+}
+
 constexpr int EN_BT = 64;
 constexpr int EN_LANES = 4;
 constexpr long EN_CHUNK = 256;
@@ -164,17 +197,14 @@ static void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld,
   fft_launch_pass(pass_args(h, 2, out, out_ld, out, out_ld, nst, sign, scale), FFT_PLAIN, h.stream);
 }
 
-// ks_hamiltonian::operator() (ks_hamiltonian.hpp:132-152) as
-//   project (P coefficients)                         :136
-//   x-fwd, y-fwd, [z-fwd, x 1/2|G+k+A|^2, z-bwd]     :138-140, :146 (shift)
-//   y-bwd, [x-bwd, x 1/N, + V psi]                   :144-146
-//   apply (ordered scatter of V_nl psi)              :149
-// The five passes of T + V on bands [b0, b0 + nb) of psi, through a dense
-// [N][nb] intermediate.
+// The local part T + V on the bands [b0, b0 + nb) of psi (pointers already
+// offset to b0).  The three-sweep path also adds the projector apply and the
+// Taylor accumulations `acc` (offset to b0) in its last sweep; the pencil
+// path leaves both to the caller.
 static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
-                         int slot) {
+                         int slot, int b0, long ldP, const taylor_acc *acc) {
   if (h.three_sweep) {
-    fft3_args a;
+    fft3_args a{};
     a.psi = psi;
     a.ld = ld;
     a.hpsi = hpsi;
@@ -186,6 +216,24 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
     for (int d = 0; d < 9; d++) a.rlat[d] = h.rlat[d];
     a.tw = h.axis[0].tw;
     a.scale = 1.0 / (double)h.npoint;
+    const projector_tables &t = h.pt;
+    if (t.nproj > 0 && t.nunique > 0) {
+      a.own = t.own;
+      a.Qe = h.vnl + b0;
+      a.ldQ = ldP;
+    }
+    if (acc && acc->p1) {
+      a.acc1 = acc->p1 + b0;
+      a.acc1_ld = acc->ld1;
+      a.c1re = acc->c1re;
+      a.c1im = acc->c1im;
+    }
+    if (acc && acc->p2) {
+      a.acc2 = acc->p2 + b0;
+      a.acc2_ld = acc->ld2;
+      a.c2re = acc->c2re;
+      a.c2im = acc->c2im;
+    }
     fft3_local_hamiltonian(a, h.n[0], h.stream);
     return;
   }
@@ -218,15 +266,34 @@ static int band_chunk(const handle &h, int nst) {
   return c < nst ? (int)c : nst;
 }
 
+static void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
+                  int nst);
+
+// ks_hamiltonian::operator() (ks_hamiltonian.hpp:132-152):
+//   phases of e^{+-i(k+A).r} for this call's k+A
+//   project -> P coefficients                         :136
+//   per band chunk: FFT, x 1/2|G+k+A|^2, inverse FFT   :138-144
+//     x 1/N + V psi (the 1/2|k+A|^2 shift of :146 is in the kinetic factor)
+//     apply (ordered, atomic-free scatter)             :149
+// and, when `acc` is given, acc_j += c_j * hpsi (the Taylor shift of
+// exponential.hpp:32, :64-65) in the same write.
 static void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
-                              int slot) {
-  projector_project_coefficients(h, psi, ld, nst, slot);
+                              int slot, const taylor_acc *acc = nullptr) {
+  projector_phases(h, slot);
+  if (h.three_sweep)
+    projector_project_expand(h, psi, ld, nst, h.vnl, nst, false);
+  else
+    projector_project_expand(h, psi, ld, nst, nullptr, 0, true);
   const int chunk = band_chunk(h, nst);
   for (int b0 = 0; b0 < nst; b0 += chunk) {
     const int nb = nst - b0 < chunk ? nst - b0 : chunk;
-    local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot);
+    local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc);
   }
-  projector_apply_coefficients(h, hpsi, hld, nst, slot);
+  if (!h.three_sweep) {
+    projector_apply_coefficients(h, hpsi, hld, nst, slot);
+    if (acc && acc->p1) shift(h, {acc->c1re, acc->c1im}, hpsi, hld, acc->p1, acc->ld1, nst);
+    if (acc && acc->p2) shift(h, {acc->c2re, acc->c2im}, hpsi, hld, acc->p2, acc->ld2, nst);
+  }
 }
 
 static void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
@@ -244,20 +311,26 @@ static void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst)
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 
-// exponential_in_place (exponential.hpp:21-34)
+// exponential_in_place (exponential.hpp:21-34): the shift of each order is
+// carried by the Hpsi that produces the term.
 static void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
                                  int nst, int slot) {
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
-  hamiltonian_apply(h, psi, ld, hn, tl, nst, slot);
   std::complex<double> coeff = 1.0;
   for (int iter = 1; iter <= order; iter++) {
-    if (iter > 1) {
-      hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot);
+    coeff *= z / double(iter);
+    taylor_acc acc;
+    acc.p1 = psi;
+    acc.ld1 = ld;
+    acc.c1re = coeff.real();
+    acc.c1im = coeff.imag();
+    if (iter == 1) {
+      hamiltonian_apply(h, psi, ld, hn, tl, nst, slot, &acc);
+    } else {
+      hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot, &acc);
       std::swap(hn, hn2);
     }
-    coeff *= z / double(iter);
-    shift(h, coeff, hn, tl, psi, ld, nst);
   }
 }
 
@@ -267,17 +340,25 @@ static void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex
   copy(h, phi, ld, expphi, ld, nst);
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
-  hamiltonian_apply(h, phi, ld, hn, tl, nst, slot);
   std::complex<double> c1 = 1.0, c2 = 1.0;
   for (int iter = 1; iter <= order; iter++) {
-    if (iter > 1) {
-      hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot);
-      std::swap(hn, hn2);
-    }
     c1 *= z1 / double(iter);
     c2 *= z2 / double(iter);
-    shift(h, c1, hn, tl, expphi, ld, nst);
-    shift(h, c2, hn, tl, phi, ld, nst);
+    taylor_acc acc;
+    acc.p1 = expphi;
+    acc.ld1 = ld;
+    acc.c1re = c1.real();
+    acc.c1im = c1.imag();
+    acc.p2 = phi;
+    acc.ld2 = ld;
+    acc.c2re = c2.real();
+    acc.c2im = c2.imag();
+    if (iter == 1) {
+      hamiltonian_apply(h, phi, ld, hn, tl, nst, slot, &acc);
+    } else {
+      hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot, &acc);
+      std::swap(hn, hn2);
+    }
   }
 }
 
@@ -310,11 +391,11 @@ static void release(handle &h) {
   for (int a = 0; a < 3; a++)
     if (h.axis[a].tw) (void)hipFree(h.axis[a].tw), h.axis[a].tw = nullptr;
   for (void *p : {(void *)h.V, (void *)h.kA, (void *)h.work, (void *)h.hn, (void *)h.hn2,
-                  (void *)h.proj, (void *)h.partial})
+                  (void *)h.proj, (void *)h.vnl, (void *)h.partial})
     if (p) (void)hipFree(p);
   h.V = nullptr;
   h.kA = nullptr;
-  h.work = h.hn = h.hn2 = h.proj = nullptr;
+  h.work = h.hn = h.hn2 = h.proj = h.vnl = nullptr;
   h.partial = nullptr;
   projector_free(h);
 }
@@ -377,6 +458,8 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
     const bool pencil_forced = path && std::string(path) == "pencil";
     h.three_sweep = !pencil_forced && h.n[0] == h.n[1] && h.n[1] == h.n[2] &&
                     fft3_supported(h.n[0], h.max_lds);
+    // kernel attributes are set now so that no later call (or capture) has to
+    if (h.three_sweep) fft3_prepare(h.n[0]);
   }
   h.V = (double *)dalloc(h.npoint * sizeof(double));
   INQAMD_HIP_CHECK(hipMemcpy(h.V, d.vloc, h.npoint * sizeof(double), hipMemcpyHostToDevice));
@@ -395,6 +478,8 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
                    d.coeff, d.nlm, d.sphere_size);
   if (h.pt.nproj > 0)
     h.proj = (cplx *)dalloc((size_t)h.pt.nproj * h.pt.max_nlm * h.max_bands * sizeof(cplx));
+  if (h.pt.nproj > 0 && h.pt.nunique > 0 && h.three_sweep)
+    h.vnl = (cplx *)dalloc((size_t)h.pt.nent * h.max_bands * sizeof(cplx));
 }
 
 }  // namespace inqamd
@@ -446,6 +531,17 @@ thread_local std::string g_create_error;
 extern "C" {
 
 const char *inqamd_version(void) { return "inq_amd 0.1 (gfx950)"; }
+
+int inqamd_selftest_mfma(double *out_device, void *stream) {
+  if (!out_device) return INQAMD_ERR_ARGUMENT;
+  try {
+    mfma_selftest(out_device, (hipStream_t)stream);
+  } catch (const inqamd::error &e) {
+    g_create_error = e.msg;
+    return e.code;
+  }
+  return INQAMD_OK;
+}
 
 const char *inqamd_last_error(inqamd_hamiltonian_t h) {
   if (!h) return g_create_error.c_str();
@@ -536,7 +632,8 @@ int inqamd_non_local(inqamd_hamiltonian_t h, const void *psi, void *vnlpsi, int 
     INQAMD_REQUIRE(psi != vnlpsi, "psi and vnlpsi must not alias");
     // vnlphi.fill(0) then project + apply (ks_hamiltonian.hpp:119-126)
     INQAMD_HIP_CHECK(hipMemsetAsync(vnlpsi, 0, (size_t)h->npoint * ld * sizeof(cplx), h->stream));
-    projector_project_coefficients(*h, (const cplx *)psi, ld, nst, KA_CURRENT);
+    projector_phases(*h, KA_CURRENT);
+    projector_project_expand(*h, (const cplx *)psi, ld, nst, nullptr, 0, true);
     projector_apply_coefficients(*h, (cplx *)vnlpsi, ld, nst, KA_CURRENT);
   });
 }
@@ -571,7 +668,8 @@ int inqamd_projector_project(inqamd_hamiltonian_t h, const void *psi, void *sphe
     check_orbital(*h, psi, "psi");
     check_orbital(*h, sphere, "sphere");
     check_bands(*h, nst, ld, true);
-    projector_project_coefficients(*h, (const cplx *)psi, ld, nst, KA_CURRENT);
+    projector_phases(*h, KA_CURRENT);
+    projector_project_expand(*h, (const cplx *)psi, ld, nst, nullptr, 0, true);
     projector_expand(*h, (cplx *)sphere, nst);
   });
 }
@@ -583,6 +681,7 @@ int inqamd_projector_apply(inqamd_hamiltonian_t h, const void *sphere, void *hps
     check_orbital(*h, sphere, "sphere");
     check_orbital(*h, hpsi, "hpsi");
     check_bands(*h, nst, ld, false);
+    projector_phases(*h, KA_CURRENT);
     projector_apply_sphere(*h, (const cplx *)sphere, (cplx *)hpsi, ld, nst, KA_CURRENT);
   });
 }
@@ -657,6 +756,134 @@ int inqamd_etrs_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov[3],
     for (int d = 0; d < 3; d++) h->A_cov[d] = A_tdt_cov[d];
     update_current_slot(*h);
   });
+}
+
+}  // extern "C"
+
+struct inqamd_graph {
+  inqamd_hamiltonian *h = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture = nullptr;
+  double *step = nullptr;
+  bool series = false;
+};
+
+namespace {
+
+void graph_release(inqamd_graph *g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->capture) (void)hipStreamDestroy(g->capture);
+  if (g->step) (void)hipFree(g->step);
+}
+
+}  // namespace
+
+extern "C" {
+
+int inqamd_etrs_graph_create(inqamd_hamiltonian_t h, void *psi, int nst, int ld, double dt,
+                             int order, const inqamd_vector_potential_series *series,
+                             inqamd_graph_t *out) {
+  if (!h || !out) return INQAMD_ERR_ARGUMENT;
+  *out = nullptr;
+  inqamd_graph *g = new inqamd_graph();
+  g->h = h;
+  int rc = guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(order >= 1, "order must be >= 1");
+    series_args sa{};
+    if (series) {
+      INQAMD_REQUIRE(series->n >= 0 && series->n <= 64, "at most 64 series coefficients");
+      INQAMD_REQUIRE(series->n == 0 || series->coeff, "series coefficients are null");
+      sa.n = series->n;
+      for (int j = 0; j < sa.n; j++) sa.c[j] = series->coeff[j];
+      sa.omega = series->omega;
+      sa.dt = dt;
+      // to_covariant(cartesian) = lattice . v (cell.hpp:184-187)
+      for (int i = 0; i < 3; i++) {
+        sa.dir_cov[i] = h->lattice[3 * i + 0] * series->direction[0] +
+                        h->lattice[3 * i + 1] * series->direction[1] +
+                        h->lattice[3 * i + 2] * series->direction[2];
+        sa.k_cov[i] = h->k_cov[i];
+      }
+      g->series = true;
+      INQAMD_HIP_CHECK(hipMalloc(&g->step, sizeof(double)));
+      const double zero = 0.0;
+      INQAMD_HIP_CHECK(hipMemcpy(g->step, &zero, sizeof(double), hipMemcpyHostToDevice));
+    }
+    INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
+    // make the pending work of the handle's stream (tables, V) visible first
+    INQAMD_HIP_CHECK(hipStreamSynchronize(h->stream));
+    hipStream_t saved = h->stream;
+    h->stream = g->capture;
+    INQAMD_HIP_CHECK(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
+    try {
+      if (g->series) {
+        hipLaunchKernelGGL(advance_series_kernel, dim3(1), dim3(64), 0, g->capture, sa, g->step,
+                           h->kA);
+        INQAMD_HIP_CHECK(hipGetLastError());
+      }
+      const std::complex<double> z(0.0, dt / 2.0);  // etrs.hpp:36, :67
+      exponential_in_place(*h, z, order, (cplx *)psi, ld, nst, KA_T);
+      exponential_in_place(*h, z, order, (cplx *)psi, ld, nst, KA_TDT);
+    } catch (...) {
+      hipGraph_t broken = nullptr;
+      (void)hipStreamEndCapture(g->capture, &broken);
+      if (broken) (void)hipGraphDestroy(broken);
+      h->stream = saved;
+      throw;
+    }
+    h->stream = saved;
+    INQAMD_HIP_CHECK(hipStreamEndCapture(g->capture, &g->graph));
+    INQAMD_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  });
+  if (rc != INQAMD_OK) {
+    graph_release(g);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return INQAMD_OK;
+}
+
+int inqamd_etrs_graph_launch(inqamd_graph_t g, const double A_t_cov[3], const double A_tdt_cov[3]) {
+  if (!g || !g->h) return INQAMD_ERR_ARGUMENT;
+  inqamd_hamiltonian *h = g->h;
+  return guarded(h, [&] {
+    if (!g->series || A_t_cov || A_tdt_cov) {
+      INQAMD_REQUIRE(A_t_cov && A_tdt_cov, "vector potentials are required for this graph");
+      INQAMD_REQUIRE(!g->series, "a series graph computes A(t) itself");
+      double kt[3], ktdt[3];
+      for (int d = 0; d < 3; d++) {
+        kt[d] = h->k_cov[d] + A_t_cov[d];
+        ktdt[d] = h->k_cov[d] + A_tdt_cov[d];
+      }
+      set_slot(*h, KA_T, kt);
+      set_slot(*h, KA_TDT, ktdt);
+      for (int d = 0; d < 3; d++) h->A_cov[d] = A_tdt_cov[d];
+    }
+    INQAMD_HIP_CHECK(hipGraphLaunch(g->exec, h->stream));
+    if (!g->series) update_current_slot(*h);
+  });
+}
+
+int inqamd_etrs_graph_set_step(inqamd_graph_t g, long step) {
+  if (!g || !g->h) return INQAMD_ERR_ARGUMENT;
+  return guarded(g->h, [&] {
+    INQAMD_REQUIRE(g->series, "only series graphs have a step counter");
+    hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, g->h->stream, g->step,
+                       (double)step);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  });
+}
+
+void inqamd_graph_destroy(inqamd_graph_t g) {
+  if (!g) return;
+  if (g->h) (void)hipStreamSynchronize(g->h->stream);
+  graph_release(g);
+  delete g;
 }
 
 }  // extern "C"

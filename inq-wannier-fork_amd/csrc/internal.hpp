@@ -28,6 +28,40 @@ struct projector_tables {
   int *upoint = nullptr;   // [nunique] linear grid index
   int *uoff = nullptr;     // [nunique + 1]
   int *uent = nullptr;     // [sum K] p * max_sphere + i, sorted by (point, p, i)
+  int nent = 0;            // sum K (entries of uent)
+  int *owner = nullptr;    // [N] row of the ownership map holding a grid point, -1 if none
+  // e^{+i(k+A).r} per sphere point and e^{-i(k+A).r} per ownership entry,
+  // recomputed for the k+A of each Hpsi call
+  cplx *phase_sph = nullptr;  // [nproj][max_sphere]
+  cplx *phase_ent = nullptr;  // [nent]
+  int *ent = nullptr;         // [nproj][max_sphere] ownership entry of each sphere point
+  int *own = nullptr;         // [N] (first entry << 4) | count for owned points, -1 otherwise
+};
+
+struct project_expand_args {
+  const cplx *psi;
+  long ld;
+  int nst;
+  int max_sphere, max_nlm;
+  const int *point, *size;
+  const double *mat, *coeff;
+  const cplx *phase;  // e^{+i(k+A).r} per sphere point
+  const int *ent;
+  double dV;
+  cplx *P;            // [nproj][max_nlm][nst] or nullptr
+  cplx *Qe;           // [nent][ldQ] V_nl psi per ownership entry, or nullptr
+  long ldQ;
+};
+
+// psi_acc += c * (H psi) accumulations fused into the Hpsi epilogue (Taylor
+// sums); p == nullptr disables a slot.
+struct taylor_acc {
+  cplx *p1 = nullptr;
+  long ld1 = 0;
+  double c1re = 0.0, c1im = 0.0;
+  cplx *p2 = nullptr;
+  long ld2 = 0;
+  double c2re = 0.0, c2im = 0.0;
 };
 
 }  // namespace inqamd
@@ -54,6 +88,7 @@ struct inqamd_hamiltonian {
   inqamd::cplx *hn = nullptr;   // [N][max_bands] Taylor H^n psi
   inqamd::cplx *hn2 = nullptr;  // [N][max_bands] Taylor ping-pong
   inqamd::cplx *proj = nullptr; // [nproj][max_nlm][max_bands]
+  inqamd::cplx *vnl = nullptr;  // [nent][max_bands] V_nl psi per ownership entry
   double *partial = nullptr;    // reduction partials
   long npartial = 0;
 
@@ -71,10 +106,19 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
                       const float *positions, const double *matrices, const double *coeff,
                       const int *nlm, const int *sphere_size);
 void projector_free(handle &h);
+// phase_sph / phase_ent for the k+A in `slot`
+void projector_phases(handle &h, int slot);
 // P[p][lm][b] = dV * D[p][lm] * sum_i M[p][lm][i] e^{i(k+A).r_pi} psi(r_pi, b)
 void projector_project_coefficients(handle &h, const cplx *psi, long ld, int nst, int slot);
 // hpsi(r, b) += sum_{(p,i) at r} e^{-i(k+A).r_pi} sum_lm M[p][lm][i] P[p][lm][b]
 void projector_apply_coefficients(handle &h, cplx *hpsi, long ld, int nst, int slot);
+// Q[u][b] = the same sum for every owned grid point u (compact, ld = nst)
+void projector_owned_sum(handle &h, cplx *Q, int nst, int slot);
+// projector_mfma.hip: gather + MFMA GEMM1 (-> P if write_P) + MFMA GEMM2 ->
+// per-entry V_nl psi in Qe (if Qe != nullptr)
+void projector_project_expand(handle &h, const cplx *psi, long ld, int nst, cplx *Qe, long ldQ,
+                              bool write_P);
+void mfma_selftest(double *out_device, hipStream_t stream);
 // sphere[p][i][b] = sum_lm M[p][lm][i] P[p][lm][b]   (INQ-shaped project output)
 void projector_expand(handle &h, cplx *sphere, int nst);
 // hpsi(r, b) += sum_{(p,i) at r} e^{-i(k+A).r_pi} sphere[p][i][b]

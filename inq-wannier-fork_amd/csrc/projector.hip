@@ -30,7 +30,7 @@ struct project_args {
   const double *mat;
   const double *coeff;
   const int *size;
-  const double *kA;
+  const cplx *phase;  // e^{+i(k+A).r} per sphere point
   double dV;
   cplx *P;  // [nproj][max_nlm][nst]
 };
@@ -44,7 +44,6 @@ __global__ __launch_bounds__(256) void project_kernel(const project_args a) {
   const int b = blockIdx.x * PROJ_BT + bb;
   const int K = a.size[p];
   const int nlm = a.max_nlm;
-  const double k0 = a.kA[0], k1 = a.kA[1], k2 = a.kA[2];
 
   cplx acc[NLM];
 #pragma unroll
@@ -55,11 +54,7 @@ __global__ __launch_bounds__(256) void project_kernel(const project_args a) {
     const double *M = a.mat + (long)p * a.max_nlm * a.max_sphere;
     for (int i = lane; i < K; i += PROJ_LANES) {
       const long gp = a.point[off + i];
-      const float *r = a.pos + (off + i) * 3;
-      const double arg = k0 * (double)r[0] + k1 * (double)r[1] + k2 * (double)r[2];
-      double sn, cs;
-      sincos(arg, &sn, &cs);
-      const cplx v = cmul(mk(cs, sn), a.psi[gp * a.ld + b]);
+      const cplx v = cmul(a.phase[off + i], a.psi[gp * a.ld + b]);
 #pragma unroll
       for (int lm = 0; lm < NLM; lm++) {
         if (lm < nlm) {
@@ -94,9 +89,8 @@ struct apply_args {
   int nst;
   int max_sphere;
   int max_nlm;
-  const float *pos;
   const double *mat;
-  const double *kA;
+  const cplx *phase;   // e^{-i(k+A).r} per ownership entry
   const cplx *P;       // coefficients [nproj][max_nlm][nst]  (mode 0)
   const cplx *sphere;  // sphere array [nproj][max_sphere][nst] (mode 1)
   int nunique;
@@ -105,23 +99,21 @@ struct apply_args {
   const int *uent;
 };
 
-template <int FROM_SPHERE>
+// FROM_SPHERE: contributions come from an INQ-shaped sphere array instead of
+// the P coefficients.  TO_Q: store the per-point sum into the compact
+// [nunique][nst] buffer a.hpsi (ld = nst) instead of adding it to hpsi.
+template <int FROM_SPHERE, int TO_Q = 0>
 __global__ __launch_bounds__(256) void apply_kernel(const apply_args a) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long u = gid / a.nst;
   const int b = (int)(gid % a.nst);
   if (u >= a.nunique) return;
-  const double k0 = a.kA[0], k1 = a.kA[1], k2 = a.kA[2];
   cplx sum = mk(0.0, 0.0);
   const int e1 = a.uoff[u + 1];
   for (int e = a.uoff[u]; e < e1; e++) {
     const int g = a.uent[e];
     const int p = g / a.max_sphere;
     const int i = g - p * a.max_sphere;
-    const float *r = a.pos + (long)g * 3;
-    const double arg = k0 * (double)r[0] + k1 * (double)r[1] + k2 * (double)r[2];
-    double sn, cs;
-    sincos(arg, &sn, &cs);
     cplx s;
     if constexpr (FROM_SPHERE) {
       s = a.sphere[(long)g * a.nst + b];
@@ -137,11 +129,35 @@ __global__ __launch_bounds__(256) void apply_kernel(const apply_args a) {
       }
     }
     // polar(1, -dot(k, r)) (projector_all.hpp:228)
-    const cplx ph = mk(cs, -sn);
-    sum = cadd(sum, cmul(ph, s));
+    sum = cadd(sum, cmul(a.phase[e], s));
   }
-  cplx *dst = a.hpsi + (long)a.upoint[u] * a.ld + b;
-  *dst = cadd(*dst, sum);
+  if constexpr (TO_Q) {
+    a.hpsi[u * a.nst + b] = sum;
+  } else {
+    cplx *dst = a.hpsi + (long)a.upoint[u] * a.ld + b;
+    *dst = cadd(*dst, sum);
+  }
+}
+
+// e^{+i(k+A).r} for every sphere point and e^{-i(k+A).r} for every ownership
+// entry (the polar(1, +-dot(k, pos)) of projector_all.hpp:109, :228).
+__global__ __launch_bounds__(256) void phase_kernel(const float *pos, const int *uent, int nsph,
+                                                    int nent, const double *kA, cplx *phase_sph,
+                                                    cplx *phase_ent) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const double k0 = kA[0], k1 = kA[1], k2 = kA[2];
+  if (g < nsph) {
+    const float *r = pos + (long)g * 3;
+    double sn, cs;
+    sincos(k0 * (double)r[0] + k1 * (double)r[1] + k2 * (double)r[2], &sn, &cs);
+    phase_sph[g] = mk(cs, sn);
+  }
+  if (g < nent) {
+    const float *r = pos + (long)uent[g] * 3;
+    double sn, cs;
+    sincos(k0 * (double)r[0] + k1 * (double)r[1] + k2 * (double)r[2], &sn, &cs);
+    phase_ent[g] = mk(cs, -sn);
+  }
 }
 
 struct expand_args {
@@ -238,6 +254,20 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
   }
   uoff.push_back((int)owners.size());
 
+  std::vector<int> owner(h.npoint, -1), own(h.npoint, -1), ent(nk, -1);
+  for (size_t u = 0; u < upoint.size(); u++) {
+    owner[upoint[u]] = (int)u;
+    const int cnt = uoff[u + 1] - uoff[u];
+    if (cnt > 15) throw error{ERR_UNSUPPORTED, "more than 15 projector spheres share a grid point"};
+    own[upoint[u]] = (uoff[u] << 4) | cnt;
+  }
+  for (size_t e = 0; e < uent.size(); e++) ent[uent[e]] = (int)e;
+  t.own = upload(own);
+  t.ent = upload(ent);
+  t.nent = (int)uent.size();
+  t.owner = upload(owner);
+  INQAMD_HIP_CHECK(hipMalloc(&t.phase_sph, (size_t)nk * sizeof(cplx)));
+  INQAMD_HIP_CHECK(hipMalloc(&t.phase_ent, (size_t)(t.nent > 0 ? t.nent : 1) * sizeof(cplx)));
   t.point = upload(point);
   t.pos = upload(pos);
   t.mat = upload(mat);
@@ -252,9 +282,20 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
 void projector_free(handle &h) {
   projector_tables &t = h.pt;
   for (void *p : {(void *)t.point, (void *)t.pos, (void *)t.mat, (void *)t.coeff, (void *)t.size,
-                  (void *)t.upoint, (void *)t.uoff, (void *)t.uent})
+                  (void *)t.upoint, (void *)t.uoff, (void *)t.uent, (void *)t.owner,
+                  (void *)t.phase_sph, (void *)t.phase_ent, (void *)t.own, (void *)t.ent})
     if (p) (void)hipFree(p);
   t = projector_tables{};
+}
+
+void projector_phases(handle &h, int slot) {
+  const projector_tables &t = h.pt;
+  if (t.nproj == 0) return;
+  const int nsph = t.nproj * t.max_sphere;
+  const int n = nsph > t.nent ? nsph : t.nent;
+  hipLaunchKernelGGL(phase_kernel, dim3((n + 255) / 256), dim3(256), 0, h.stream, t.pos, t.uent,
+                     nsph, t.nent, h.kA + 3 * slot, t.phase_sph, t.phase_ent);
+  INQAMD_HIP_CHECK(hipGetLastError());
 }
 
 void projector_project_coefficients(handle &h, const cplx *psi, long ld, int nst, int slot) {
@@ -271,7 +312,8 @@ void projector_project_coefficients(handle &h, const cplx *psi, long ld, int nst
   a.mat = t.mat;
   a.coeff = t.coeff;
   a.size = t.size;
-  a.kA = h.kA + 3 * slot;
+  a.phase = t.phase_sph;
+  (void)slot;
   a.dV = h.volume / (double)h.npoint;
   a.P = h.proj;
   dim3 grid((nst + PROJ_BT - 1) / PROJ_BT, t.nproj);
@@ -290,9 +332,9 @@ static apply_args make_apply(handle &h, cplx *hpsi, long ld, int nst, int slot) 
   a.nst = nst;
   a.max_sphere = t.max_sphere;
   a.max_nlm = t.max_nlm;
-  a.pos = t.pos;
   a.mat = t.mat;
-  a.kA = h.kA + 3 * slot;
+  a.phase = t.phase_ent;
+  (void)slot;
   a.P = h.proj;
   a.sphere = nullptr;
   a.nunique = t.nunique;
@@ -307,6 +349,15 @@ void projector_apply_coefficients(handle &h, cplx *hpsi, long ld, int nst, int s
   apply_args a = make_apply(h, hpsi, ld, nst, slot);
   const long total = (long)a.nunique * nst;
   hipLaunchKernelGGL(apply_kernel<0>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     h.stream, a);
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+void projector_owned_sum(handle &h, cplx *Q, int nst, int slot) {
+  if (h.pt.nproj == 0 || h.pt.nunique == 0) return;
+  apply_args a = make_apply(h, Q, nst, nst, slot);
+  const long total = (long)a.nunique * nst;
+  hipLaunchKernelGGL((apply_kernel<0, 1>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      h.stream, a);
   INQAMD_HIP_CHECK(hipGetLastError());
 }

@@ -46,6 +46,17 @@ class SystemDesc(ctypes.Structure):
     ]
 
 
+class VectorPotentialSeries(ctypes.Structure):
+    """inqamd_vector_potential_series."""
+
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("coeff", _c_dbl_p),
+        ("omega", ctypes.c_double),
+        ("direction", ctypes.c_double * 3),
+    ]
+
+
 # (name, restype, argtypes) for every symbol of include/inq_amd.h
 SIGNATURES = [
     ("inqamd_hamiltonian_create", _i, [ctypes.POINTER(SystemDesc), _vp, ctypes.POINTER(_vp)]),
@@ -53,6 +64,7 @@ SIGNATURES = [
     ("inqamd_set_stream", _i, [_vp, _vp]),
     ("inqamd_last_error", ctypes.c_char_p, [_vp]),
     ("inqamd_version", ctypes.c_char_p, []),
+    ("inqamd_selftest_mfma", _i, [_vp, _vp]),
     ("inqamd_set_kpoint", _i, [_vp, _c_dbl_p]),
     ("inqamd_set_uniform_vector_potential", _i, [_vp, _c_dbl_p]),
     ("inqamd_set_scalar_potential", _i, [_vp, _vp]),
@@ -67,6 +79,11 @@ SIGNATURES = [
     ("inqamd_exponential_in_place", _i, [_vp, _d, _d, _i, _vp, _i, _i]),
     ("inqamd_exponential_2_for_1", _i, [_vp, _d, _d, _d, _d, _i, _vp, _vp, _i, _i]),
     ("inqamd_etrs_step", _i, [_vp, _d, _c_dbl_p, _c_dbl_p, _i, _vp, _i, _i]),
+    ("inqamd_etrs_graph_create", _i, [_vp, _vp, _i, _i, _d, _i, ctypes.POINTER(VectorPotentialSeries),
+                                      ctypes.POINTER(_vp)]),
+    ("inqamd_etrs_graph_launch", _i, [_vp, _c_dbl_p, _c_dbl_p]),
+    ("inqamd_etrs_graph_set_step", _i, [_vp, ctypes.c_long]),
+    ("inqamd_graph_destroy", None, [_vp]),
     ("inqamd_containing_cube", _i, [_c_dbl_p, _c_int_p, _c_dbl_p, _d, _c_int_p, _c_int_p]),
     ("inqamd_sphere_build", _i, [_c_dbl_p, _c_int_p, _c_dbl_p, _d, _i, _vp, _vp, _vp, _c_int_p]),
     ("inqamd_gaussian_projector_matrix", _i, [_c_dbl_p, _i, _vp, _vp, _d, _i, _vp, _i]),

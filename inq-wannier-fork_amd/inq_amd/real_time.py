@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .hamiltonian import _orbital_args
+from .hamiltonian import _orbital_args, _stream_ptr
 
 
 def _to_covariant(lattice, A_cart):
@@ -44,6 +44,67 @@ def propagate(ham, phi, dt, num_steps, pert=None, start_step=0, order=4, callbac
         if callback is not None:
             callback(istep + 1, (istep + 1) * dt, phi)
     return phi
+
+
+class etrs_graph:
+    """One ETRS step on a fixed orbital buffer captured as a hipGraph.
+
+    With `pert=None` each `launch(time)` needs the covariant A(t), A(t+dt)
+    from the host (written to device memory before the replay); with a
+    fourier_series_vector_potential the graph evaluates A itself from a device
+    step counter and `launch()` takes no arguments.
+    """
+
+    def __init__(self, ham, phi, dt, order=4, pert=None):
+        p, nst, ld = _orbital_args(phi)
+        self._ham = ham
+        self._phi = phi
+        self.dt = float(dt)
+        self.pert = pert
+        ser = None
+        if isinstance(pert, fourier_series_vector_potential):
+            self._c = np.ascontiguousarray(pert.c, np.float64)
+            ser = _lib.VectorPotentialSeries()
+            ser.n = int(self._c.size)
+            ser.coeff = self._c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            ser.omega = float(pert.w)
+            ser.direction = (ctypes.c_double * 3)(*[float(x) for x in pert.dir])
+        g = ctypes.c_void_p()
+        ham.handle.call("inqamd_etrs_graph_create", p, nst, ld, ctypes.c_double(dt), int(order),
+                        ctypes.byref(ser) if ser is not None else None, ctypes.byref(g))
+        self._g = g
+        self.series = ser is not None
+
+    def launch(self, time=None):
+        h = self._ham.handle
+        _lib.check(h._lib.inqamd_set_stream(h.h, _stream_ptr()), h.h)
+        if self.series:
+            rc = h._lib.inqamd_etrs_graph_launch(self._g, None, None)
+        else:
+            lat = self._ham.system.lattice
+            if self.pert is not None and self.pert.has_uniform_vector_potential():
+                A_t = _to_covariant(lat, self.pert.uniform_vector_potential(time))
+                A_tdt = _to_covariant(lat, self.pert.uniform_vector_potential(time + self.dt))
+            else:
+                A_t = A_tdt = np.zeros(3)
+            rc = h._lib.inqamd_etrs_graph_launch(self._g, _lib.dvec(A_t), _lib.dvec(A_tdt))
+            self._ham._A = np.asarray(A_tdt, float).copy()
+        _lib.check(rc, h.h)
+
+    def set_step(self, step):
+        _lib.check(self._ham.handle._lib.inqamd_etrs_graph_set_step(self._g, int(step)),
+                   self._ham.handle.h)
+
+    def close(self):
+        if getattr(self, "_g", None):
+            _lib.lib().inqamd_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class fourier_series_vector_potential:

@@ -217,3 +217,72 @@ def test_hpsi_config1_full_size(device, cfg1):
     say("config1 hermiticity", herm)
     assert herm < TOL
     ham.close()
+
+
+def test_etrs_graph_replays_match_eager_bitwise(device, cfg0):
+    """hipGraph-captured ETRS step: replayed 4 times with eager work and host
+    synchronisations between the replays; every replay equals the eager step."""
+    s, psi, osys = cfg0
+    say("etrs graph vs eager")
+    pert = inq_amd.real_time.fourier_series_vector_potential([0.03, -0.02, 0.01], 0.1)
+    dt = 0.05
+    ham_g = inq_amd.ks_hamiltonian(s, max_bands=16)
+    ham_e = inq_amd.ks_hamiltonian(s, max_bands=16)
+    dg = to_dev(psi[..., :8], device)
+    de = dg.clone()
+    g = inq_amd.real_time.etrs_graph(ham_g, dg, dt, 4, pert=None)
+    g.pert = pert  # host-supplied A(t) mode
+    for step in range(4):
+        t = step * dt
+        g.launch(t)
+        inq_amd.real_time.etrs(ham_e, de, t, dt, pert)
+        junk = ham_e(de)  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        a, b = to_host(dg), to_host(de)
+        say("replay", step, "max |diff|", float(np.abs(a - b).max()))
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+        del junk
+    g.close()
+    ham_g.close()
+    ham_e.close()
+
+
+def test_etrs_series_graph_matches_oracle(device, cfg0):
+    """Series-mode graph: A(t) evaluated on the device from a step counter.
+    dt = 0.02 keeps Taylor-4 inside its stability bound on this grid
+    (|lambda_max| dt / 2 = 1.44 < 2 sqrt 2), so 8 steps stay comparable."""
+    s, psi, osys = cfg0
+    say("etrs series graph")
+    pert = inq_amd.real_time.fourier_series_vector_potential([0.04, -0.03, 0.02, 0.01], 0.1)
+    dt = 0.02
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    d = to_dev(psi[..., :4], device)
+    g = inq_amd.real_time.etrs_graph(ham, d, dt, 4, pert=pert)
+    g.set_step(0)
+    ref = psi[..., :4].copy()
+    lat = s.lattice
+    for step in range(8):
+        g.launch()
+        t = step * dt
+        ref = osys.etrs_step(ref, dt, lat @ pert.uniform_vector_potential(t),
+                             lat @ pert.uniform_vector_potential(t + dt))
+        err = relerr(to_host(d), ref)
+        say("series step", step, "rel err", err)
+        assert err < TOL
+    g.close()
+    ham.close()
+
+
+def test_aa_mfma_f64_lane_layout(device):
+    """The FP64 matrix-core lane layout the projector kernels assume."""
+    say("mfma selftest")
+    out = torch.zeros(256, dtype=torch.float64, device=device)
+    L = inq_amd._lib.lib()
+    inq_amd._lib.check(L.inqamd_selftest_mfma(ctypes.c_void_p(out.data_ptr()),
+                                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    got = out.cpu().numpy().reshape(16, 16)
+    i = np.arange(16)[:, None]
+    k = np.arange(4)[None, :]
+    A = 1.0 + i + 16.0 * k
+    B = 2.0 + 3.0 * np.arange(16)[None, :] - 5.0 * np.arange(4)[:, None]
+    assert np.array_equal(got, A @ B)
