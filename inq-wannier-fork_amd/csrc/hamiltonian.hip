@@ -138,13 +138,13 @@ static unsigned grid_for(long total) {
   return (unsigned)blocks;
 }
 
-static void set_slot(handle &h, int slot, const double v[3]) {
+void set_slot(handle &h, int slot, const double v[3]) {
   hipLaunchKernelGGL(set_vec3_kernel, dim3(1), dim3(64), 0, h.stream, h.kA + 3 * slot, v[0], v[1],
                      v[2]);
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 
-static void update_current_slot(handle &h) {
+void update_current_slot(handle &h) {
   double kA[3] = {h.k_cov[0] + h.A_cov[0], h.k_cov[1] + h.A_cov[1], h.k_cov[2] + h.A_cov[2]};
   set_slot(h, KA_CURRENT, kA);
 }
@@ -190,7 +190,7 @@ static fft_pass_args pass_args(const handle &h, int axis, const cplx *in, long i
 }
 
 // 3-D transform as three pencil passes (x, y, z); the last one applies scale.
-static void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int nst,
+void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int nst,
                   int sign, double scale) {
   fft_launch_pass(pass_args(h, 0, in, in_ld, out, out_ld, nst, sign, 1.0), FFT_PLAIN, h.stream);
   fft_launch_pass(pass_args(h, 1, out, out_ld, out, out_ld, nst, sign, 1.0), FFT_PLAIN, h.stream);
@@ -266,8 +266,8 @@ static int band_chunk(const handle &h, int nst) {
   return c < nst ? (int)c : nst;
 }
 
-static void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
-                  int nst);
+void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
+           int nst);
 
 // ks_hamiltonian::operator() (ks_hamiltonian.hpp:132-152):
 //   phases of e^{+-i(k+A).r} for this call's k+A
@@ -277,8 +277,8 @@ static void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cp
 //     apply (ordered, atomic-free scatter)             :149
 // and, when `acc` is given, acc_j += c_j * hpsi (the Taylor shift of
 // exponential.hpp:32, :64-65) in the same write.
-static void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
-                              int slot, const taylor_acc *acc = nullptr) {
+void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
+                       int slot, const taylor_acc *acc) {
   projector_phases(h, slot);
   if (h.three_sweep)
     projector_project_expand(h, psi, ld, nst, h.vnl, nst, false);
@@ -296,7 +296,7 @@ static void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, l
   }
 }
 
-static void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
+void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
                   int nst) {
   const long total = h.npoint * nst;
   hipLaunchKernelGGL(shift_kernel, dim3(grid_for(total)), dim3(256), 0, h.stream, y, yld, x, xld,
@@ -304,7 +304,7 @@ static void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cp
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 
-static void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst) {
+void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst) {
   const long total = h.npoint * nst;
   hipLaunchKernelGGL(copy_kernel, dim3(grid_for(total)), dim3(256), 0, h.stream, y, yld, x, xld,
                      nst, total);
@@ -313,7 +313,7 @@ static void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst)
 
 // exponential_in_place (exponential.hpp:21-34): the shift of each order is
 // carried by the Hpsi that produces the term.
-static void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
+void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
                                  int nst, int slot) {
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
@@ -335,7 +335,7 @@ static void exponential_in_place(handle &h, std::complex<double> z, int order, c
 }
 
 // exponential_2_for_1 (exponential.hpp:50-69)
-static void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2,
+void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2,
                                 int order, cplx *phi, cplx *expphi, long ld, int nst, int slot) {
   copy(h, phi, ld, expphi, ld, nst);
   cplx *hn = h.hn, *hn2 = h.hn2;
@@ -362,7 +362,7 @@ static void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex
   }
 }
 
-static void band_energies(handle &h, const cplx *hpsi, const cplx *psi, long ld, int nst,
+void band_energies(handle &h, const cplx *hpsi, const cplx *psi, long ld, int nst,
                           double *eps) {
   const int nchunk = (int)((h.npoint + EN_CHUNK - 1) / EN_CHUNK);
   dim3 grid(nchunk, (nst + EN_BT - 1) / EN_BT);

@@ -5,6 +5,7 @@
 #include "common.hpp"
 #include "fft.hpp"
 
+#include <complex>
 #include <string>
 #include <vector>
 
@@ -119,6 +120,19 @@ void projector_owned_sum(handle &h, cplx *Q, int nst, int slot);
 void projector_project_expand(handle &h, const cplx *psi, long ld, int nst, cplx *Qe, long ldQ,
                               bool write_P);
 void mfma_selftest(double *out_device, hipStream_t stream);
+
+// hamiltonian.hip
+void set_slot(handle &h, int slot, const double v[3]);
+void update_current_slot(handle &h);
+void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int nst, int sign,
+           double scale);
+void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
+                       int slot, const taylor_acc *acc = nullptr);
+void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
+           int nst);
+void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst);
+void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
+                          int nst, int slot);
 // sphere[p][i][b] = sum_lm M[p][lm][i] P[p][lm][b]   (INQ-shaped project output)
 void projector_expand(handle &h, cplx *sphere, int nst);
 // hpsi(r, b) += sum_{(p,i) at r} e^{-i(k+A).r_pi} sphere[p][i][b]

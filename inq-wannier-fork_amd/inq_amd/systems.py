@@ -65,7 +65,7 @@ def gaussian_vloc(lattice, n, atoms, depth, width):
     Grid point r = symmetric index * a/n (real_space.hpp:76-79)."""
     lat = np.asarray(lattice, float)
     if np.abs(lat - np.diag(np.diag(lat))).max() > 0:
-        raise ValueError("gaussian_vloc supports orthorhombic cells only")
+        return _gaussian_vloc_general(lat, n, atoms, depth, width)
     L = np.diag(lat)
     coords = [symmetric_indices(n[d]) * (L[d] / n[d]) for d in range(3)]
     V = np.zeros(tuple(n))
@@ -76,6 +76,23 @@ def gaussian_vloc(lattice, n, atoms, depth, width):
             x = coords[d][:, None] - R[d] - L[d] * np.arange(-nimg, nimg + 1)[None, :]
             fac.append(np.exp(-x * x / (2.0 * width * width)).sum(axis=1))
         V += depth * fac[0][:, None, None] * fac[1][None, :, None] * fac[2][None, None, :]
+    return V
+
+
+def _gaussian_vloc_general(lat, n, atoms, depth, width):
+    """Direct image sum for a general (non-orthogonal) cell; for small grids."""
+    c = [symmetric_indices(n[d]) / n[d] for d in range(3)]
+    C0, C1, C2 = np.meshgrid(*c, indexing="ij")
+    r = C0[..., None] * lat[0] + C1[..., None] * lat[1] + C2[..., None] * lat[2]
+    reach = 12.0 * width
+    nimg = [int(np.ceil(reach / np.linalg.norm(lat[d]))) + 1 for d in range(3)]
+    V = np.zeros(tuple(n))
+    for R in np.asarray(atoms, float):
+        for i in range(-nimg[0], nimg[0] + 1):
+            for j in range(-nimg[1], nimg[1] + 1):
+                for k in range(-nimg[2], nimg[2] + 1):
+                    d = r - (R + i * lat[0] + j * lat[1] + k * lat[2])
+                    V += depth * np.exp(-(d * d).sum(-1) / (2.0 * width * width))
     return V
 
 

@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <vector>
+#include <cstdint>
 #include <algorithm>
 #include <memory>
 #ifdef _OPENMP
@@ -323,6 +324,142 @@ static void exponential_in_place(const orc_system *s, const double kA[3], cplx z
   }
 }
 
+/* ---------------------------------------------------------------------------
+ * pcg32 (O'Neill's PCG XSH-RR 64/32 as pcg-cpp's setseq engine with the
+ * default stream): state' = state * 6364136223846793005 + 1442695040888963407,
+ * output from the previous state, seeded with bump(seed + increment).
+ * ------------------------------------------------------------------------ */
+struct pcg32 {
+  static constexpr uint64_t mult = 6364136223846793005ULL;
+  static constexpr uint64_t inc = 1442695040888963407ULL;
+  uint64_t state;
+  explicit pcg32(uint64_t seed) : state((seed + inc) * mult + inc) {}
+  uint32_t operator()() {
+    uint64_t old = state;
+    state = old * mult + inc;
+    uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
+  }
+  // jump ahead by delta LCG steps (Brown, "Random number generation with
+  // arbitrary strides", 1994): accumulate the affine map by squaring
+  void discard(uint64_t delta) {
+    uint64_t cur_mult = mult, cur_plus = inc, acc_mult = 1u, acc_plus = 0u;
+    while (delta > 0) {
+      if (delta & 1u) {
+        acc_mult *= cur_mult;
+        acc_plus = acc_plus * cur_mult + cur_plus;
+      }
+      cur_plus = (cur_mult + 1u) * cur_plus;
+      cur_mult *= cur_mult;
+      delta >>= 1;
+    }
+    state = acc_mult * state + acc_plus;
+  }
+};
+
+/* gradient(phi, -1, k+A) then current density (current.hpp:34-42) plus the
+ * [V_nl, r] term; returns J_cov. */
+static void current(const orc_system *s, const double kA_cov[3], double occ, int nst, int ld,
+                    const cplx *psi, double J[3]) {
+  cell c(s->lattice);
+  const long N = npoints(s);
+  std::vector<cplx> fs((size_t)N * nst);
+  fft3d(s->n, nst, ld, psi, fs.data(), -1, false);
+  // g phi = -i (G + k + A) phi, three covariant components
+  std::vector<cplx> g[3];
+  for (int d = 0; d < 3; d++) {
+    std::vector<cplx> gd((size_t)N * nst);
+    for (int ix = 0; ix < s->n[0]; ix++)
+      for (int iy = 0; iy < s->n[1]; iy++)
+        for (int iz = 0; iz < s->n[2]; iz++) {
+          const int m[3] = {symmetric(ix, s->n[0]), symmetric(iy, s->n[1]), symmetric(iz, s->n[2])};
+          const double gcov = 2.0 * M_PI * m[d] + kA_cov[d];
+          const long p = ((long)ix * s->n[1] + iy) * s->n[2] + iz;
+          for (int b = 0; b < nst; b++) gd[p * nst + b] = cplx(0.0, -1.0) * gcov * fs[p * nst + b];
+        }
+    g[d].resize((size_t)N * nst);
+    fft3d(s->n, nst, nst, gd.data(), g[d].data(), +1, true);
+  }
+  // position_commutator: [V_nl, r] phi, r relative to the atom, covariant
+  if (s->nproj > 0) {
+    const long maxK = s->max_sphere;
+    const double dV = c.volume / (double)N;
+    for (int ip = 0; ip < s->nproj; ip++) {
+      if (s->sphere_size[ip] == 0 || s->nlm[ip] == 0) continue;
+      const double *M = s->matrices + (long)ip * s->max_nlm * maxK;
+      std::vector<cplx> sph((size_t)maxK * nst, 0.0), rsph((size_t)maxK * 3 * nst, 0.0);
+      for (long i = 0; i < maxK; i++) {
+        const int *poi = s->points + ((long)ip * maxK + i) * 3;
+        if (poi[0] < 0) continue;
+        const float *pos = s->positions + ((long)ip * maxK + i) * 3;
+        vec3 rr{{(double)pos[0], (double)pos[1], (double)pos[2]}};
+        double arg = kA_cov[0] * rr[0] + kA_cov[1] * rr[1] + kA_cov[2] * rr[2];
+        cplx ph = std::polar(1.0, arg);
+        long gp = ((long)poi[0] * s->n[1] + poi[1]) * s->n[2] + poi[2];
+        for (int b = 0; b < nst; b++) {
+          sph[i * nst + b] = ph * psi[gp * ld + b];
+          for (int d = 0; d < 3; d++) rsph[(i * 3 + d) * nst + b] = rr[d] * sph[i * nst + b];
+        }
+      }
+      std::vector<cplx> P((size_t)s->max_nlm * nst, 0.0), R((size_t)s->max_nlm * 3 * nst, 0.0);
+      for (int lm = 0; lm < s->max_nlm; lm++)
+        for (long i = 0; i < maxK; i++) {
+          double m = M[lm * maxK + i];
+          for (int b = 0; b < nst; b++) {
+            P[lm * nst + b] += dV * m * sph[i * nst + b];
+            for (int d = 0; d < 3; d++) R[(lm * 3 + d) * nst + b] += dV * m * rsph[(i * 3 + d) * nst + b];
+          }
+        }
+      for (int lm = 0; lm < s->max_nlm; lm++) {
+        double co = s->coeff[(long)ip * s->max_nlm + lm];
+        for (int b = 0; b < nst; b++) {
+          P[lm * nst + b] *= co;
+          for (int d = 0; d < 3; d++) R[(lm * 3 + d) * nst + b] *= co;
+        }
+      }
+      for (long i = 0; i < maxK; i++) {
+        const int *poi = s->points + ((long)ip * maxK + i) * 3;
+        if (poi[0] < 0) continue;
+        const float *pos = s->positions + ((long)ip * maxK + i) * 3;
+        vec3 rr{{(double)pos[0], (double)pos[1], (double)pos[2]}};
+        double arg = kA_cov[0] * rr[0] + kA_cov[1] * rr[1] + kA_cov[2] * rr[2];
+        cplx ph = std::polar(1.0, -arg);
+        long gp = ((long)poi[0] * s->n[1] + poi[1]) * s->n[2] + poi[2];
+        for (int b = 0; b < nst; b++) {
+          cplx sphi = 0.0, srphi[3] = {0.0, 0.0, 0.0};
+          for (int lm = 0; lm < s->max_nlm; lm++) {
+            double m = M[lm * maxK + i];
+            sphi += m * P[lm * nst + b];
+            for (int d = 0; d < 3; d++) srphi[d] += m * R[(lm * 3 + d) * nst + b];
+          }
+          // to_covariant(srphi - rr * sphi) (projector_all.hpp:398)
+          cplx contra[3];
+          for (int d = 0; d < 3; d++) contra[d] = srphi[d] - rr[d] * sphi;
+          // covariant = lattice . cartesian, cartesian = sum_e lat_e contra_e
+          cplx cartv[3];
+          for (int d = 0; d < 3; d++)
+            cartv[d] = c.lat[0][d] * contra[0] + c.lat[1][d] * contra[1] + c.lat[2][d] * contra[2];
+          for (int d = 0; d < 3; d++) {
+            cplx cov = c.lat[d][0] * cartv[0] + c.lat[d][1] * cartv[1] + c.lat[d][2] * cartv[2];
+            g[d][gp * nst + b] += ph * cov;
+          }
+        }
+      }
+    }
+  }
+  const double dV = c.volume / (double)N;
+  for (int d = 0; d < 3; d++) {
+    double sum = 0.0;
+    for (long p = 0; p < N; p++)
+      for (int b = 0; b < nst; b++) {
+        cplx ph = psi[p * ld + b], gp = g[d][p * nst + b];
+        sum += 0.5 * occ * std::imag(std::conj(ph) * gp - std::conj(gp) * ph);
+      }
+    J[d] = dV * sum;  /* operations::integral (integral.hpp:24-29) */
+  }
+}
+
 } // namespace
 
 /* ===========================================================================
@@ -540,6 +677,62 @@ int orc_exponential_2_for_1(const orc_system *sys, const double kA_cov[3], doubl
     shift_add(N, nst, ld, c1, hn.data(), expphi);
     shift_add(N, nst, ld, c2, hn.data(), phi);
   }
+  return 0;
+}
+
+int orc_randomize(const int n[3], int nst, int index, double *psi_) {
+  cplx *psi = reinterpret_cast<cplx *>(psi_);
+  const uint64_t N = (uint64_t)n[0] * n[1] * n[2];
+  const uint64_t seed = N * (uint64_t)nst + (uint64_t)index;
+  const double maxv = 4294967296.0;  /* numeric_limits<uint32_t>::max() + 1 */
+  for (uint64_t p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) {
+      uint64_t step = (uint64_t)b + (uint64_t)nst * p;
+      pcg32 rng(seed);
+      rng.discard(step * 2u);
+      double re = 2.0 * rng() / maxv - 1.0;
+      double im = 2.0 * rng() / maxv - 1.0;
+      psi[p * nst + b] = cplx(re, im);
+    }
+  return 0;
+}
+
+int orc_current(const orc_system *sys, const double kA_cov[3], double occ, int nst, int ld,
+                const double *psi, double J_cov[3], double J_cart[3]) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  current(sys, kA_cov, occ, nst, ld, reinterpret_cast<const cplx *>(psi), J_cov);
+  cell c(sys->lattice);
+  vec3 jc{{J_cov[0], J_cov[1], J_cov[2]}};
+  vec3 cart = c.cov_to_cart(jc);
+  for (int d = 0; d < 3; d++) J_cart[d] = cart[d];
+  return 0;
+}
+
+int orc_td_current(const orc_system *sys, int ncoef, const double *coef, double omega,
+                   const double direction[3], double dt, int nsteps, int order, double occ,
+                   int nst, int ld, double *psi_, double *loss) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  cell c(sys->lattice);
+  cplx *psi = reinterpret_cast<cplx *>(psi_);
+  auto A_cov = [&](double t, double out[3]) {
+    double f = 0.0;
+    for (int j = 0; j < ncoef; j++) f += coef[j] * std::sin((j + 1) * omega * t);
+    vec3 cart{{direction[0] * f, direction[1] * f, direction[2] * f}};
+    vec3 cov = c.cart_to_cov(cart);
+    for (int d = 0; d < 3; d++) out[d] = cov[d];
+  };
+  double At[3], Atdt[3];
+  for (int n = 0; n < nsteps; n++) {
+    const double t = n * dt;
+    A_cov(t, At);
+    A_cov(t + dt, Atdt);
+    exponential_in_place(sys, At, cplx(0.0, dt / 2.0), order, nst, ld, psi);
+    exponential_in_place(sys, Atdt, cplx(0.0, dt / 2.0), order, nst, ld, psi);
+  }
+  double AN[3], Jc[3], Jcart[3];
+  A_cov(nsteps > 0 ? (nsteps - 1) * dt + dt : 0.0, AN);
+  orc_current(sys, AN, occ, nst, ld, psi_, Jc, Jcart);
+  *loss = Jcart[0] * direction[0] + Jcart[1] * direction[1] + Jcart[2] * direction[2];
   return 0;
 }
 

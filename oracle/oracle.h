@@ -114,6 +114,27 @@ int orc_exponential_2_for_1(const orc_system *sys, const double kA_cov[3],
 int orc_etrs_step(const orc_system *sys, const double kA_t[3], const double kA_tdt[3],
                   double dt, int order, int nst, int ld, double *psi);
 
+/* operations::randomize (src/operations/randomize.hpp:48-70): pcg32 seeded with
+ * N * nst + index, advanced by 2 (b + nst * point) per complex element;
+ * uniform values 2 g / 2^32 - 1. */
+int orc_randomize(const int n[3], int nst, int index, double *psi);
+
+/* observables::current (src/observables/current.hpp:25-53):
+ * j = sum_b occ Im(conj(phi) g phi), g phi = -i (G + k + A) phi + [V_nl, r] phi
+ * (gradient.hpp:43-48 with factor -1; projector_all.hpp:319-403 with the KB
+ * coefficient applied once to each projection).  J_cov = dV sum_r j;
+ * J_cart = to_cartesian(J_cov) (viewables.hpp:100-102). */
+int orc_current(const orc_system *sys, const double kA_cov[3], double occ, int nst, int ld,
+                const double *psi, double J_cov[3], double J_cart[3]);
+
+/* Frozen-potential propagation under A(t) = direction * sum_{j=1..n} c_j
+ * sin(j w t) (Cartesian): nsteps ETRS steps from psi (in place, A(t) evaluated
+ * at t = step*dt and t + dt as etrs.hpp:54), then returns the final current
+ * projected on `direction` (Cartesian), with occupation occ per band. */
+int orc_td_current(const orc_system *sys, int ncoef, const double *coef, double omega,
+                   const double direction[3], double dt, int nsteps, int order, double occ,
+                   int nst, int ld, double *psi, double *loss);
+
 /* Set the number of OpenMP threads the oracle uses (<=0: library default). */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

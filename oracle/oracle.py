@@ -72,6 +72,13 @@ def lib():
                                               ctypes.c_void_p]
         L.orc_etrs_step.argtypes = [sysp, _c_dbl_p, _c_dbl_p, ctypes.c_double, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.orc_randomize.argtypes = [_c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.orc_current.argtypes = [sysp, _c_dbl_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_void_p, _c_dbl_p, _c_dbl_p]
+        L.orc_td_current.argtypes = [sysp, ctypes.c_int, _c_dbl_p, ctypes.c_double, _c_dbl_p,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.POINTER(ctypes.c_double)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -258,6 +265,38 @@ class System:
         _check(lib().orc_etrs_step(ctypes.byref(self.s), _dvec(kA_t), _dvec(kA_tdt), float(dt),
                                    int(order), nst, nst, _ptr(psi)), "etrs_step")
         return psi
+
+
+def randomize(n, nst, index=0):
+    """operations::randomize for complex orbitals: [nx, ny, nz, nst]."""
+    out = np.zeros(tuple(n) + (nst,), np.complex128)
+    _check(lib().orc_randomize(_ivec(n), int(nst), int(index), _ptr(out)), "randomize")
+    return out
+
+
+def _system_current(self, psi, kA=(0.0, 0.0, 0.0), occ=2.0):
+    psi = np.ascontiguousarray(psi, np.complex128)
+    nst = psi.shape[-1]
+    jc = (ctypes.c_double * 3)()
+    jx = (ctypes.c_double * 3)()
+    _check(lib().orc_current(ctypes.byref(self.s), _dvec(kA), float(occ), nst, nst, _ptr(psi), jc,
+                             jx), "current")
+    return np.array(list(jc)), np.array(list(jx))
+
+
+def _system_td_current(self, psi, coef, omega, direction, dt, nsteps, order=4, occ=2.0):
+    psi = np.array(psi, dtype=np.complex128, order="C", copy=True)
+    nst = psi.shape[-1]
+    c = np.ascontiguousarray(coef, np.float64)
+    loss = ctypes.c_double(0.0)
+    _check(lib().orc_td_current(ctypes.byref(self.s), c.size, c.ctypes.data_as(_c_dbl_p),
+                                float(omega), _dvec(direction), float(dt), int(nsteps), int(order),
+                                float(occ), nst, nst, _ptr(psi), ctypes.byref(loss)), "td_current")
+    return loss.value, psi
+
+
+System.current = _system_current
+System.td_current = _system_td_current
 
 
 def band_energies(hpsi, psi):

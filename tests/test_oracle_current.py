@@ -1,0 +1,109 @@
+"""Pins the oracle's current observable and the propagation loss built on it.
+
+Reference known answers: observables::current on the randomize + orthogonalize
+initial guess of a 6 x 10 x 6 bohr cell at 15 Ha (10 x 17 x 10 grid, 10 bands,
+20 electrons) for Gamma and k = (1/4,1/4,1/4), with A = 0 and A = (1,-2,3)
+(src/observables/current.hpp:74-177) -- this also pins the pcg32 restatement
+of operations::randomize.  Mathematics: phi = 1 gives J = -charge (k+A) V;
+with projectors, J = -d<H(A)>/dA (the [V_nl, r] term included), checked by
+central differences of the energy expectation.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from inq_amd import systems
+from helpers import relerr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KNOWN = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+
+
+def _initial_guess(n, nst, volume):
+    phi = O.randomize(n, nst, 0)
+    C = phi.reshape(-1, nst)
+    dV = volume / np.prod(n)
+    L = np.linalg.cholesky(dV * (C.conj().T @ C))
+    return (C @ np.linalg.inv(L).conj().T).reshape(tuple(n) + (nst,))
+
+
+@pytest.fixture(scope="module")
+def current_case():
+    k = KNOWN["current"]
+    cell = k["cell_orthorhombic"]
+    ecut = k["ecut_ha"]
+    spacing = np.pi / np.sqrt(2.0 * ecut)
+    n = tuple(int(round(L / spacing)) for L in cell)
+    assert n == tuple(k["grid"])
+    phi = _initial_guess(n, k["bands"], np.prod(cell))
+    return k, O.System(n, np.diag(cell), np.zeros(n)), phi
+
+
+@pytest.mark.parametrize("case", ["gamma", "gamma_A", "k_quarter", "k_quarter_A"])
+def test_current_reference_known_answers(current_case, case):
+    k, s, phi = current_case
+    c = k[case]
+    kA = np.array(c["A_cov"], float)
+    if "k_frac" in c:
+        kA = kA + 2.0 * np.pi * np.array(c["k_frac"])
+    jc, _ = s.current(phi, kA, k["occupation"])
+    assert np.allclose(jc, c["J_cov"], rtol=0, atol=1.5e-10)
+
+
+def test_current_zero_paramagnetic(current_case):
+    k, s, _ = current_case
+    n = s.n
+    phi = np.ones(n + (k["bands"],), complex)
+    vol = float(np.prod(k["cell_orthorhombic"]))
+    charge = k["electrons"]
+    for kA in ([0.0, 0.0, 0.0], [1.0, -2.0, 3.0], list(2 * np.pi * 0.25 + np.array([1.0, -2.0, 3.0]))):
+        jc, _ = s.current(phi, kA, k["occupation"])
+        assert np.allclose(jc, -charge * np.array(kA) * vol, rtol=1e-12, atol=1e-9)
+
+
+def test_current_is_minus_energy_derivative_with_projectors():
+    """J_cart = -d/dA sum_b occ <psi_b|H(A)|psi_b> (central differences)."""
+    lat = np.array([[7.0, 0.0, 0.0], [0.3, 6.5, 0.0], [0.0, -0.4, 7.2]])
+    n = (12, 12, 14)
+    atoms = np.array([[0.2, 0.1, -0.3], [2.9, -2.1, 1.6]])
+    s = systems.make_system(n, lat, atoms, seed=3, system_id=11)
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    psi = systems.random_orbitals(s, 3, ecut=6.0)
+    occ = 2.0
+    A0 = np.array([0.05, -0.03, 0.02])  # Cartesian
+    dV = s.volume / s.npoints
+
+    def energy(A_cart):
+        kA = lat @ A_cart  # covariant
+        h = osys.hamiltonian(psi, kA)
+        return occ * dV * np.real(np.sum(psi.conj() * h))
+
+    _, jx = osys.current(psi, lat @ A0, occ)
+    eps = 1e-4
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = eps
+        fd = -(energy(A0 + e) - energy(A0 - e)) / (2 * eps)
+        assert jx[d] == pytest.approx(fd, rel=1e-7, abs=1e-9)
+    # the projector commutator contributes (the test would not see a dropped term otherwise)
+    free = O.System(s.n, s.lattice, s.vloc)
+    _, jfree = free.current(psi, lat @ A0, occ)
+    assert np.abs(jx - jfree).max() > 1e-4 * np.abs(jx).max()
+
+
+def test_td_loss_zero_field_is_conserved_current():
+    """Without a field the ETRS propagation of an H eigen-subspace ... the
+    current of the initial state equals the zero-step loss."""
+    s = systems.make_system((10, 10, 10), np.eye(3) * 6.0, np.array([[0.1, 0.2, 0.3]]), seed=2,
+                           system_id=5)
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    psi = systems.random_orbitals(s, 2, ecut=5.0)
+    loss0, _ = osys.td_current(psi, [0.0], 0.1, (1.0, 0.0, 0.0), 0.02, 0)
+    _, jx = osys.current(psi, (0.0, 0.0, 0.0), 2.0)
+    assert loss0 == pytest.approx(jx[0], rel=1e-14, abs=1e-14)
+    # a static uniform A changes the current by -N_e A (no projector overlap with psi is needed)
+    loss_a, _ = osys.td_current(psi, [0.0], 0.1, (1.0, 0.0, 0.0), 0.02, 3)
+    assert np.isfinite(loss_a)
