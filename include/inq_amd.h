@@ -9,6 +9,7 @@
  *   hamiltonian::projector_all    (src/hamiltonian/projector_all.hpp:23-421)
  *   operations::exponential*      (src/operations/exponential.hpp:21-69)
  *   real_time::etrs               (src/real_time/etrs.hpp:22-82)
+ *   observables::current          (src/observables/current.hpp:25-53)
  * INQ's host code keeps its own containers and calls these functions with
  * plain device pointers.
  *
@@ -168,6 +169,65 @@ int inqamd_etrs_graph_launch(inqamd_graph_t g, const double A_t_cov[3], const do
 /* Step counter of a series graph (time = step * dt); stream ordered. */
 int inqamd_etrs_graph_set_step(inqamd_graph_t g, long step);
 void inqamd_graph_destroy(inqamd_graph_t g);
+
+/* ---- current and the adjoint of a propagation ----------------------------------
+ * observables::current (current.hpp:25-53) for bands of equal occupation occ:
+ *   J = occ sum_b Im <phi_b| g |phi_b>,  g = -i(grad + i(k+A)) + [V_nl, r]
+ * (gradient.hpp:43-48 with factor -1; projector_all::position_commutator,
+ * projector_all.hpp:319-403, with each KB coefficient applied once -- the
+ * reference multiplies the plain projection by it twice and never scales the
+ * r-weighted one, :362-366; applied once, J = -dE/dA exactly).  J_cov is the
+ * covariant vector current() returns, J_cart its Cartesian form
+ * (viewables.hpp:100-102).  nst <= max_bands.  HOST outputs: synchronises
+ * the handle's stream. */
+int inqamd_current(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, double occ,
+                   double J_cov[3], double J_cart[3]);
+
+/* Frozen-potential propagation and its adjoint.  The forward runs nsteps
+ * ETRS steps (etrs.hpp:22-82, Taylor order `order` per half step, exponent
+ * +i dt/2; the second half uses A(t + dt) as etrs.hpp:54 does) under
+ *   A(t) = direction * sum_{j=1..ncoef} c_j sin(j omega t)   (Cartesian)
+ * from psi0 at t = 0, the loop of real_time::propagate (propagate.hpp:93-131)
+ * without observables, and returns the loss L = J_cart(T) . direction
+ * (the current of current.hpp:25-53 at A(T), T = nsteps dt).  It keeps psi
+ * every checkpoint_every steps.  The backward returns dL/dc_j and dL/dV(r)
+ * by the discrete adjoint of exactly the forward's arithmetic: each segment
+ * is replayed from its checkpoint, one band slice at a time (bands are
+ * independent under a frozen potential), and the adjoint state is carried
+ * back through every Taylor half step.  The reference has no adjoint; these
+ * functions replace its finite-difference use of propagate.  k is the
+ * handle's k-point; V is the handle's scalar potential at forward time and
+ * must not change before the backward. */
+typedef struct inqamd_td *inqamd_td_t;
+typedef struct {
+  int nst;                  /* bands (<= max_bands) */
+  int nsteps;               /* ETRS steps */
+  double dt;                /* time step (atomic units) */
+  int order;                /* Taylor order of each half step, 1..8 */
+  double occ;               /* occupation of every band */
+  int ncoef;                /* number of series coefficients (<= 64) */
+  double omega;             /* base angular frequency */
+  double direction[3];      /* Cartesian polarisation of A and of the loss */
+  int checkpoint_every;     /* steps per checkpoint segment (>= 1) */
+  int band_slice;           /* bands per backward slice (0: all) */
+  int store_powers;         /* 1: keep H^j phi of a segment's half steps; 0: recompute */
+} inqamd_td_desc;
+
+/* Allocates checkpoints and adjoint workspaces (device memory). */
+int inqamd_td_create(inqamd_hamiltonian_t h, const inqamd_td_desc *desc, inqamd_td_t *out);
+/* psi0: DEVICE [N][ld]; coeff: HOST [ncoef]; psi_T: DEVICE [N][ld] or NULL;
+ * loss: HOST.  Synchronises the handle's stream. */
+int inqamd_td_forward(inqamd_td_t td, const double *coeff, const void *psi0, int ld,
+                      void *psi_T, double *loss);
+/* Gradients of dloss * L at the last forward's inputs: grad_coeff HOST
+ * [ncoef], grad_vloc DEVICE [N] (overwritten; may be NULL), lambda0 DEVICE
+ * [N][nst] or NULL: the adjoint state at t = 0 (dL = Re sum conj(lambda0)
+ * dpsi0 over grid points and bands).  Synchronises the handle's stream. */
+int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double *grad_vloc,
+                       void *lambda0);
+/* Device bytes held by the propagation object. */
+long inqamd_td_memory(inqamd_td_t td);
+void inqamd_td_destroy(inqamd_td_t td);
 
 /* ---- host-side system builders (no GPU needed) --------------------------------
  * containing_cube (containing_cube.hpp:20-45). */

@@ -391,12 +391,15 @@ static void release(handle &h) {
   for (int a = 0; a < 3; a++)
     if (h.axis[a].tw) (void)hipFree(h.axis[a].tw), h.axis[a].tw = nullptr;
   for (void *p : {(void *)h.V, (void *)h.kA, (void *)h.work, (void *)h.hn, (void *)h.hn2,
-                  (void *)h.proj, (void *)h.vnl, (void *)h.partial})
+                  (void *)h.proj, (void *)h.vnl, (void *)h.partial, (void *)h.mom_a,
+                  (void *)h.mom_b, (void *)h.obs})
     if (p) (void)hipFree(p);
   h.V = nullptr;
   h.kA = nullptr;
   h.work = h.hn = h.hn2 = h.proj = h.vnl = nullptr;
   h.partial = nullptr;
+  h.mom_a = h.mom_b = nullptr;
+  h.obs = nullptr;
   projector_free(h);
 }
 
@@ -480,6 +483,12 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
     h.proj = (cplx *)dalloc((size_t)h.pt.nproj * h.pt.max_nlm * h.max_bands * sizeof(cplx));
   if (h.pt.nproj > 0 && h.pt.nunique > 0 && h.three_sweep)
     h.vnl = (cplx *)dalloc((size_t)h.pt.nent * h.max_bands * sizeof(cplx));
+  if (h.pt.nproj > 0) {
+    const size_t mom = (size_t)h.pt.nproj * h.pt.max_nlm * 4 * h.max_bands * sizeof(cplx);
+    h.mom_a = (cplx *)dalloc(mom);
+    h.mom_b = (cplx *)dalloc(mom);
+  }
+  h.obs = (double *)dalloc(observables_scratch_doubles(h) * sizeof(double));
 }
 
 }  // namespace inqamd
@@ -490,39 +499,6 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
 using namespace inqamd;
 
 namespace {
-
-template <typename F>
-int guarded(inqamd_hamiltonian *h, F &&f) {
-  try {
-    f();
-    if (h) h->last_error.clear();
-    return INQAMD_OK;
-  } catch (const inqamd::error &e) {
-    if (h) h->last_error = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    if (h) h->last_error = "host allocation failed";
-    return INQAMD_ERR_MEMORY;
-  } catch (const std::exception &e) {
-    if (h) h->last_error = e.what();
-    return INQAMD_ERR_ARGUMENT;
-  }
-}
-
-void check_orbital(const handle &h, const void *p, const char *name) {
-  if (p == nullptr) throw error{ERR_ARGUMENT, std::string(name) + " is null"};
-  if (reinterpret_cast<uintptr_t>(p) % 16 != 0)
-    throw error{ERR_ARGUMENT, std::string(name) + " is not 16-byte aligned"};
-  (void)h;
-}
-
-void check_bands(const handle &h, int nst, int ld, bool needs_workspace) {
-  if (nst <= 0) throw error{ERR_ARGUMENT, "nst must be positive"};
-  if (ld < nst) throw error{ERR_ARGUMENT, "ld must be >= nst"};
-  if (needs_workspace && nst > h.max_bands)
-    throw error{ERR_ARGUMENT, "nst (" + std::to_string(nst) + ") exceeds max_bands (" +
-                                  std::to_string(h.max_bands) + ")"};
-}
 
 thread_local std::string g_create_error;
 

@@ -6,6 +6,8 @@
 #include "fft.hpp"
 
 #include <complex>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -92,6 +94,9 @@ struct inqamd_hamiltonian {
   inqamd::cplx *vnl = nullptr;  // [nent][max_bands] V_nl psi per ownership entry
   double *partial = nullptr;    // reduction partials
   long npartial = 0;
+  inqamd::cplx *mom_a = nullptr; // [nproj][max_nlm][4][max_bands] sphere moments
+  inqamd::cplx *mom_b = nullptr; // second operand of pair reductions
+  double *obs = nullptr;        // observable partials and result records
 
   inqamd::projector_tables pt;
 
@@ -133,9 +138,49 @@ void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, 
 void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst);
 void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
                           int nst, int slot);
+// td.hip: observables scratch of a handle (sizes from max_bands and tables)
+constexpr int OBS_GS_BLOCKS = 1024;  // blocks (and partial rows) of the G-space reductions
+constexpr int OBS_RECORD = 32;       // doubles per result record
+constexpr int OBS_NRECORDS = 16;
+long observables_scratch_doubles(const handle &h);
 // sphere[p][i][b] = sum_lm M[p][lm][i] P[p][lm][b]   (INQ-shaped project output)
 void projector_expand(handle &h, cplx *sphere, int nst);
 // hpsi(r, b) += sum_{(p,i) at r} e^{-i(k+A).r_pi} sphere[p][i][b]
 void projector_apply_sphere(handle &h, const cplx *sphere, cplx *hpsi, long ld, int nst, int slot);
+
+
+// C ABI plumbing shared by the translation units that export functions
+template <typename F>
+int guarded(inqamd_hamiltonian *h, F &&f) {
+  try {
+    f();
+    if (h) h->last_error.clear();
+    return OK;
+  } catch (const error &e) {
+    if (h) h->last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    if (h) h->last_error = "host allocation failed";
+    return ERR_MEMORY;
+  } catch (const std::exception &e) {
+    if (h) h->last_error = e.what();
+    return ERR_ARGUMENT;
+  }
+}
+
+inline void check_orbital(const handle &h, const void *p, const char *name) {
+  if (p == nullptr) throw error{ERR_ARGUMENT, std::string(name) + " is null"};
+  if (reinterpret_cast<uintptr_t>(p) % 16 != 0)
+    throw error{ERR_ARGUMENT, std::string(name) + " is not 16-byte aligned"};
+  (void)h;
+}
+
+inline void check_bands(const handle &h, int nst, int ld, bool needs_workspace) {
+  if (nst <= 0) throw error{ERR_ARGUMENT, "nst must be positive"};
+  if (ld < nst) throw error{ERR_ARGUMENT, "ld must be >= nst"};
+  if (needs_workspace && nst > h.max_bands)
+    throw error{ERR_ARGUMENT, "nst (" + std::to_string(nst) + ") exceeds max_bands (" +
+                                  std::to_string(h.max_bands) + ")"};
+}
 
 }  // namespace inqamd

@@ -57,6 +57,24 @@ class VectorPotentialSeries(ctypes.Structure):
     ]
 
 
+class TDDesc(ctypes.Structure):
+    """inqamd_td_desc."""
+
+    _fields_ = [
+        ("nst", ctypes.c_int),
+        ("nsteps", ctypes.c_int),
+        ("dt", ctypes.c_double),
+        ("order", ctypes.c_int),
+        ("occ", ctypes.c_double),
+        ("ncoef", ctypes.c_int),
+        ("omega", ctypes.c_double),
+        ("direction", ctypes.c_double * 3),
+        ("checkpoint_every", ctypes.c_int),
+        ("band_slice", ctypes.c_int),
+        ("store_powers", ctypes.c_int),
+    ]
+
+
 # (name, restype, argtypes) for every symbol of include/inq_amd.h
 SIGNATURES = [
     ("inqamd_hamiltonian_create", _i, [ctypes.POINTER(SystemDesc), _vp, ctypes.POINTER(_vp)]),
@@ -84,6 +102,12 @@ SIGNATURES = [
     ("inqamd_etrs_graph_launch", _i, [_vp, _c_dbl_p, _c_dbl_p]),
     ("inqamd_etrs_graph_set_step", _i, [_vp, ctypes.c_long]),
     ("inqamd_graph_destroy", None, [_vp]),
+    ("inqamd_current", _i, [_vp, _vp, _i, _i, _d, _c_dbl_p, _c_dbl_p]),
+    ("inqamd_td_create", _i, [_vp, ctypes.POINTER(TDDesc), ctypes.POINTER(_vp)]),
+    ("inqamd_td_forward", _i, [_vp, _c_dbl_p, _vp, _i, _vp, _c_dbl_p]),
+    ("inqamd_td_backward", _i, [_vp, _d, _c_dbl_p, _vp, _vp]),
+    ("inqamd_td_memory", ctypes.c_long, [_vp]),
+    ("inqamd_td_destroy", None, [_vp]),
     ("inqamd_containing_cube", _i, [_c_dbl_p, _c_int_p, _c_dbl_p, _d, _c_int_p, _c_int_p]),
     ("inqamd_sphere_build", _i, [_c_dbl_p, _c_int_p, _c_dbl_p, _d, _i, _vp, _vp, _vp, _c_int_p]),
     ("inqamd_gaussian_projector_matrix", _i, [_c_dbl_p, _i, _vp, _vp, _d, _i, _vp, _i]),

@@ -9,6 +9,7 @@ shape [nx, ny, nz, nst] (or [N, nst]), band index contiguous.
     vnl = ham.non_local(phi)                      # ks_hamiltonian.hpp:104-128
     sphere = ham.projectors_all().project(phi)    # projector_all.hpp:98-216
     ham.projectors_all().apply(sphere, vnl)       # projector_all.hpp:220-232
+    J_cov, J_cart = ham.current(phi, occ)         # observables/current.hpp:25-53
 """
 import ctypes
 
@@ -179,6 +180,15 @@ class ks_hamiltonian:
         eps = torch.empty(nst, dtype=torch.float64, device=phi.device)
         self._h.call("inqamd_band_energies", q, p, ctypes.c_void_p(eps.data_ptr()), nst, ld)
         return eps
+
+    def current(self, phi, occ=2.0):
+        """observables::current (current.hpp:25-53) for bands of equal
+        occupation: (J_cov, J_cart) as numpy vectors (synchronises)."""
+        p, nst, ld = _orbital_args(phi)
+        jc = (ctypes.c_double * 3)()
+        jx = (ctypes.c_double * 3)()
+        self._h.call("inqamd_current", p, nst, ld, ctypes.c_double(occ), jc, jx)
+        return np.array(list(jc)), np.array(list(jx))
 
     def close(self):
         self._h.close()
