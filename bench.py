@@ -32,6 +32,21 @@ def algorithmic_bytes_per_band(N, nst, sum_k):
     return 80.0 * N + 8.0 * N / nst + 48.0 * sum_k
 
 
+def kernel_design_bytes(N, nst, sum_k, nlm, chunks):
+    """Bytes each kernel of one Hpsi must move by design (its own compulsory
+    reads and writes), per launch: the projector's gather of psi and write of
+    V_nl psi per sphere entry (plus its tables), each sweep's read and write of
+    the band chunk, and the last sweep's extra reads of psi (V psi), V, the
+    ownership map and the V_nl psi entries."""
+    nb = nst / chunks
+    return {
+        "projector": 32.0 * sum_k * nst + (8.0 * nlm + 4.0 + 12.0 + 32.0) * sum_k,
+        "z_fwd": 32.0 * N * nb,
+        "xy_kin": 32.0 * N * nb,
+        "z_bwd_epi": 48.0 * N * nb + 12.0 * N + 16.0 * sum_k * nb,
+    }
+
+
 def cpu_baseline(s, psi, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -117,15 +132,39 @@ def main():
     N = s.npoints
     sum_k = int(np.sum(s.tables["sphere_size"]))
     b_h = algorithmic_bytes_per_band(N, nst, sum_k)
-    achieved = b_h * nst / (ms_per_step / 1e3) / 1e9
+    step_achieved = b_h * nst / (ms_per_step / 1e3) / 1e9
+    # per-kernel timing: the same Hpsi with HIP events around every launch on
+    # the handle's stream, averaged over the timed repetitions
+    reps = max(1, min(args.steps, 10))
+    tot = {}
+    for _ in range(reps):
+        for k, (ms_k, n_k) in ham.profile(d_psi, d_out).items():
+            t, n = tot.get(k, (0.0, 0))
+            tot[k] = (t + ms_k, n + n_k)
+    chunks = tot["z_fwd"][1] // reps
+    design = kernel_design_bytes(N, nst, sum_k, int(s.tables["max_nlm"]), chunks)
+    kernels = {}
+    for k, (t, n) in tot.items():
+        avg_ms = t / max(n, 1)
+        per_launch = design[k] if k != "projector" else design[k] / 2
+        kernels[k] = {"ms_per_hpsi": t / reps, "launches_per_hpsi": n // reps,
+                      "avg_launch_ms": avg_ms, "design_bytes_per_launch": per_launch,
+                      "GBs": per_launch / (avg_ms / 1e3) / 1e9}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_hpsi"])
+    log("per-kernel: " + ", ".join(f"{k} {v['ms_per_hpsi']:.3f} ms ({v['GBs']:.0f} GB/s)"
+                                   for k, v in kernels.items()))
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "hpsi_64_traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+        traffic = json.load(open(tfile)).get("bytes_per_dispatch", {}).get(dom)
+    achieved = kernels[dom]["GBs"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "launch": "one Hpsi of 256 bands (all its kernels, HIP events on its stream)",
-                "algorithmic_bytes_per_band": b_h}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "avg_launch_ms": kernels[dom]["avg_launch_ms"],
+                "design_bytes_per_launch": kernels[dom]["design_bytes_per_launch"],
+                "kernels": kernels,
+                "step": {"algorithmic_bytes_per_band": b_h, "achieved": step_achieved,
+                         "frac": step_achieved / HBM_PEAK_GBS}}
 
     result_host = d_out.cpu().numpy()
     cpu = None

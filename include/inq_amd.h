@@ -96,6 +96,13 @@ int inqamd_set_scalar_potential(inqamd_hamiltonian_t h, const double *vloc_devic
  * psi and hpsi must not alias. */
 int inqamd_hamiltonian_apply(inqamd_hamiltonian_t h, const void *psi, void *hpsi, int nst,
                              int ld);
+/* Diagnostic: one Hpsi (as inqamd_hamiltonian_apply, three-sweep path only)
+ * with HIP events around each kernel on the handle's stream.  ms[0]: the
+ * projector (phases + gather/MFMA/expand), ms[1..3]: the z-forward, x-y and
+ * z-backward sweeps summed over band chunks; launches[] the kernel launches
+ * each figure covers.  Synchronises.  No reference counterpart. */
+int inqamd_hamiltonian_profile(inqamd_hamiltonian_t h, const void *psi, void *hpsi, int nst,
+                               int ld, double ms[4], int launches[4]);
 /* ks_hamiltonian::non_local(orbital_set<real_space, complex>)
  * (ks_hamiltonian.hpp:104-128): vnlpsi = V_nl(k+A) psi. */
 int inqamd_non_local(inqamd_hamiltonian_t h, const void *psi, void *vnlpsi, int nst, int ld);
@@ -228,6 +235,37 @@ int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double 
 /* Device bytes held by the propagation object. */
 long inqamd_td_memory(inqamd_td_t td);
 void inqamd_td_destroy(inqamd_td_t td);
+
+/* ---- uneven batches ----------------------------------------------------------
+ * Hpsi of many independent systems (different grids, band counts and
+ * projector sets, each its own handle) in one call: every phase of Hpsi is a
+ * single persistent launch over the whole batch that drains a queue of
+ * per-system work items (projector blocks, then the three sweeps' tiles of
+ * band pieces packed into cache-sized chunks).  Results equal the
+ * one-system inqamd_hamiltonian_apply of each handle bit for bit.  Replaces
+ * the per-system loop a caller would write around ks_hamiltonian::operator()
+ * (ks_hamiltonian.hpp:132-152); the reference has no batched form.
+ * Systems whose grid has no three-sweep kernels run their one-system Hpsi
+ * inside the call.  Pointer arrays are HOST arrays of DEVICE pointers; work
+ * tables are rebuilt when they change.  Uses each handle's workspaces: a
+ * handle must not run other work concurrently with a batch call. */
+typedef struct inqamd_batch *inqamd_batch_t;
+int inqamd_batch_create(inqamd_hamiltonian_t const *handles, int nsys, void *stream,
+                        inqamd_batch_t *out);
+int inqamd_batch_apply(inqamd_batch_t b, const void *const *psi, void *const *hpsi,
+                       const int *nst, const int *ld);
+/* Tiling choices (the autotuner's knobs): W bytes per batch chunk, the y
+ * extent of full z tiles (8, 16 or 32 with 64/ty bands), whether band
+ * remainders use narrower tiles, and whether each sweep of a chunk is one
+ * launch over all its grids (0) or one launch per grid length (1). */
+int inqamd_batch_set_tuning(inqamd_batch_t b, long chunk_bytes, int ty_main,
+                            int narrow_remainders, int group_by_grid);
+int inqamd_batch_set_stream(inqamd_batch_t b, void *stream);
+/* Cache chunks, kernel launches per apply, band pieces and one-system
+ * fallbacks of the last work tables. */
+int inqamd_batch_info(inqamd_batch_t b, int *chunks, int *launches, int *pieces, int *fallback);
+const char *inqamd_batch_last_error(inqamd_batch_t b);
+void inqamd_batch_destroy(inqamd_batch_t b);
 
 /* ---- host-side system builders (no GPU needed) --------------------------------
  * containing_cube (containing_cube.hpp:20-45). */

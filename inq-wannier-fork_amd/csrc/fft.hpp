@@ -17,6 +17,10 @@ constexpr int FFT_BLOCK = 256;
 
 enum fft_mode : int { FFT_PLAIN = 0, FFT_KINETIC = 1, FFT_EPILOGUE = 2 };
 
+// Ownership-map word of a grid point: first entry << OWN_COUNT_BITS | entry count.
+constexpr int OWN_COUNT_BITS = 8;
+constexpr int OWN_MAX_COUNT = (1 << OWN_COUNT_BITS) - 1;
+
 struct fft_axis_plan {
   int len = 0;
   int nstages = 0;
@@ -90,7 +94,7 @@ struct fft3_args {
   const cplx *tw;    // W_L^k table
   double scale;      // 1/N
   // projector apply fused into the last sweep (own == nullptr: not fused):
-  // hpsi(p, b) += sum_{j < cnt} Qe[e0 + j][b] with own[p] = (e0 << 4) | cnt
+  // hpsi(p, b) += sum_{j < cnt} Qe[e0 + j][b] with own[p] = (e0 << OWN_COUNT_BITS) | cnt
   const int *own;
   const cplx *Qe;        // [nent][ldQ], offset to the chunk's first band
   long ldQ;
@@ -107,7 +111,8 @@ struct fft3_args {
 bool fft3_supported(int L, size_t max_lds);
 // Sets the kernels' LDS attributes for length L (once, before any launch).
 void fft3_prepare(int L);
-// hpsi = IFFT(1/2|G+k+A|^2 FFT(psi)) / N + V psi for the chunk's bands.
-void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream);
+// hpsi = IFFT(1/2|G+k+A|^2 FFT(psi)) / N + V psi for the chunk's bands: all
+// three sweeps (sweep < 0) or only sweep 0 (z forward), 1 (x-y) or 2 (z back).
+void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream, int sweep = -1);
 
 }  // namespace inqamd

@@ -1,214 +1,35 @@
-// Three-sweep local part of Hpsi (T + V) for cubic grids L x L x L.
-//
-//   z_fwd     psi [x][y][z][band] (INQ layout) -> W[band][z][x][y]
-//             tile: one x, 8 y, 8 bands, all z.  A band-fastest coalesced
-//             load into LDS, a z-FFT over the 64 (y, band) columns, and a
-//             last stage that writes 8 consecutive y (128 B) per (band, z).
-//   xy_kin    one (band, z) plane W[b][z][.][.] (contiguous L*L) per
-//             workgroup: x-FFT, y-FFT, x 1/2|G+k+A|^2, inverse y, inverse x,
-//             in a padded LDS plane; first/last stage touch HBM directly.
-//   z_bwd_epi inverse z-FFT from W, then a transposing write
-//             hpsi = FFT/N + V psi in INQ layout (psi, V prefetched into
-//             registers at kernel start).
-//
-// The projector's V_nl psi (precomputed per owned point) and the Taylor
-// accumulations acc += c * hpsi are added in the last sweep's write.
+// Three-sweep local part of Hpsi (T + V) for one system on a cubic grid
+// L x L x L: one launch per sweep over a band chunk (tile bodies in
+// fft3_tiles.hpp).  The projector's V_nl psi (precomputed per owned point)
+// and the Taylor accumulations acc += c * hpsi are added in the last sweep's
+// write.
 //
 // Traffic per band: 2 x 16N (z_fwd) + 2 x 16N (xy_kin) + 3 x 16N (z_bwd_epi)
 // = 112 N bytes, against 176 N for five pencil passes; run per band chunk
 // so W stays resident in the Infinity Cache between sweeps.
 
-#include "fft.hpp"
-#include "fft_dft.hpp"
-#include "fft_stage.hpp"
+#include "fft3_tiles.hpp"
 
 namespace inqamd {
 
-constexpr int Z_TY = 8;                 // y values per z tile
-constexpr int Z_TB = 8;                 // bands per z tile
-constexpr int Z_COLS = Z_TY * Z_TB;     // FFT columns per z tile
-constexpr int Z_NT = 512;
-static_assert(Z_NT % Z_TB == 0, "thread -> band mapping of the epilogue");
+constexpr int Z_TY1 = 8;  // z tile of the one-system kernels: 8 y x 8 bands
 
-constexpr int plane_threads(int L) {
-  int t = (L * L / 16 + 63) / 64 * 64;
-  return t < 64 ? 64 : (t > 1024 ? 1024 : t);
-}
-
-// ---------------------------------------------------------------------------
-// sweep 1: z forward, INQ layout -> band-major W
-// ---------------------------------------------------------------------------
 template <int R0, int R1, int R2>
 __global__ __launch_bounds__(Z_NT) void z_fwd_kernel(const fft3_args a) {
-  using P = plan3<R0, R1, R2>;
-  constexpr int L = P::LEN;
-  extern __shared__ cplx lds[];  // [L][Z_COLS], column = b * Z_TY + y
-
-  const int nyt = (L + Z_TY - 1) / Z_TY;
-  const int nbt = (a.nb + Z_TB - 1) / Z_TB;
-  long t = blockIdx.x;
-  const int bt = (int)(t % nbt);
-  t /= nbt;
-  const int yt = (int)(t % nyt);
-  const int x = (int)(t / nyt);
-  const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
-
-  // coalesced load: band fastest (128 B rows of psi)
-  for (int e = threadIdx.x; e < L * Z_COLS; e += Z_NT) {
-    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-    cplx v = mk(0.0, 0.0);
-    if (y0 + y < L && cb0 + b < a.nb)
-      v = a.psi[(((long)x * L + y0 + y) * L + z) * a.ld + cb0 + b];
-    lds[z * Z_COLS + b * Z_TY + y] = v;
-  }
-  __syncthreads();
-
-  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + c]; };
-  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + c] = v; };
-  auto gs = [&](int i, int c, cplx v, int, int) {
-    const int b = c / Z_TY, y = c % Z_TY;
-    if (y0 + y < L && cb0 + b < a.nb)
-      a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y] = v;
-  };
-  run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, -1, true, ll, ll, ls, gs, false);
+  extern __shared__ cplx lds[];
+  z_fwd_tile<R0, R1, R2, Z_TY1>(a, blockIdx.x, lds);
 }
 
-// ---------------------------------------------------------------------------
-// sweep 2: 2-D x-y transform of one plane with the kinetic factor
-// ---------------------------------------------------------------------------
 template <int R0, int R1, int R2>
 __global__ __launch_bounds__(plane_threads(R0 * R1 * R2)) void xy_kin_kernel(const fft3_args a) {
-  using P = plan3<R0, R1, R2>;
-  constexpr int L = P::LEN;
-  constexpr int NT = plane_threads(L);
-  constexpr int PAD = L + 1;  // row stride of the LDS plane (breaks bank aliasing)
-  extern __shared__ cplx lds[];  // [x][PAD]
-
-  const long plane = blockIdx.x;  // = band * L + z
-  const int z = (int)(plane % L);
-  cplx *W = a.W + plane * L * L;
-
-  // x-axis accessors: element (i = x, c = y)
-  auto gx = [&](int i, int c) -> cplx { return W[i * L + c]; };
-  auto lx = [&](int i, int c) -> cplx { return lds[i * PAD + c]; };
-  auto sx = [&](int i, int c, cplx v, int, int) { lds[i * PAD + c] = v; };
-  auto wx = [&](int i, int c, cplx v, int, int) { W[i * L + c] = v; };
-  // y-axis accessors: element (i = y, c = x)
-  auto ly = [&](int i, int c) -> cplx { return lds[c * PAD + i]; };
-  auto sy = [&](int i, int c, cplx v, int, int) { lds[c * PAD + i] = v; };
-
-  // 1/2 |G + k + A|^2 with G = m_x b0 + m_y b1 + m_z b2 (cartesian)
-  const double inv2pi = 0.15915494309189533576888376337251;
-  const int mz = symmetric_index(z, L);
-  double gz[3], b0[3], b1[3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const double kc =
-        (a.rlat[0 * 3 + d] * a.kA[0] + a.rlat[1 * 3 + d] * a.kA[1] + a.rlat[2 * 3 + d] * a.kA[2]) *
-        inv2pi;
-    gz[d] = mz * a.rlat[2 * 3 + d] + kc;
-    b0[d] = a.rlat[0 * 3 + d];
-    b1[d] = a.rlat[1 * 3 + d];
-  }
-  auto ky = [&](int i, int c, cplx v, int, int) {
-    const int my = symmetric_index(i, L), mx = symmetric_index(c, L);
-    const double q0 = mx * b0[0] + my * b1[0] + gz[0];
-    const double q1 = mx * b0[1] + my * b1[1] + gz[1];
-    const double q2 = mx * b0[2] + my * b1[2] + gz[2];
-    lds[c * PAD + i] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), v);
-  };
-
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, -1, false, gx, lx, sx, sx, true);  // x forward
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, -1, true, ly, ly, sy, ky, true);   // y forward, x T
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, +1, true, ly, ly, sy, sy, true);   // y backward
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, +1, true, lx, lx, sx, wx, false);  // x backward
+  extern __shared__ cplx lds[];
+  xy_kin_plane<R0, R1, R2>(a, blockIdx.x, lds);
 }
 
-// ---------------------------------------------------------------------------
-// sweep 3: z backward, x 1/N + V psi, band-major W -> INQ layout
-// ---------------------------------------------------------------------------
 template <int R0, int R1, int R2>
 __global__ __launch_bounds__(Z_NT) void z_bwd_epi_kernel(const fft3_args a) {
-  using P = plan3<R0, R1, R2>;
-  constexpr int L = P::LEN;
-  constexpr int NE = (L * Z_COLS + Z_NT - 1) / Z_NT;  // output elements per thread
   extern __shared__ cplx lds[];
-
-  const int nyt = (L + Z_TY - 1) / Z_TY;
-  const int nbt = (a.nb + Z_TB - 1) / Z_TB;
-  long t = blockIdx.x;
-  const int bt = (int)(t % nbt);
-  t /= nbt;
-  const int yt = (int)(t % nyt);
-  const int x = (int)(t / nyt);
-  const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
-
-  // prefetch V psi (and the ownership entries) of this thread's outputs
-  cplx pre[NE];
-  int own[NE];
-#pragma unroll
-  for (int k = 0; k < NE; k++) {
-    const int e = threadIdx.x + k * Z_NT;
-    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-    pre[k] = mk(0.0, 0.0);
-    own[k] = -1;
-    if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
-      const long p = ((long)x * L + y0 + y) * L + z;
-      const cplx ps = a.psi[p * a.ld + cb0 + b];
-      const double vp = a.V[p];
-      pre[k] = mk(vp * ps.x, vp * ps.y);
-      if (a.own) own[k] = a.own[p];
-    }
-  }
-
-  auto gl = [&](int i, int c) -> cplx {
-    const int b = c / Z_TY, y = c % Z_TY;
-    return (y0 + y < L && cb0 + b < a.nb) ? a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y]
-                                          : mk(0.0, 0.0);
-  };
-  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + c]; };
-  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + c] = v; };
-  run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, +1, false, gl, ll, ls, ls, true);
-
-  // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
-  // points, its ownership entries summed in fixed (sphere, point) order
-#pragma unroll
-  for (int k = 0; k < NE; k++) {
-    if (own[k] >= 0) {
-      const int e = threadIdx.x + k * Z_NT;
-      const int cb = cb0 + e % Z_TB;
-      const int e0 = own[k] >> 4, cnt = own[k] & 15;
-      for (int j = 0; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
-    }
-  }
-
-#pragma unroll
-  for (int k = 0; k < NE; k++) {
-    const int e = threadIdx.x + k * Z_NT;
-    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-    if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
-      const long p = ((long)x * L + y0 + y) * L + z;
-      const int cb = cb0 + b;
-      cplx v = cscale(a.scale, lds[z * Z_COLS + b * Z_TY + y]);
-      v.x += pre[k].x;
-      v.y += pre[k].y;
-      a.hpsi[p * a.hld + cb] = v;
-      if (a.acc1) {
-        cplx *d = a.acc1 + p * a.acc1_ld + cb;
-        cplx o = *d;
-        o.x += a.c1re * v.x - a.c1im * v.y;
-        o.y += a.c1re * v.y + a.c1im * v.x;
-        *d = o;
-      }
-      if (a.acc2) {
-        cplx *d = a.acc2 + p * a.acc2_ld + cb;
-        cplx o = *d;
-        o.x += a.c2re * v.x - a.c2im * v.y;
-        o.y += a.c2re * v.y + a.c2im * v.x;
-        *d = o;
-      }
-    }
-  }
+  z_bwd_epi_tile<R0, R1, R2, Z_TY1>(a, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,35 +51,27 @@ static void prepare3() {
 }
 
 template <int R0, int R1, int R2>
-static void launch3(const fft3_args &a, hipStream_t stream) {
+static void launch3(const fft3_args &a, hipStream_t stream, int sweep) {
   constexpr int L = R0 * R1 * R2;
-  const long nyt = (L + Z_TY - 1) / Z_TY;
-  const long nbt = (a.nb + Z_TB - 1) / Z_TB;
+  const long ztiles = z_tiles<Z_TY1>(L, a.nb);
   const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
   const size_t plds = (size_t)L * (L + 1) * sizeof(cplx);
-  hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2>), dim3((unsigned)(L * nyt * nbt)), dim3(Z_NT), zlds,
-                     stream, a);
-  INQAMD_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL((xy_kin_kernel<R0, R1, R2>), dim3((unsigned)((long)a.nb * L)),
-                     dim3(plane_threads(L)), plds, stream, a);
-  INQAMD_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2>), dim3((unsigned)(L * nyt * nbt)), dim3(Z_NT),
-                     zlds, stream, a);
-  INQAMD_HIP_CHECK(hipGetLastError());
+  if (sweep < 0 || sweep == 0) {
+    hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2>), dim3((unsigned)ztiles), dim3(Z_NT), zlds,
+                       stream, a);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+  if (sweep < 0 || sweep == 1) {
+    hipLaunchKernelGGL((xy_kin_kernel<R0, R1, R2>), dim3((unsigned)((long)a.nb * L)),
+                       dim3(plane_threads(L)), plds, stream, a);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+  if (sweep < 0 || sweep == 2) {
+    hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2>), dim3((unsigned)ztiles), dim3(Z_NT), zlds,
+                       stream, a);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
 }
-
-#define INQAMD_FFT3_TABLE(X) \
-  X(24, 8, 3, 1)             \
-  X(30, 2, 3, 5)             \
-  X(32, 8, 4, 1)             \
-  X(36, 4, 3, 3)             \
-  X(40, 8, 5, 1)             \
-  X(48, 8, 2, 3)             \
-  X(60, 4, 3, 5)             \
-  X(64, 8, 8, 1)             \
-  X(72, 8, 3, 3)             \
-  X(80, 8, 2, 5)             \
-  X(96, 8, 4, 3)
 
 bool fft3_supported(int L, size_t max_lds) {
   switch (L) {
@@ -281,10 +94,10 @@ void fft3_prepare(int L) {
   }
 }
 
-void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream) {
+void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream, int sweep) {
   switch (L) {
 #define INQAMD_FFT3_CASE(LL, R0, R1, R2) \
-  case LL: launch3<R0, R1, R2>(a, stream); return;
+  case LL: launch3<R0, R1, R2>(a, stream, sweep); return;
     INQAMD_FFT3_TABLE(INQAMD_FFT3_CASE)
 #undef INQAMD_FFT3_CASE
     default: throw error{ERR_UNSUPPORTED, "three-sweep Hpsi: unsupported grid length"};

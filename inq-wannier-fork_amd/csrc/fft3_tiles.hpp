@@ -1,0 +1,226 @@
+// Tile bodies of the three-sweep local Hpsi (T + V) for cubic grids L^3,
+// shared by the one-system kernels (fft3.hip) and the uneven-batch
+// work-queue kernels (batch.hip).
+//
+//   z_fwd_tile     psi [x][y][z][band] (INQ layout) -> W[band][z][x][y]
+//                  tile: one x, TY y, TB = 64/TY bands, all z.  A band-fastest
+//                  load into LDS, a z-FFT over the 64 (y, band) columns, and a
+//                  last stage that writes TY consecutive y per (band, z).
+//   xy_kin_plane   one (band, z) plane W[b][z][.][.] (contiguous L*L):
+//                  x-FFT, y-FFT, x 1/2|G+k+A|^2, inverse y, inverse x, in a
+//                  padded LDS plane; first/last stage touch HBM directly.
+//   z_bwd_epi_tile inverse z-FFT from W, then a transposing write
+//                  hpsi = FFT/N + V psi (+ V_nl psi of owned points, + Taylor
+//                  accumulations) in INQ layout.
+#pragma once
+
+#include "fft.hpp"
+#include "fft_dft.hpp"
+#include "fft_stage.hpp"
+
+namespace inqamd {
+
+constexpr int Z_COLS = 64;  // FFT columns per z tile: TY y values x TB bands
+constexpr int Z_NT = 512;
+
+constexpr int plane_threads(int L) {
+  int t = (L * L / 16 + 63) / 64 * 64;
+  return t < 64 ? 64 : (t > 1024 ? 1024 : t);
+}
+
+// ---------------------------------------------------------------------------
+// sweep 1: z forward, INQ layout -> band-major W
+// ---------------------------------------------------------------------------
+template <int R0, int R1, int R2, int Z_TY>
+__device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds) {
+  using P = plan3<R0, R1, R2>;
+  constexpr int L = P::LEN;
+  constexpr int Z_TB = Z_COLS / Z_TY;
+  // lds: [L][Z_COLS], column = b * Z_TY + y
+
+  const int nyt = (L + Z_TY - 1) / Z_TY;
+  const int nbt = (a.nb + Z_TB - 1) / Z_TB;
+  const int bt = (int)(t % nbt);
+  t /= nbt;
+  const int yt = (int)(t % nyt);
+  const int x = (int)(t / nyt);
+  const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
+
+  // coalesced load: band fastest (TB * 16 B runs of psi)
+  for (int e = threadIdx.x; e < L * Z_COLS; e += Z_NT) {
+    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+    cplx v = mk(0.0, 0.0);
+    if (y0 + y < L && cb0 + b < a.nb)
+      v = a.psi[(((long)x * L + y0 + y) * L + z) * a.ld + cb0 + b];
+    lds[z * Z_COLS + b * Z_TY + y] = v;
+  }
+  __syncthreads();
+
+  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + c]; };
+  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + c] = v; };
+  auto gs = [&](int i, int c, cplx v, int, int) {
+    const int b = c / Z_TY, y = c % Z_TY;
+    if (y0 + y < L && cb0 + b < a.nb)
+      a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y] = v;
+  };
+  run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, -1, true, ll, ll, ls, gs, false);
+}
+
+// ---------------------------------------------------------------------------
+// sweep 2: 2-D x-y transform of one plane with the kinetic factor
+// ---------------------------------------------------------------------------
+template <int R0, int R1, int R2>
+__device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cplx *lds) {
+  using P = plan3<R0, R1, R2>;
+  constexpr int L = P::LEN;
+  constexpr int NT = plane_threads(L);
+  constexpr int PAD = L + 1;  // row stride of the LDS plane (breaks bank aliasing)
+  // lds: [x][PAD]; plane = band * L + z
+  const int z = (int)(plane % L);
+  cplx *W = a.W + plane * L * L;
+
+  // x-axis accessors: element (i = x, c = y)
+  auto gx = [&](int i, int c) -> cplx { return W[i * L + c]; };
+  auto lx = [&](int i, int c) -> cplx { return lds[i * PAD + c]; };
+  auto sx = [&](int i, int c, cplx v, int, int) { lds[i * PAD + c] = v; };
+  auto wx = [&](int i, int c, cplx v, int, int) { W[i * L + c] = v; };
+  // y-axis accessors: element (i = y, c = x)
+  auto ly = [&](int i, int c) -> cplx { return lds[c * PAD + i]; };
+  auto sy = [&](int i, int c, cplx v, int, int) { lds[c * PAD + i] = v; };
+
+  // 1/2 |G + k + A|^2 with G = m_x b0 + m_y b1 + m_z b2 (cartesian)
+  const double inv2pi = 0.15915494309189533576888376337251;
+  const int mz = symmetric_index(z, L);
+  double gz[3], b0[3], b1[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double kc =
+        (a.rlat[0 * 3 + d] * a.kA[0] + a.rlat[1 * 3 + d] * a.kA[1] + a.rlat[2 * 3 + d] * a.kA[2]) *
+        inv2pi;
+    gz[d] = mz * a.rlat[2 * 3 + d] + kc;
+    b0[d] = a.rlat[0 * 3 + d];
+    b1[d] = a.rlat[1 * 3 + d];
+  }
+  auto ky = [&](int i, int c, cplx v, int, int) {
+    const int my = symmetric_index(i, L), mx = symmetric_index(c, L);
+    const double q0 = mx * b0[0] + my * b1[0] + gz[0];
+    const double q1 = mx * b0[1] + my * b1[1] + gz[1];
+    const double q2 = mx * b0[2] + my * b1[2] + gz[2];
+    lds[c * PAD + i] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), v);
+  };
+
+  run_plan<P, R0, R1, R2, L, NT>(a.tw, -1, false, gx, lx, sx, sx, true);  // x forward
+  run_plan<P, R0, R1, R2, L, NT>(a.tw, -1, true, ly, ly, sy, ky, true);   // y forward, x T
+  run_plan<P, R0, R1, R2, L, NT>(a.tw, +1, true, ly, ly, sy, sy, true);   // y backward
+  run_plan<P, R0, R1, R2, L, NT>(a.tw, +1, true, lx, lx, sx, wx, false);  // x backward
+}
+
+// ---------------------------------------------------------------------------
+// sweep 3: z backward, x 1/N + V psi, band-major W -> INQ layout
+// ---------------------------------------------------------------------------
+template <int R0, int R1, int R2, int Z_TY>
+__device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx *lds) {
+  using P = plan3<R0, R1, R2>;
+  constexpr int L = P::LEN;
+  constexpr int Z_TB = Z_COLS / Z_TY;
+  static_assert(Z_NT % Z_TB == 0, "thread -> band mapping of the epilogue");
+  constexpr int NE = (L * Z_COLS + Z_NT - 1) / Z_NT;  // output elements per thread
+
+  const int nyt = (L + Z_TY - 1) / Z_TY;
+  const int nbt = (a.nb + Z_TB - 1) / Z_TB;
+  const int bt = (int)(t % nbt);
+  t /= nbt;
+  const int yt = (int)(t % nyt);
+  const int x = (int)(t / nyt);
+  const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
+
+  // prefetch V psi (and the ownership entries) of this thread's outputs
+  cplx pre[NE];
+  int own[NE];
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    const int e = threadIdx.x + k * Z_NT;
+    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+    pre[k] = mk(0.0, 0.0);
+    own[k] = -1;
+    if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
+      const long p = ((long)x * L + y0 + y) * L + z;
+      const cplx ps = a.psi[p * a.ld + cb0 + b];
+      const double vp = a.V[p];
+      pre[k] = mk(vp * ps.x, vp * ps.y);
+      if (a.own) own[k] = a.own[p];
+    }
+  }
+
+  auto gl = [&](int i, int c) -> cplx {
+    const int b = c / Z_TY, y = c % Z_TY;
+    return (y0 + y < L && cb0 + b < a.nb) ? a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y]
+                                          : mk(0.0, 0.0);
+  };
+  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + c]; };
+  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + c] = v; };
+  run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, +1, false, gl, ll, ls, ls, true);
+
+  // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
+  // points, its ownership entries summed in fixed (sphere, point) order
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    if (own[k] >= 0) {
+      const int e = threadIdx.x + k * Z_NT;
+      const int cb = cb0 + e % Z_TB;
+      const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
+      for (int j = 0; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    const int e = threadIdx.x + k * Z_NT;
+    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+    if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
+      const long p = ((long)x * L + y0 + y) * L + z;
+      const int cb = cb0 + b;
+      cplx v = cscale(a.scale, lds[z * Z_COLS + b * Z_TY + y]);
+      v.x += pre[k].x;
+      v.y += pre[k].y;
+      a.hpsi[p * a.hld + cb] = v;
+      if (a.acc1) {
+        cplx *d = a.acc1 + p * a.acc1_ld + cb;
+        cplx o = *d;
+        o.x += a.c1re * v.x - a.c1im * v.y;
+        o.y += a.c1re * v.y + a.c1im * v.x;
+        *d = o;
+      }
+      if (a.acc2) {
+        cplx *d = a.acc2 + p * a.acc2_ld + cb;
+        cplx o = *d;
+        o.x += a.c2re * v.x - a.c2im * v.y;
+        o.y += a.c2re * v.y + a.c2im * v.x;
+        *d = o;
+      }
+    }
+  }
+}
+
+// Grid lengths with three-sweep kernels: (L, radices of the plan)
+#define INQAMD_FFT3_TABLE(X) \
+  X(24, 8, 3, 1)             \
+  X(30, 2, 3, 5)             \
+  X(32, 8, 4, 1)             \
+  X(36, 4, 3, 3)             \
+  X(40, 8, 5, 1)             \
+  X(48, 8, 2, 3)             \
+  X(60, 4, 3, 5)             \
+  X(64, 8, 8, 1)             \
+  X(72, 8, 3, 3)             \
+  X(80, 8, 2, 5)             \
+  X(96, 8, 4, 3)
+
+// Work items of a band chunk of nb bands
+template <int Z_TY>
+inline long z_tiles(int L, int nb) {
+  constexpr int Z_TB = Z_COLS / Z_TY;
+  return (long)L * ((L + Z_TY - 1) / Z_TY) * ((nb + Z_TB - 1) / Z_TB);
+}
+
+}  // namespace inqamd

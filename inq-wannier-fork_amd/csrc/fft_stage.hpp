@@ -4,7 +4,8 @@
 // butterfly j = w / B of column w % B; inputs j + r*LEN/R come through the
 // `ld(i, c)` functor, outputs go to (j/NS)*NS*R + j%NS + r*NS through
 // `st(i, c, v, q, r)` (q, r: the thread's butterfly slot).  Reading LDS and
-// writing LDS in place needs `sync_mid` between the two phases.
+// writing LDS in place needs `sync_mid` between the two phases.  Threads at
+// or above NT (a workgroup sized for a longer plan) only join the barriers.
 #pragma once
 
 #include "fft_dft.hpp"
@@ -33,7 +34,7 @@ __device__ __forceinline__ void pstage(const cplx *__restrict__ tw, int sign, LD
 #pragma unroll
   for (int q = 0; q < NBF; q++) {
     const int w = threadIdx.x + q * NT;
-    if (NBF * NT == NB * B || w < NB * B) {
+    if ((int)threadIdx.x < NT && (NBF * NT == NB * B || w < NB * B)) {
       const int j = w / B, bb = w % B, k = j % NS;
 #pragma unroll
       for (int r = 0; r < R; r++) v[q][r] = ld(j + r * NB, bb);
@@ -48,7 +49,7 @@ __device__ __forceinline__ void pstage(const cplx *__restrict__ tw, int sign, LD
 #pragma unroll
   for (int q = 0; q < NBF; q++) {
     const int w = threadIdx.x + q * NT;
-    if (NBF * NT == NB * B || w < NB * B) {
+    if ((int)threadIdx.x < NT && (NBF * NT == NB * B || w < NB * B)) {
       const int j = w / B, bb = w % B, k = j % NS;
       const int base = (j / NS) * NS * R + k;
 #pragma unroll

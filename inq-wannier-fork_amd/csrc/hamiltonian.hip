@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <complex>
 #include <new>
+#include <vector>
 
 namespace inqamd {
 
@@ -202,7 +203,7 @@ void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int ns
 // Taylor accumulations `acc` (offset to b0) in its last sweep; the pencil
 // path leaves both to the caller.
 static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
-                         int slot, int b0, long ldP, const taylor_acc *acc) {
+                         int slot, int b0, long ldP, const taylor_acc *acc, int sweep = -1) {
   if (h.three_sweep) {
     fft3_args a{};
     a.psi = psi;
@@ -234,7 +235,7 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       a.c2re = acc->c2re;
       a.c2im = acc->c2im;
     }
-    fft3_local_hamiltonian(a, h.n[0], h.stream);
+    fft3_local_hamiltonian(a, h.n[0], h.stream, sweep);
     return;
   }
   cplx *w = h.work;
@@ -708,6 +709,60 @@ int inqamd_exponential_2_for_1(inqamd_hamiltonian_t h, double z1re, double z1im,
     INQAMD_REQUIRE(phi != expphi, "phi and expphi must not alias");
     exponential_2_for_1(*h, std::complex<double>(z1re, z1im), std::complex<double>(z2re, z2im),
                         order, (cplx *)phi, (cplx *)expphi, ld, nst, KA_CURRENT);
+  });
+}
+
+int inqamd_hamiltonian_profile(inqamd_hamiltonian_t h, const void *psi, void *hpsi, int nst,
+                               int ld, double ms[4], int launches[4]) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_orbital(*h, hpsi, "hpsi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(ms && launches, "outputs are null");
+    if (!h->three_sweep)
+      throw error{ERR_UNSUPPORTED, "per-sweep timing needs the three-sweep Hpsi (cubic grid)"};
+    const int chunk = band_chunk(*h, nst);
+    const int nchunk = (nst + chunk - 1) / chunk;
+    std::vector<hipEvent_t> ev(2 + 3 * nchunk, nullptr);
+    auto cleanup = [&] {
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    };
+    try {
+      for (auto &e : ev) INQAMD_HIP_CHECK(hipEventCreate(&e));
+      INQAMD_HIP_CHECK(hipEventRecord(ev[0], h->stream));
+      projector_phases(*h, KA_CURRENT);
+      projector_project_expand(*h, (const cplx *)psi, ld, nst, h->vnl, nst, false);
+      INQAMD_HIP_CHECK(hipEventRecord(ev[1], h->stream));
+      int k = 1;
+      for (int b0 = 0; b0 < nst; b0 += chunk) {
+        const int nb = nst - b0 < chunk ? nst - b0 : chunk;
+        for (int sweep = 0; sweep < 3; sweep++) {
+          local_passes(*h, (const cplx *)psi + b0, ld, (cplx *)hpsi + b0, ld, nb, KA_CURRENT, b0,
+                       nst, nullptr, sweep);
+          INQAMD_HIP_CHECK(hipEventRecord(ev[++k], h->stream));
+        }
+      }
+      INQAMD_HIP_CHECK(hipEventSynchronize(ev[k]));
+      float t = 0.0f;
+      INQAMD_HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+      ms[0] = t;
+      launches[0] = h->pt.nproj > 0 ? 2 : 0;
+      for (int sweep = 0; sweep < 3; sweep++) {
+        ms[1 + sweep] = 0.0;
+        launches[1 + sweep] = nchunk;
+      }
+      for (int c = 0; c < nchunk; c++)
+        for (int sweep = 0; sweep < 3; sweep++) {
+          INQAMD_HIP_CHECK(hipEventElapsedTime(&t, ev[1 + 3 * c + sweep], ev[2 + 3 * c + sweep]));
+          ms[1 + sweep] += t;
+        }
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
   });
 }
 

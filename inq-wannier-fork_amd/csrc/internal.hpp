@@ -38,7 +38,7 @@ struct projector_tables {
   cplx *phase_sph = nullptr;  // [nproj][max_sphere]
   cplx *phase_ent = nullptr;  // [nent]
   int *ent = nullptr;         // [nproj][max_sphere] ownership entry of each sphere point
-  int *own = nullptr;         // [N] (first entry << 4) | count for owned points, -1 otherwise
+  int *own = nullptr;         // [N] (first entry << OWN_COUNT_BITS) | count, -1 if not owned
 };
 
 struct project_expand_args {

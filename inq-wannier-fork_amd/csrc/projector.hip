@@ -8,7 +8,7 @@
 // band, which sums its contributors in a fixed (sphere, point) order -- no
 // atomics, bitwise reproducible (replaces the atomic scatter of :225-231).
 
-#include "internal.hpp"
+#include "projector_dev.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -140,24 +140,12 @@ __global__ __launch_bounds__(256) void apply_kernel(const apply_args a) {
 }
 
 // e^{+i(k+A).r} for every sphere point and e^{-i(k+A).r} for every ownership
-// entry (the polar(1, +-dot(k, pos)) of projector_all.hpp:109, :228).
+// entry.
 __global__ __launch_bounds__(256) void phase_kernel(const float *pos, const int *uent, int nsph,
                                                     int nent, const double *kA, cplx *phase_sph,
                                                     cplx *phase_ent) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const double k0 = kA[0], k1 = kA[1], k2 = kA[2];
-  if (g < nsph) {
-    const float *r = pos + (long)g * 3;
-    double sn, cs;
-    sincos(k0 * (double)r[0] + k1 * (double)r[1] + k2 * (double)r[2], &sn, &cs);
-    phase_sph[g] = mk(cs, sn);
-  }
-  if (g < nent) {
-    const float *r = pos + (long)uent[g] * 3;
-    double sn, cs;
-    sincos(k0 * (double)r[0] + k1 * (double)r[1] + k2 * (double)r[2], &sn, &cs);
-    phase_ent[g] = mk(cs, -sn);
-  }
+  phase_item(pos, uent, nsph, nent, kA, phase_sph, phase_ent,
+             blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 struct expand_args {
@@ -258,8 +246,12 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
   for (size_t u = 0; u < upoint.size(); u++) {
     owner[upoint[u]] = (int)u;
     const int cnt = uoff[u + 1] - uoff[u];
-    if (cnt > 15) throw error{ERR_UNSUPPORTED, "more than 15 projector spheres share a grid point"};
-    own[upoint[u]] = (uoff[u] << 4) | cnt;
+    if (cnt > OWN_MAX_COUNT)
+      throw error{ERR_UNSUPPORTED, "more than " + std::to_string(OWN_MAX_COUNT) +
+                                       " projector sphere points share a grid point"};
+    if (uoff[u] >= (1 << (31 - OWN_COUNT_BITS)))
+      throw error{ERR_UNSUPPORTED, "too many projector sphere points for the ownership map"};
+    own[upoint[u]] = (uoff[u] << OWN_COUNT_BITS) | cnt;
   }
   for (size_t e = 0; e < uent.size(); e++) ent[uent[e]] = (int)e;
   t.own = upload(own);

@@ -88,6 +88,7 @@ SIGNATURES = [
     ("inqamd_set_scalar_potential", _i, [_vp, _vp]),
     ("inqamd_hamiltonian_apply", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_non_local", _i, [_vp, _vp, _vp, _i, _i]),
+    ("inqamd_hamiltonian_profile", _i, [_vp, _vp, _vp, _i, _i, _c_dbl_p, _c_int_p]),
     ("inqamd_transform_to_fourier", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_transform_to_real", _i, [_vp, _vp, _vp, _i, _i, _i]),
     ("inqamd_projector_project", _i, [_vp, _vp, _vp, _i, _i]),
@@ -108,6 +109,13 @@ SIGNATURES = [
     ("inqamd_td_backward", _i, [_vp, _d, _c_dbl_p, _vp, _vp]),
     ("inqamd_td_memory", ctypes.c_long, [_vp]),
     ("inqamd_td_destroy", None, [_vp]),
+    ("inqamd_batch_create", _i, [_vp, _i, _vp, ctypes.POINTER(_vp)]),
+    ("inqamd_batch_apply", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("inqamd_batch_set_tuning", _i, [_vp, ctypes.c_long, _i, _i, _i]),
+    ("inqamd_batch_set_stream", _i, [_vp, _vp]),
+    ("inqamd_batch_info", _i, [_vp, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
+    ("inqamd_batch_last_error", ctypes.c_char_p, [_vp]),
+    ("inqamd_batch_destroy", None, [_vp]),
     ("inqamd_containing_cube", _i, [_c_dbl_p, _c_int_p, _c_dbl_p, _d, _c_int_p, _c_int_p]),
     ("inqamd_sphere_build", _i, [_c_dbl_p, _c_int_p, _c_dbl_p, _d, _i, _vp, _vp, _vp, _c_int_p]),
     ("inqamd_gaussian_projector_matrix", _i, [_c_dbl_p, _i, _vp, _vp, _d, _i, _vp, _i]),

@@ -164,6 +164,17 @@ class ks_hamiltonian:
         self._h.call("inqamd_hamiltonian_apply", p, q, nst, ld)
         return out
 
+    def profile(self, phi, out):
+        """One Hpsi into `out` with per-kernel HIP event timing:
+        {name: (ms, launches)} for the projector and the three sweeps."""
+        p, nst, ld = _orbital_args(phi)
+        q, _, _ = _orbital_args(out)
+        ms = (ctypes.c_double * 4)()
+        nl = (ctypes.c_int * 4)()
+        self._h.call("inqamd_hamiltonian_profile", p, q, nst, ld, ms, nl)
+        names = ("projector", "z_fwd", "xy_kin", "z_bwd_epi")
+        return {n: (ms[i], nl[i]) for i, n in enumerate(names)}
+
     def non_local(self, phi):
         p, nst, ld = _orbital_args(phi)
         out = torch.empty_like(phi)

@@ -154,6 +154,39 @@ def diamond_system(supercell=1, grid=None, seed=0, system_id=0, with_projectors=
                        with_projectors)
 
 
+def uneven_batch_layout(seed=None):
+    """(grid, bands, atoms) of every system of the uneven batch, all drawn from
+    one generator keyed (seed, 1000, 3): grid uniform over the listed sizes,
+    bands round(exp(U(ln lo, ln hi))), atoms uniform integers in [lo, hi]."""
+    sp = spec()
+    cfg = sp["configs"]["uneven_batch"]
+    seed = sp["seed"] if seed is None else seed
+    g = rng(seed, 1000, 3)
+    grids = np.array(cfg["grids"])
+    blo, bhi = cfg["bands_log_uniform"]
+    alo, ahi = cfg["atoms_uniform"]
+    out = []
+    for _ in range(cfg["systems"]):
+        L = int(g.choice(grids))
+        nst = int(round(np.exp(g.uniform(np.log(blo), np.log(bhi)))))
+        na = int(g.integers(alo, ahi + 1))
+        out.append((L, nst, na))
+    return out
+
+
+def uneven_batch_system(index, grid, natoms, seed=None, with_projectors=True):
+    """System `index` of the uneven batch: a cubic cell a = grid * h (the
+    config cells' spacing h = a0 / points_per_a0), atoms at uniform fractional
+    positions from (seed, 100 + index, 2)."""
+    sp = spec()
+    seed = sp["seed"] if seed is None else seed
+    h = sp["cell"]["a0_bohr"] / sp["cell"]["points_per_a0"]
+    lattice = np.eye(3) * (grid * h)
+    frac = rng(seed, 100 + index, 2).uniform(0.0, 1.0, size=(natoms, 3))
+    return make_system((grid, grid, grid), lattice, frac @ lattice, seed, 100 + index,
+                       with_projectors)
+
+
 def plane_wave_mask(sysm, ecut):
     """Boolean [nx,ny,nz] of G with |G|^2/2 <= ecut (G = sum_d m_d b_d)."""
     b = 2.0 * np.pi * np.linalg.inv(sysm.lattice).T

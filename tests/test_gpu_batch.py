@@ -1,0 +1,123 @@
+"""GPU tests of the uneven-batch Hpsi.
+
+The batch runs the same tile bodies and projector blocks as the one-system
+Hpsi, so its output must equal each system's own ks_hamiltonian(phi) bit for
+bit, under every tiling choice; one system is also checked against the
+oracle (1e-12 relative).
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from inq_amd import batch, hamiltonian, systems
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a, np.complex128)).to(device)
+
+
+# (grid, bands, atoms): band remainders of every size class, many overlapping
+# spheres in a small cell, a non-cubic grid and a grid outside the batch
+# kernels' table (both run the one-system Hpsi inside the call)
+SMALL = [((24, 24, 24), 3, 2), ((30, 30, 30), 13, 5), ((36, 36, 36), 8, 3),
+         ((24, 24, 24), 21, 40), ((40, 40, 40), 5, 1), ((16, 18, 20), 3, 2), ((80, 80, 80), 2, 1),
+         ((32, 32, 32), 17, 0)]
+
+
+@pytest.fixture(scope="module")
+def small_batch(device):
+    sysl, hams, phis = [], [], []
+    for i, (n, nst, na) in enumerate(SMALL):
+        h = 10.26 / 32
+        lat = np.diag([n[0] * h, n[1] * h, n[2] * h])
+        frac = systems.rng(5, 300 + i, 2).uniform(0, 1, size=(na, 3))
+        s = systems.make_system(n, lat, frac @ lat, seed=5, system_id=300 + i,
+                                with_projectors=na > 0)
+        psi = systems.random_orbitals(s, nst, seed=5, system_id=300 + i, ecut=8.0)
+        sysl.append((s, psi))
+        hams.append(hamiltonian.ks_hamiltonian(s, nst))
+        phis.append(to_dev(psi, device))
+    refs = [h(p) for h, p in zip(hams, phis)]
+    torch.cuda.synchronize()
+    refs = [r.cpu().numpy() for r in refs]
+    yield sysl, hams, phis, refs
+    for h in hams:
+        h.close()
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(),
+    dict(group_by_grid=True),
+    dict(ty_main=16, narrow_remainders=False),
+    dict(ty_main=32, group_by_grid=True),
+    dict(chunk_bytes=1 << 20),
+    dict(chunk_bytes=1 << 20, group_by_grid=True, narrow_remainders=False),
+])
+def test_batch_equals_per_system_bitwise(device, small_batch, tuning):
+    say("batch small", tuning)
+    sysl, hams, phis, refs = small_batch
+    b = batch.hamiltonian_batch(hams)
+    b.set_tuning(**tuning)
+    outs = b(phis)
+    torch.cuda.synchronize()
+    info = b.info()
+    assert info["fallback"] == 2
+    for i, (o, r) in enumerate(zip(outs, refs)):
+        assert np.array_equal(o.cpu().numpy(), r), (i, tuning)
+    # a second call reuses the work tables
+    outs2 = b(phis, outs)
+    torch.cuda.synchronize()
+    for o, r in zip(outs2, refs):
+        assert np.array_equal(o.cpu().numpy(), r)
+    b.close()
+
+
+def test_batch_member_matches_oracle(device, small_batch):
+    say("batch member vs oracle")
+    sysl, hams, phis, _ = small_batch
+    b = batch.hamiltonian_batch(hams)
+    outs = b(phis)
+    torch.cuda.synchronize()
+    for i in (1, 3):
+        s, psi = sysl[i]
+        ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(psi)
+        assert relerr(outs[i].cpu().numpy(), ref) < 1e-12
+    b.close()
+
+
+def test_uneven_batch_config_matches_per_system(device):
+    """The full 24-system uneven batch: batched output equals per-system Hpsi
+    bit for bit; the smallest system against the oracle."""
+    lay = systems.uneven_batch_layout()
+    say("uneven batch config", len(lay), "systems")
+    hams, phis, sysl = [], [], []
+    for i, (L, nst, na) in enumerate(lay):
+        s = systems.uneven_batch_system(i, L, na)
+        psi = systems.random_orbitals(s, nst, seed=0, system_id=100 + i)
+        sysl.append((s, psi))
+        hams.append(hamiltonian.ks_hamiltonian(s, nst))
+        phis.append(to_dev(psi, device))
+    refs = [h(p) for h, p in zip(hams, phis)]
+    b = batch.hamiltonian_batch(hams)
+    outs = b(phis)
+    torch.cuda.synchronize()
+    for o, r in zip(outs, refs):
+        assert torch.equal(o, r)
+    k = int(np.argmin([L ** 3 * nst for L, nst, _ in lay]))
+    s, psi = sysl[k]
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(psi)
+    assert relerr(outs[k].cpu().numpy(), ref) < 1e-12
+    b.close()
+    for h in hams:
+        h.close()
