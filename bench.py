@@ -1,11 +1,18 @@
-"""Benchmark: band-Hpsi/s of the plane-wave Kohn-Sham Hamiltonian at the
-64^3 grid x 256 bands FP64 workload (2x2x2 diamond supercell, a = 20.52 bohr,
-64 atoms x 4 projectors, Gamma point), on one MI355X.
-
-A "step" is one Hpsi application to all 256 bands with psi resident in HBM.
-Prints ONE JSON line on stdout; diagnostics go to stderr.
+"""Benchmark of the plane-wave Kohn-Sham Hpsi and the real-time propagator on
+one MI355X.  Prints ONE JSON line on stdout; diagnostics go to stderr.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--cpu-seconds 20]
+                    [--workload hpsi_64|uneven_batch|adjoint_48|etrs_96]
+
+hpsi_64 (default): band-Hpsi/s at the 64^3 grid x 256 bands FP64 workload
+  (2x2x2 diamond supercell, a = 20.52 bohr, 64 atoms x 4 projectors, Gamma);
+  a step is one Hpsi of all 256 bands with psi resident in HBM.
+uneven_batch: 24 seeded systems (grids 24..72, 8..256 bands, 2..128 atoms);
+  a step is one Hpsi of every system, batched (one call) and per system.
+adjoint_48: frozen-potential ETRS at 48^3 x 64 bands under a 32-term A(t),
+  400 steps with checkpoints every 20; forward + adjoint cost.
+etrs_96: the graph-captured ETRS step at 96^3 x 512 bands (216 atoms), and
+  the adjoint of 100 steps with checkpoints every 10.
 """
 import argparse
 import json
@@ -71,8 +78,243 @@ def cpu_baseline(s, psi, budget_s):
     }, ref
 
 
+def common_fields(args, value, unit, ms_per_step, higher_is_better, config):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step,
+            "higher_is_better": higher_is_better, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic", "config": config}
+
+
+def timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def run_uneven_batch(args):
+    import numpy as np
+    import torch
+
+    import inq_amd
+    from inq_amd import batch as B
+
+    t0 = time.perf_counter()
+    lay = inq_amd.systems.uneven_batch_layout()
+    dev = torch.device("cuda:0")
+    sysl, hams, phis, outs = [], [], [], []
+    for i, (L, nst, na) in enumerate(lay):
+        s = inq_amd.systems.uneven_batch_system(i, L, na)
+        psi = inq_amd.systems.random_orbitals(s, nst, seed=0, system_id=100 + i)
+        sysl.append((s, psi))
+        hams.append(inq_amd.ks_hamiltonian(s, max_bands=nst))
+        phis.append(torch.from_numpy(psi).to(dev))
+        outs.append(torch.empty_like(phis[-1]))
+        if i % 6 == 5:
+            log(f"{i + 1} systems built ({time.perf_counter() - t0:.1f} s)")
+    bands = sum(nst for _, nst, _ in lay)
+
+    def per_system():
+        for h, p, o in zip(hams, phis, outs):
+            h(p, out=o)
+
+    ms_per = timed(per_system, args.steps, args.warmup)
+    ref = [o.clone() for o in outs]
+    bt = B.hamiltonian_batch(hams)
+    t0 = time.perf_counter()
+    best, table = B.autotune(bt, phis, outs, reps=3, chunk_mib=(64, 128, 256), ty_main=(8,),
+                             narrow=(True, False), group=(False, True))
+    log(f"autotune {time.perf_counter() - t0:.1f} s: best {best}; " +
+        ", ".join(f"{k}: {v:.3f} ms" for k, v in sorted(table.items(), key=lambda x: x[1])))
+    ms_bat = timed(lambda: bt(phis, outs), args.steps, args.warmup)
+    torch.cuda.synchronize()
+    batch_err = max(float((torch.linalg.vector_norm(o - r) / torch.linalg.vector_norm(r)).item())
+                    for o, r in zip(outs, ref))
+    info = bt.info()
+    log(f"per-system {ms_per:.3f} ms, batched {ms_bat:.3f} ms -> {ms_per / ms_bat:.2f}x; {info}")
+    total_bytes = 0.0
+    for (s, _), (L, nst, _) in zip(sysl, lay):
+        total_bytes += nst * algorithmic_bytes_per_band(s.npoints, nst,
+                                                        int(np.sum(s.tables.get("sphere_size", 0))))
+    achieved = total_bytes / (ms_bat / 1e3) / 1e9
+    k = int(np.argmin([L ** 3 * nst for L, nst, _ in lay]))
+    cpu = parity = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        s, psi = sysl[k]
+        osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+        t0 = time.perf_counter()
+        r = osys.hamiltonian(psi)
+        dt = time.perf_counter() - t0
+        parity = float(np.linalg.norm(outs[k].cpu().numpy() - r) / np.linalg.norm(r))
+        cpu = {"value": lay[k][1] / dt, "unit": "band-Hpsi/s", "cores": O.get_threads(),
+               "kind": "port",
+               "sample": f"system {k} of the batch ({lay[k][0]}^3, {lay[k][1]} bands, "
+                         f"{lay[k][2]} atoms), one Hpsi, C++ oracle OpenMP"}
+    cfg = {"workload": "uneven_batch: 24 seeded systems, grids 24..72 cubed, bands log-uniform "
+                       "8..256, atoms 2..128 x 4 Gaussian projectors; a step = Hpsi of every "
+                       "system in one batched call",
+           "systems": [list(x) for x in lay], "global_batch": bands, "parallelism": "single GPU"}
+    out = common_fields(args, bands / (ms_bat / 1e3), "band-Hpsi/s", ms_bat, True, cfg)
+    out.update({"per_system_ms_per_step": ms_per, "batched_speedup": ms_per / ms_bat,
+                "batched_vs_per_system_max_rel_err": batch_err, "tuning": best, "batch_info": info,
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                             "kernel": "batched Hpsi step (all launches of one call)",
+                             "algorithmic_bytes_per_step": total_bytes},
+                "cpu_baseline": cpu, "max_rel_err_vs_oracle": parity})
+    bt.close()
+    for h in hams:
+        h.close()
+    return out
+
+
+def _etrs_bytes(N, nst, sum_k):
+    """B_step = nst (8 B_H + 256 N): eight Hpsi and two order-4 Taylor sums
+    whose accumulations read and write the accumulator."""
+    return nst * (8.0 * algorithmic_bytes_per_band(N, nst, sum_k) + 256.0 * N)
+
+
+def _oracle_etrs_rate(s, psi, budget_s, dt):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    one = np.ascontiguousarray(psi[..., :1])
+    t0 = time.perf_counter()
+    osys.etrs_step(one, dt, (0, 0, 0), (0, 0, 0))
+    t1 = time.perf_counter() - t0
+    nb = int(max(1, min(psi.shape[-1], budget_s / max(t1, 1e-3))))
+    sample = np.ascontiguousarray(psi[..., :nb])
+    t0 = time.perf_counter()
+    osys.etrs_step(sample, dt, (0, 0, 0), (0, 0, 0))
+    el = time.perf_counter() - t0
+    return {"value": nb / el, "unit": "band-ETRS-steps/s", "cores": O.get_threads(),
+            "kind": "port", "sample": f"{nb} of {psi.shape[-1]} bands, one ETRS step "
+                                      f"(two Taylor-4 half steps), C++ oracle OpenMP"}
+
+
+def run_adjoint(args, supercell, grid, nst, nsteps, ckpt, dt, workload, fd_coefs=2):
+    import numpy as np
+    import torch
+
+    import inq_amd
+    from inq_amd import adjoint
+
+    t0 = time.perf_counter()
+    s = inq_amd.systems.diamond_system(supercell, grid=grid)
+    psi = inq_amd.systems.random_orbitals(s, nst)
+    coef = inq_amd.systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=32)
+    log(f"inputs built in {time.perf_counter() - t0:.1f} s: grid {s.n}, {nst} bands")
+    dev = torch.device("cuda:0")
+    d_psi = torch.from_numpy(psi).to(dev)
+    del psi
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    prop = adjoint.propagation_adjoint(ham, nst, nsteps, dt, 32, 0.1, (1.0, 0.0, 0.0),
+                                       checkpoint_every=ckpt, band_slice=min(nst, 64))
+    log(f"propagation memory {prop.memory_bytes / 2**30:.1f} GiB")
+    torch.cuda.synchronize()
+    w = time.perf_counter()
+    loss, _ = prop.forward(coef, d_psi)
+    t_fwd = time.perf_counter() - w
+    w = time.perf_counter()
+    gc, gV, _ = prop.backward(1.0)
+    t_bwd = time.perf_counter() - w
+    log(f"forward {t_fwd:.2f} s (loss {loss:.12e}), backward {t_bwd:.2f} s "
+        f"-> ratio {t_bwd / t_fwd:.2f}")
+    # central differences of the GPU forward on the largest-gradient coefficients
+    h = 1e-5
+    fd = {}
+    for j in np.argsort(-np.abs(gc))[:fd_coefs]:
+        e = np.zeros_like(coef)
+        e[j] = h
+        lp, _ = prop.forward(coef + e, d_psi)
+        lm, _ = prop.forward(coef - e, d_psi)
+        fd[int(j)] = ((lp - lm) / (2 * h), float(gc[j]))
+    scale = float(np.abs(gc).max())
+    fd_err = max(abs(a - b) for a, b in fd.values()) / scale
+    log(f"FD check {fd} -> max rel err {fd_err:.2e}")
+    N = s.npoints
+    sum_k = int(np.sum(s.tables["sphere_size"]))
+    step_bytes = _etrs_bytes(N, nst, sum_k)
+    fwd_ms_step = t_fwd * 1e3 / nsteps
+    achieved = step_bytes / (fwd_ms_step / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = _oracle_etrs_rate(s, inq_amd.systems.random_orbitals(s, min(nst, 8)),
+                                args.cpu_seconds, dt)
+    cfg = {"workload": workload, "grid": list(s.n), "bands": nst, "atoms": s.natoms,
+           "steps": nsteps, "checkpoint_every": ckpt, "dt": dt, "taylor_order": 4,
+           "coefficients": 32, "omega": 0.1, "parallelism": "single GPU"}
+    out = common_fields(args, t_bwd / t_fwd, "adjoint time / forward time", fwd_ms_step, False,
+                        cfg)
+    out.update({"forward_s": t_fwd, "backward_s": t_bwd, "loss": loss,
+                "fd_check": {"h": h, "coefficients": fd, "max_rel_err": fd_err},
+                "memory_GiB": prop.memory_bytes / 2**30,
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                             "kernel": "forward ETRS step (all launches)",
+                             "algorithmic_bytes_per_step": step_bytes},
+                "cpu_baseline": cpu})
+    prop.close()
+    ham.close()
+    return out
+
+
+def run_etrs_96(args):
+    import numpy as np
+    import torch
+
+    import inq_amd
+    from inq_amd import real_time
+
+    t0 = time.perf_counter()
+    s = inq_amd.systems.diamond_system(3)
+    nst = 512
+    psi = inq_amd.systems.random_orbitals(s, nst)
+    log(f"inputs built in {time.perf_counter() - t0:.1f} s: grid {s.n}, {s.natoms} atoms")
+    dev = torch.device("cuda:0")
+    d_psi = torch.from_numpy(psi).to(dev)
+    del psi
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    coef = inq_amd.systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=32)
+    pert = real_time.fourier_series_vector_potential(coef, 0.1)
+    dt = 0.03
+    g = real_time.etrs_graph(ham, d_psi, dt, 4, pert)
+    ms = timed(g.launch, args.steps, args.warmup)
+    g.close()
+    N = s.npoints
+    sum_k = int(np.sum(s.tables["sphere_size"]))
+    step_bytes = _etrs_bytes(N, nst, sum_k)
+    achieved = step_bytes / (ms / 1e3) / 1e9
+    log(f"ETRS step {ms:.2f} ms ({achieved:.0f} GB/s algorithmic)")
+    ham.close()
+    cfg = {"workload": "etrs_96: 96^3 grid, 512 bands complex128, 3x3x3 diamond supercell "
+                       "a=30.78 bohr, 216 atoms x 4 projectors, graph-captured ETRS step "
+                       "(two Taylor-4 half steps) under a 32-term A(t), dt=0.03",
+           "grid": list(s.n), "bands": nst, "atoms": s.natoms, "parallelism": "single GPU"}
+    out = common_fields(args, 1e3 / ms, "ETRS steps/s", ms, True, cfg)
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "graph-captured ETRS step (all launches)",
+                       "algorithmic_bytes_per_step": step_bytes}
+    out["cpu_baseline"] = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="hpsi_64",
+                    choices=["hpsi_64", "uneven_batch", "adjoint_48", "etrs_96", "adjoint_96"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -86,6 +328,28 @@ def main():
         print(f"bench.py measures one GPU only (the bands shard with no communication inside "
               f"Hpsi); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "uneven_batch":
+        print(json.dumps(run_uneven_batch(args)), flush=True)
+        return
+    if args.workload == "adjoint_48":
+        out = run_adjoint(args, 1, 48, 64, 400, 20, 0.015,
+                          "adjoint_48: 48^3 grid (a0 cell, 8 atoms x 4 projectors), 64 bands, "
+                          "velocity gauge A_x(t) = sum_j c_j sin(j 0.1 t), 32 seeded c_j, "
+                          "Taylor-4 ETRS, 400 steps of dt=0.015, checkpoints every 20; "
+                          "loss J_x(T)")
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "adjoint_96":
+        out = run_adjoint(args, 3, None, 512, 100, 10, 0.03,
+                          "adjoint_96: 96^3 grid, 512 bands, 216 atoms x 4 projectors, "
+                          "Taylor-4 ETRS, 100 steps of dt=0.03, checkpoints every 10; "
+                          "loss J_x(T)", fd_coefs=1)
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "etrs_96":
+        print(json.dumps(run_etrs_96(args)), flush=True)
+        return
 
     import numpy as np
     import torch

@@ -241,8 +241,9 @@ void inqamd_td_destroy(inqamd_td_t td);
  * projector sets, each its own handle) in one call: every phase of Hpsi is a
  * single persistent launch over the whole batch that drains a queue of
  * per-system work items (projector blocks, then the three sweeps' tiles of
- * band pieces packed into cache-sized chunks).  Results equal the
- * one-system inqamd_hamiltonian_apply of each handle bit for bit.  Replaces
+ * band pieces packed into cache-sized chunks).  Every band goes through the
+ * same tile bodies as in inqamd_hamiltonian_apply (agreement to rounding,
+ * 1e-14 relative in the tests).  Replaces
  * the per-system loop a caller would write around ks_hamiltonian::operator()
  * (ks_hamiltonian.hpp:132-152); the reference has no batched form.
  * Systems whose grid has no three-sweep kernels run their one-system Hpsi

@@ -18,8 +18,9 @@
 // tiles (4 or 2 bands x 16 or 32 y) instead of padding to 8 bands.
 //
 // The arithmetic of every band is that of the one-system Hpsi (same tile
-// bodies, same projector blocks), so batched and per-system results agree
-// bit for bit.
+// bodies, same projector blocks); batched and per-system results agree to
+// rounding (the compiler may contract multiply-adds differently in the
+// batch kernels).
 
 #include "fft3_tiles.hpp"
 #include "projector_dev.hpp"
@@ -74,15 +75,18 @@ constexpr int XY_MAX_NT = 384;  // plane_threads(72)
 // merged switch.
 template <int R0, int R1, int R2, int TY>
 __device__ __attribute__((noinline)) void z_fwd_call(const fft3_args *a, long t, cplx *lds) {
-  z_fwd_tile<R0, R1, R2, TY>(*a, t, lds);
+  const fft3_args c = *a;
+  z_fwd_tile<R0, R1, R2, TY>(c, t, lds);
 }
 template <int R0, int R1, int R2>
 __device__ __attribute__((noinline)) void xy_call(const fft3_args *a, long t, cplx *lds) {
-  xy_kin_plane<R0, R1, R2>(*a, t, lds);
+  const fft3_args c = *a;
+  xy_kin_plane<R0, R1, R2>(c, t, lds);
 }
 template <int R0, int R1, int R2, int TY>
 __device__ __attribute__((noinline)) void z_bwd_call(const fft3_args *a, long t, cplx *lds) {
-  z_bwd_epi_tile<R0, R1, R2, TY>(*a, t, lds);
+  const fft3_args c = *a;
+  z_bwd_epi_tile<R0, R1, R2, TY>(c, t, lds);
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -102,15 +106,29 @@ __device__ __forceinline__ int find_segment(int n, int item, OFF off) {
 }
 
 // Drains the queue: every workgroup takes items until the counter passes
-// `total`; all waves of a workgroup leave together.
+// `total`; the next item is claimed while the current one runs, and all
+// waves of a workgroup leave together.
 template <class F>
 __device__ __forceinline__ void drain_queue(unsigned long long *counter, int total, F &&work) {
-  __shared__ int s_item;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1ULL);
+  __shared__ int s_next;
+  if (threadIdx.x == 0) s_next = (int)atomicAdd(counter, 1ULL);
+  __syncthreads();
+  int cur = uniform(s_next);
+  while (cur < total) {
+    int next = 0;
+    if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1ULL);
+    work(cur);
+    __syncthreads();  // the item's LDS use and every read of s_next are done
+    if (threadIdx.x == 0) s_next = next;
     __syncthreads();
-    const int item = uniform(s_item);
-    if (item >= total) break;
+    cur = uniform(s_next);
+  }
+}
+
+// Items of equal cost: workgroup g takes items g, g + grid, g + 2 grid, ...
+template <class F>
+__device__ __forceinline__ void drain_strided(int total, F &&work) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
     work(item);
     __syncthreads();
   }
@@ -132,7 +150,8 @@ __global__ __launch_bounds__(PE_NT) void batch_project_kernel(const batch_proj_s
     const int s = uniform(find_segment(nsys, item, [&](int i) { return sys[i].item0; }));
     const int local = item - sys[s].item0;
     const int nbt = sys[s].nbt;
-    project_expand_block(sys[s].a, local / nbt, (local % nbt) * PE_BANDS);
+    const project_expand_args a = sys[s].a;
+    project_expand_block(a, local / nbt, (local % nbt) * PE_BANDS);
   });
 }
 
@@ -196,36 +215,40 @@ __global__ __launch_bounds__(Z_NT) void batch_z_bwd_kernel(const batch_piece *pc
   });
 }
 
-// One-variant kernels: the pieces of a launch share grid length and tile shape.
+// One-variant kernels: the pieces of a launch share grid length and tile
+// shape, so their items cost the same and are dealt out in stride.
 template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(Z_NT) void group_z_fwd_kernel(const batch_piece *pc, int np,
-                                                           unsigned long long *counter,
+                                                           unsigned long long *,
                                                            int total) {
   extern __shared__ cplx lds[];
-  drain_queue(counter, total, [&](int item) {
+  drain_strided(total, [&](int item) {
     const int i = uniform(find_segment(np, item, [&](int k) { return pc[k].z0; }));
-    z_fwd_tile<R0, R1, R2, TY>(pc[i].a, item - pc[i].z0, lds);
+    const fft3_args a = pc[i].a;  // a private copy: the tile's stores cannot alias it
+    z_fwd_tile<R0, R1, R2, TY>(a, item - pc[i].z0, lds);
   });
 }
 
 template <int R0, int R1, int R2>
 __global__ __launch_bounds__(plane_threads(R0 * R1 * R2)) void group_xy_kernel(
-    const batch_piece *pc, int np, unsigned long long *counter, int total) {
+    const batch_piece *pc, int np, unsigned long long *, int total) {
   extern __shared__ cplx lds[];
-  drain_queue(counter, total, [&](int item) {
+  drain_strided(total, [&](int item) {
     const int i = uniform(find_segment(np, item, [&](int k) { return pc[k].xy0; }));
-    xy_kin_plane<R0, R1, R2>(pc[i].a, item - pc[i].xy0, lds);
+    const fft3_args a = pc[i].a;
+    xy_kin_plane<R0, R1, R2>(a, item - pc[i].xy0, lds);
   });
 }
 
 template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(Z_NT) void group_z_bwd_kernel(const batch_piece *pc, int np,
-                                                           unsigned long long *counter,
+                                                           unsigned long long *,
                                                            int total) {
   extern __shared__ cplx lds[];
-  drain_queue(counter, total, [&](int item) {
+  drain_strided(total, [&](int item) {
     const int i = uniform(find_segment(np, item, [&](int k) { return pc[k].z0; }));
-    z_bwd_epi_tile<R0, R1, R2, TY>(pc[i].a, item - pc[i].z0, lds);
+    const fft3_args a = pc[i].a;
+    z_bwd_epi_tile<R0, R1, R2, TY>(a, item - pc[i].z0, lds);
   });
 }
 

@@ -105,6 +105,9 @@ struct fft3_args {
   cplx *acc2;
   long acc2_ld;
   double c2re, c2im;
+  // optional copy of psi's forward transform (before the kinetic factor),
+  // band-major [nb][kz][kx][ky] like W, chunk-offset (nullptr: none)
+  cplx *gdump;
 };
 
 // True when the three-sweep kernels exist for length L and fit in max_lds.

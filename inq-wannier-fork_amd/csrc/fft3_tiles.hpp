@@ -101,7 +101,9 @@ __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cpl
     b0[d] = a.rlat[0 * 3 + d];
     b1[d] = a.rlat[1 * 3 + d];
   }
+  cplx *G = a.gdump ? a.gdump + plane * L * L : nullptr;
   auto ky = [&](int i, int c, cplx v, int, int) {
+    if (G) G[c * L + i] = v;
     const int my = symmetric_index(i, L), mx = symmetric_index(c, L);
     const double q0 = mx * b0[0] + my * b1[0] + gz[0];
     const double q1 = mx * b0[1] + my * b1[1] + gz[1];

@@ -235,6 +235,7 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       a.c2re = acc->c2re;
       a.c2im = acc->c2im;
     }
+    if (acc && acc->gdump) a.gdump = acc->gdump + (size_t)b0 * h.npoint;
     fft3_local_hamiltonian(a, h.n[0], h.stream, sweep);
     return;
   }
@@ -393,7 +394,7 @@ static void release(handle &h) {
     if (h.axis[a].tw) (void)hipFree(h.axis[a].tw), h.axis[a].tw = nullptr;
   for (void *p : {(void *)h.V, (void *)h.kA, (void *)h.work, (void *)h.hn, (void *)h.hn2,
                   (void *)h.proj, (void *)h.vnl, (void *)h.partial, (void *)h.mom_a,
-                  (void *)h.mom_b, (void *)h.obs})
+                  (void *)h.mom_b, (void *)h.obs, (void *)h.mom_partial})
     if (p) (void)hipFree(p);
   h.V = nullptr;
   h.kA = nullptr;
@@ -401,6 +402,7 @@ static void release(handle &h) {
   h.partial = nullptr;
   h.mom_a = h.mom_b = nullptr;
   h.obs = nullptr;
+  h.mom_partial = nullptr;
   projector_free(h);
 }
 
@@ -488,6 +490,9 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
     const size_t mom = (size_t)h.pt.nproj * h.pt.max_nlm * 4 * h.max_bands * sizeof(cplx);
     h.mom_a = (cplx *)dalloc(mom);
     h.mom_b = (cplx *)dalloc(mom);
+    // 64 bands x nlm x 4 weights per workgroup of the moments kernel
+    h.mom_partial_elems = MOM_PARTIAL_BLOCKS * 64 * h.pt.max_nlm * 4;
+    h.mom_partial = (cplx *)dalloc(h.mom_partial_elems * sizeof(cplx));
   }
   h.obs = (double *)dalloc(observables_scratch_doubles(h) * sizeof(double));
 }

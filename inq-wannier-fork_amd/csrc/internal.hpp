@@ -65,6 +65,9 @@ struct taylor_acc {
   cplx *p2 = nullptr;
   long ld2 = 0;
   double c2re = 0.0, c2im = 0.0;
+  // three-sweep Hpsi only: the forward transform of psi, band-major
+  // [nst][kz][kx][ky] (nullptr: not written)
+  cplx *gdump = nullptr;
 };
 
 }  // namespace inqamd
@@ -97,6 +100,8 @@ struct inqamd_hamiltonian {
   inqamd::cplx *mom_a = nullptr; // [nproj][max_nlm][4][max_bands] sphere moments
   inqamd::cplx *mom_b = nullptr; // second operand of pair reductions
   double *obs = nullptr;        // observable partials and result records
+  inqamd::cplx *mom_partial = nullptr;  // point-chunk partial sums of sphere moments
+  long mom_partial_elems = 0;
 
   inqamd::projector_tables pt;
 
@@ -142,6 +147,8 @@ void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *ps
 constexpr int OBS_GS_BLOCKS = 1024;  // blocks (and partial rows) of the G-space reductions
 constexpr int OBS_RECORD = 32;       // doubles per result record
 constexpr int OBS_NRECORDS = 16;
+constexpr int MOM_MAX_CHUNKS = 16;     // point chunks per sphere in the moments kernel
+constexpr long MOM_PARTIAL_BLOCKS = 4096;  // workgroups whose partials the buffer holds
 long observables_scratch_doubles(const handle &h);
 // sphere[p][i][b] = sum_lm M[p][lm][i] P[p][lm][b]   (INQ-shaped project output)
 void projector_expand(handle &h, cplx *sphere, int nst);

@@ -38,8 +38,9 @@ __device__ __forceinline__ void phase_item(const float *pos, const int *uent, in
 //           for nlm = 4)
 //   write   Q[i] e^{-i(k+A).r_pi} to its ownership-map entry (:225-231 minus
 //           the scatter, which the Hpsi epilogue performs in fixed order).
-// The four waves of a workgroup take interleaved point blocks; their GEMM1
-// partial tiles are summed in LDS in wave order, so results are reproducible.
+// The PE_WAVES waves of a workgroup take interleaved point blocks; their
+// GEMM1 partial tiles are summed in LDS in wave order, so results are
+// reproducible.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // Operand / result lane layout of v_mfma_f64_16x16x4f64 (checked on the
@@ -53,7 +54,8 @@ __device__ __forceinline__ int mfma_col_b(int l) { return l & 15; }
 __device__ __forceinline__ int mfma_row_d(int l, int v) { return 4 * v + (l >> 4); }
 __device__ __forceinline__ int mfma_col_d(int l) { return l & 15; }
 
-constexpr int PE_WAVES = 4;
+constexpr int PE_WAVES = 8;
+constexpr int PE_UNROLL = 4;  // gather steps whose loads are issued together
 constexpr int PE_NT = 64 * PE_WAVES;
 constexpr int PE_BANDS = 8;
 
@@ -72,26 +74,40 @@ __device__ __forceinline__ void project_expand_block(const project_expand_args &
   const int band = b0 + (col >> 1);
   const bool band_ok = band < a.nst;
 
-  // GEMM1: rows = lm, k = 4 points, columns = real-doubled bands
+  // GEMM1: rows = lm, k = 4 points, columns = real-doubled bands; the point
+  // indices, phases, psi values and table entries of PE_UNROLL steps are
+  // loaded before their MFMAs so the dependent gathers overlap
   v4d acc = {0.0, 0.0, 0.0, 0.0};
   const int lm_a = mfma_row_a(lane);
-  for (int i0 = wave * 4; i0 < K; i0 += PE_WAVES * 4) {
-    const int i = i0 + kk;
-    double bv = 0.0, av = 0.0;
-    if (i < K) {
-      if (band_ok) {
-        const cplx x = cmul(a.phase[off + i], a.psi[(long)a.point[off + i] * a.ld + band]);
-        bv = (col & 1) ? x.y : x.x;
-      }
-      if (lm_a < nlm) av = M[(long)lm_a * a.max_sphere + i];
+  constexpr int STEP = PE_WAVES * 4;
+  for (int i0 = wave * 4 + kk; i0 < K; i0 += STEP * PE_UNROLL) {
+    int idx[PE_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PE_UNROLL; u++) {
+      const int i = i0 + u * STEP;
+      idx[u] = i < K ? a.point[off + i] : -1;
     }
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    cplx ps[PE_UNROLL], ph[PE_UNROLL];
+    double av[PE_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PE_UNROLL; u++) {
+      const int i = i0 + u * STEP;
+      ps[u] = (idx[u] >= 0 && band_ok) ? a.psi[(long)idx[u] * a.ld + band] : mk(0.0, 0.0);
+      ph[u] = i < K ? a.phase[off + i] : mk(0.0, 0.0);
+      av[u] = (i < K && lm_a < nlm) ? M[(long)lm_a * a.max_sphere + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < PE_UNROLL; u++) {
+      const cplx x = cmul(ph[u], ps[u]);
+      const double bv = (col & 1) ? x.y : x.x;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv, acc, 0, 0, 0);
+    }
   }
 #pragma unroll
   for (int v = 0; v < 4; v++) red[wave][mfma_row_d(lane, v)][mfma_col_d(lane)] = acc[v];
   __syncthreads();
-  {
-    const int lm = threadIdx.x >> 4, c = threadIdx.x & 15;  // 256 threads = 16 x 16
+  if (threadIdx.x < 256) {
+    const int lm = threadIdx.x >> 4, c = threadIdx.x & 15;  // 16 x 16 tile
     double s = red[0][lm][c];
     for (int w = 1; w < PE_WAVES; w++) s += red[w][lm][c];
     // GEMM alpha = dV (projector_all.hpp:131-139), then x coeff (:160-164)
@@ -100,7 +116,7 @@ __device__ __forceinline__ void project_expand_block(const project_expand_args &
     Pr[lm][c] = s;
   }
   __syncthreads();
-  if (a.P) {
+  if (a.P && threadIdx.x < 256) {
     // P[p][lm][b] for callers that need the coefficients (INQ-shaped project)
     const int lm = threadIdx.x >> 4, c = threadIdx.x & 15;
     const int bb = b0 + (c >> 1);

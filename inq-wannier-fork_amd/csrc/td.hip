@@ -31,21 +31,29 @@ namespace inqamd {
 // kernels
 // ---------------------------------------------------------------------------
 
-// Moments of a field on the projector spheres:
-//   out[p][lm][c][b] = sum_i M[p][lm][i] w_c(i) e^{i(k+A).x_i} f(x_i, b)
+// Moments of fields on the projector spheres:
+//   out[f][p][lm][c][b] = sum_i M[p][lm][i] w_c(i) e^{i(k+A).x_i} f(x_i, b)
 // w_0 = 1, w_c = sum_e pos_e L[c-1][e] (c = 1..3), or w_2 = w_1^2 if square2.
+// A workgroup takes 64 bands x one point chunk of one sphere and field; with
+// several chunks the partial sums go to `partial` and a second kernel adds
+// them in chunk order.
+constexpr int MOM_MAXF = 16;
 struct moments_args {
-  const cplx *f;
+  const cplx *f[MOM_MAXF];
+  int nf;
   long ld;
   int nst;
-  int max_sphere, max_nlm;
+  int nproj, max_sphere, max_nlm;
   const int *point, *size;
   const double *mat;
   const float *pos;
   const cplx *phase;  // e^{+i(k+A).x} per sphere point
   double L[9];
   int square2;
-  cplx *out;
+  int pch;            // point chunks per sphere
+  cplx *out;          // field stride out_stride
+  long out_stride;
+  cplx *partial;      // [nf][pch][nproj][nlm][4][nst] when pch > 1
 };
 
 constexpr int MOM_BT = 64;
@@ -54,12 +62,17 @@ constexpr int MOM_G = 4;
 __global__ __launch_bounds__(MOM_BT *MOM_G) void moments_kernel(const moments_args a) {
   __shared__ cplx red[MOM_G - 1][4][4][MOM_BT];
   const int p = blockIdx.y;
+  const int fi = blockIdx.z / a.pch, ci = blockIdx.z % a.pch;
   const int bb = threadIdx.x % MOM_BT, g = threadIdx.x / MOM_BT;
   const int b = blockIdx.x * MOM_BT + bb;
   const int K = a.size[p];
+  const int i0 = (int)((long)K * ci / a.pch), i1 = (int)((long)K * (ci + 1) / a.pch);
   const int nlm = a.max_nlm;
   const long off = (long)p * a.max_sphere;
   const double *M = a.mat + (long)p * nlm * a.max_sphere;
+  const cplx *f = a.f[fi];
+  const long fsize = (long)a.nproj * nlm * 4 * a.nst;
+  cplx *dst = a.pch > 1 ? a.partial + ((long)fi * a.pch + ci) * fsize : a.out + fi * a.out_stride;
   for (int lm0 = 0; lm0 < nlm; lm0 += 4) {
     cplx acc[4][4];
 #pragma unroll
@@ -67,8 +80,8 @@ __global__ __launch_bounds__(MOM_BT *MOM_G) void moments_kernel(const moments_ar
 #pragma unroll
       for (int c = 0; c < 4; c++) acc[l][c] = mk(0.0, 0.0);
     if (b < a.nst) {
-      for (int i = g; i < K; i += MOM_G) {
-        const cplx x = cmul(a.phase[off + i], a.f[(long)a.point[off + i] * a.ld + b]);
+      for (int i = i0 + g; i < i1; i += MOM_G) {
+        const cplx x = cmul(a.phase[off + i], f[(long)a.point[off + i] * a.ld + b]);
         const float *ps = a.pos + (off + i) * 3;
         const double p0 = ps[0], p1 = ps[1], p2 = ps[2];
         double w[4];
@@ -109,9 +122,23 @@ __global__ __launch_bounds__(MOM_BT *MOM_G) void moments_kernel(const moments_ar
         if (lm0 + l < nlm)
 #pragma unroll
           for (int c = 0; c < 4; c++)
-            a.out[(((long)p * nlm + lm0 + l) * 4 + c) * a.nst + b] = acc[l][c];
+            dst[(((long)p * nlm + lm0 + l) * 4 + c) * a.nst + b] = acc[l][c];
     }
     __syncthreads();
+  }
+}
+
+// out[f][e] = sum over chunks of partial[f][chunk][e], in chunk order
+__global__ __launch_bounds__(256) void moments_reduce_kernel(const cplx *partial, int nf, int pch,
+                                                             long fsize, cplx *out,
+                                                             long out_stride) {
+  const long total = (long)nf * fsize;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long fi = e / fsize, r = e - fi * fsize;
+    const cplx *src = partial + fi * pch * fsize + r;
+    cplx s = src[0];
+    for (int c = 1; c < pch; c++) s = cadd(s, src[(long)c * fsize]);
+    out[fi * out_stride + r] = s;
   }
 }
 
@@ -382,6 +409,167 @@ __global__ __launch_bounds__(256) void dzero_kernel(double *x, long n) {
 }
 
 // ---------------------------------------------------------------------------
+// fused half-step gradient kernels (three-sweep Hpsi)
+//
+// With A_k = H^k lambda' and Y_j = H^j phi (k, j < M) and the coefficient
+// matrix E[k][j] = z^(k+j+1)/(k+j+1)! (zero for k + j >= M), the sums over
+// X_j of the half step become
+//   dL/dV(r) += Re sum_b sum_kj E[k][j] conj(A_k) Y_j
+//   dL/ds    += Re sum_kj E[k][j] <A_k|W|Y_j>
+// so no X_j is formed; the forward transforms of A_k and Y_j come from the
+// G-space dumps of the Hpsi calls that produce the powers.
+// ---------------------------------------------------------------------------
+constexpr int TD_MAXM = 8;
+
+struct coef_matrix {
+  int M;
+  double er[TD_MAXM][TD_MAXM], ei[TD_MAXM][TD_MAXM];
+};
+
+// Re(e conj(a) y)
+__device__ __forceinline__ double re_e_conj_a_y(double er, double ei, cplx a, cplx y) {
+  const double pr = a.x * y.x + a.y * y.y, pi = a.x * y.y - a.y * y.x;
+  return er * pr - ei * pi;
+}
+
+struct vgrad_multi_args {
+  const cplx *A[TD_MAXM];
+  const cplx *Y[TD_MAXM];
+  long ld;
+  int nb;
+  long npoint;
+  double *gV;
+};
+
+__global__ __launch_bounds__(256) void vgrad_multi_kernel(const vgrad_multi_args a,
+                                                          const coef_matrix E) {
+  const int lane = threadIdx.x & 63;
+  const long p = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= a.npoint) return;
+  const int M = E.M;
+  double s = 0.0;
+  for (int b = lane; b < a.nb; b += 64) {
+    cplx y[TD_MAXM];
+#pragma unroll
+    for (int j = 0; j < TD_MAXM; j++)
+      if (j < M) y[j] = a.Y[j][p * a.ld + b];
+#pragma unroll
+    for (int k = 0; k < TD_MAXM; k++) {
+      if (k < M) {
+        const cplx x = a.A[k][p * a.ld + b];
+#pragma unroll
+        for (int j = 0; j < TD_MAXM; j++)
+          if (k + j < M) s += re_e_conj_a_y(E.er[k][j], E.ei[k][j], x, y[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if (lane == 0) a.gV[p] += s;
+}
+
+struct gpair_args {
+  const cplx *A[TD_MAXM];  // band-major [b][kz][kx][ky] forward transforms
+  const cplx *Y[TD_MAXM];
+  int L;
+  long total;  // nb * L^3
+  double bd[3], kad;
+};
+
+// per-block partial sums of Re sum_kj E[k][j] w(G) conj(A_k^) Y_j^
+__global__ __launch_bounds__(256) void gpair_kernel(const gpair_args a, const coef_matrix E,
+                                                    double *partial) {
+  __shared__ double red[256];
+  const int M = E.M;
+  const int L = a.L;
+  const long L2 = (long)L * L, L3 = L2 * L;
+  double s = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < a.total; e += (long)gridDim.x * 256) {
+    const long r = e % L3;
+    const int kz = (int)(r / L2), kx = (int)((r / L) % L), ky = (int)(r % L);
+    const double w = sym_index(kx, L) * a.bd[0] + sym_index(ky, L) * a.bd[1] +
+                     sym_index(kz, L) * a.bd[2] + a.kad;
+    cplx y[TD_MAXM];
+#pragma unroll
+    for (int j = 0; j < TD_MAXM; j++)
+      if (j < M) y[j] = a.Y[j][e];
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < TD_MAXM; k++) {
+      if (k < M) {
+        const cplx x = a.A[k][e];
+#pragma unroll
+        for (int j = 0; j < TD_MAXM; j++)
+          if (k + j < M) t += re_e_conj_a_y(E.er[k][j], E.ei[k][j], x, y[j]);
+      }
+    }
+    s += w * t;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// per (sphere, lm): S = sum_b D sum_kj E[k][j] (conj(p^A_k) r^Y_j - conj(r^A_k) p^Y_j)
+__global__ __launch_bounds__(64) void nl_multi_kernel(const cplx *mA, const cplx *mY,
+                                                      long field_stride, const double *coeff,
+                                                      int nb, const coef_matrix E,
+                                                      double *partial) {
+  __shared__ double red[2][64];
+  const int row = blockIdx.x;
+  const int M = E.M;
+  const double D = coeff[row];
+  double sr = 0.0, si = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 64) {
+    for (int k = 0; k < M; k++) {
+      const cplx pa = mA[k * field_stride + ((long)row * 4 + 0) * nb + b];
+      const cplx ra = mA[k * field_stride + ((long)row * 4 + 1) * nb + b];
+      for (int j = 0; j + k < M; j++) {
+        const cplx py = mY[j * field_stride + ((long)row * 4 + 0) * nb + b];
+        const cplx ry = mY[j * field_stride + ((long)row * 4 + 1) * nb + b];
+        // u = conj(pa) ry - conj(ra) py
+        const double ur = (pa.x * ry.x + pa.y * ry.y) - (ra.x * py.x + ra.y * py.y);
+        const double ui = (pa.x * ry.y - pa.y * ry.x) - (ra.x * py.y - ra.y * py.x);
+        const double er = E.er[k][j], ei = E.ei[k][j];
+        sr += D * (er * ur - ei * ui);
+        si += D * (er * ui + ei * ur);
+      }
+    }
+  }
+  red[0][threadIdx.x] = sr;
+  red[1][threadIdx.x] = si;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) partial[(long)row * 2 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+struct grad_fused_args {
+  double inv_n, dV;
+  double t, omega;
+  int ncoef;
+  double scale;
+};
+
+// rec[0] = kinetic sum, rec[1..2] = S of nl_multi; g = inv_n kin + Re(i dV S)
+__global__ void grad_fused_kernel(const double *rec, const grad_fused_args a, double *gcoef) {
+  __shared__ double g;
+  if (threadIdx.x == 0) g = a.scale * (a.inv_n * rec[0] - a.dV * rec[2]);
+  __syncthreads();
+  for (int k = threadIdx.x; k < a.ncoef; k += blockDim.x)
+    gcoef[k] += g * sin((k + 1) * a.omega * a.t);
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 static unsigned blocks_for(long total) {
@@ -403,15 +591,26 @@ static double *obs_record(handle &h, int r) {
   return h.obs + (nl > gs ? nl : gs) + (long)r * OBS_RECORD;
 }
 
-// sphere moments of f (phases of the slot must be current)
-static void sphere_moments(handle &h, const cplx *f, long ld, int nb, const double L[9],
-                           bool square2, cplx *out) {
+// Point chunks per sphere so that a moments launch fills the device.
+static int moment_chunks(const handle &h, int nb, int nf) {
+  const long blocks = (long)((nb + MOM_BT - 1) / MOM_BT) * h.pt.nproj * nf;
+  long pch = (2048 + blocks - 1) / (blocks > 0 ? blocks : 1);
+  if (pch > MOM_MAX_CHUNKS) pch = MOM_MAX_CHUNKS;
+  return (int)(pch < 1 ? 1 : pch);
+}
+
+// sphere moments of nf fields (phases of the slot must be current); field
+// f's moments go to out + f * out_stride
+static void sphere_moments_multi(handle &h, const cplx *const *f, int nf, long ld, int nb,
+                                 const double L[9], bool square2, cplx *out, long out_stride) {
   const projector_tables &t = h.pt;
   if (t.nproj == 0) return;
-  moments_args a;
-  a.f = f;
+  moments_args a{};
+  for (int i = 0; i < nf; i++) a.f[i] = f[i];
+  a.nf = nf;
   a.ld = ld;
   a.nst = nb;
+  a.nproj = t.nproj;
   a.max_sphere = t.max_sphere;
   a.max_nlm = t.max_nlm;
   a.point = t.point;
@@ -422,9 +621,25 @@ static void sphere_moments(handle &h, const cplx *f, long ld, int nb, const doub
   for (int i = 0; i < 9; i++) a.L[i] = L[i];
   a.square2 = square2 ? 1 : 0;
   a.out = out;
-  dim3 grid((nb + MOM_BT - 1) / MOM_BT, t.nproj);
+  a.out_stride = out_stride;
+  const long fsize = (long)t.nproj * t.max_nlm * 4 * nb;
+  a.pch = moment_chunks(h, nb, nf);
+  if ((long)a.pch * nf * fsize > h.mom_partial_elems) a.pch = 1;
+  a.partial = h.mom_partial;
+  dim3 grid((nb + MOM_BT - 1) / MOM_BT, t.nproj, nf * a.pch);
   hipLaunchKernelGGL(moments_kernel, grid, dim3(MOM_BT * MOM_G), 0, h.stream, a);
   INQAMD_HIP_CHECK(hipGetLastError());
+  if (a.pch > 1) {
+    const long total = nf * fsize;
+    hipLaunchKernelGGL(moments_reduce_kernel, dim3(blocks_for(total)), dim3(256), 0, h.stream,
+                       (const cplx *)h.mom_partial, nf, a.pch, fsize, out, out_stride);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+}
+
+static void sphere_moments(handle &h, const cplx *f, long ld, int nb, const double L[9],
+                           bool square2, cplx *out) {
+  sphere_moments_multi(h, &f, 1, ld, nb, L, square2, out, 0);
 }
 
 // record[0..7] = G-space moments of (Xh, Yh)
@@ -528,6 +743,13 @@ struct inqamd_td {
   cplx *Ybuf[8] = {};         // recomputed powers when not stored
   cplx *gA = nullptr, *gB = nullptr;  // G-space temporaries [N][nb]
   cplx *sph = nullptr;        // [nproj][max_sphere][nb]
+  // fused path (three-sweep Hpsi): the tape holds Y_j and their forward
+  // transforms; A_k = H^k lambda' (k >= 1) and their transforms
+  bool fused = false;
+  cplx *Ak[8] = {};
+  cplx *Ah[8] = {};
+  cplx *momA = nullptr, *momY = nullptr;  // [order][nproj][nlm][4][nb]
+  long mom_stride = 0;
   double *gcoef = nullptr;    // device [ncoef]
   double *gV = nullptr;       // device [N]
   double *loss = nullptr;     // device [1]
@@ -553,7 +775,11 @@ void td_release(inqamd_td *td) {
   for (int j = 0; j < 8; j++) {
     ps.push_back(td->X[j]);
     ps.push_back(td->Ybuf[j]);
+    ps.push_back(td->Ak[j]);
+    ps.push_back(td->Ah[j]);
   }
+  ps.push_back(td->momA);
+  ps.push_back(td->momY);
   for (void *p : ps)
     if (p) (void)hipFree(p);
 }
@@ -596,9 +822,111 @@ void replay_half(inqamd_td *td, cplx *cur, int nb, cplx *const *Y, int slot) {
     acc.ld1 = nb;
     acc.c1re = coeff.real();
     acc.c1im = coeff.imag();
+    if (td->fused) acc.gdump = Y[order + it - 1];  // transform of Y_{it-1}
     cplx *out = it < order ? Y[it] : td->hk[0];
     hamiltonian_apply(h, Y[it - 1], nb, out, nb, nb, slot, &acc);
   }
+}
+
+// The fused half step: Y[0..M) real powers, Y[M..2M) their transforms.
+void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double t_h,
+                        const double kA[3]) {
+  handle &h = *td->h;
+  const int M = td->d.order;
+  const std::complex<double> z(0.0, td->d.dt / 2.0);
+  const std::complex<double> zc = std::conj(z);
+  // lambda_in = sum_k conj(z)^k/k! H^k lambda' (the k >= 1 terms fused)
+  copy(h, td->lam, nb, td->lam_in, nb, nb);
+  const cplx *A[TD_MAXM];
+  A[0] = td->lam;
+  std::complex<double> ck = 1.0;
+  for (int k = 1; k <= M; k++) {
+    ck *= zc / double(k);
+    taylor_acc acc;
+    acc.p1 = td->lam_in;
+    acc.ld1 = nb;
+    acc.c1re = ck.real();
+    acc.c1im = ck.imag();
+    acc.gdump = td->Ah[k - 1];
+    cplx *out = k < M ? td->Ak[k] : td->hk[0];
+    hamiltonian_apply(h, A[k - 1], nb, out, nb, nb, slot, &acc);
+    if (k < M) A[k] = out;
+  }
+  coef_matrix E{};
+  E.M = M;
+  {
+    std::vector<std::complex<double>> e(M + 1);
+    e[0] = 1.0;
+    for (int m = 1; m <= M; m++) e[m] = e[m - 1] * z / double(m);
+    for (int k = 0; k < M; k++)
+      for (int j = 0; k + j < M; j++) {
+        E.er[k][j] = e[k + j + 1].real();
+        E.ei[k][j] = e[k + j + 1].imag();
+      }
+  }
+  // dL/dV
+  {
+    vgrad_multi_args a{};
+    for (int k = 0; k < M; k++) {
+      a.A[k] = A[k];
+      a.Y[k] = Y[k];
+    }
+    a.ld = nb;
+    a.nb = nb;
+    a.npoint = h.npoint;
+    a.gV = td->gV;
+    hipLaunchKernelGGL(vgrad_multi_kernel, dim3((unsigned)((h.npoint + 3) / 4)), dim3(256), 0,
+                       h.stream, a, E);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+  double *rec = obs_record(h, 0);
+  double *part = obs_partial(h);
+  const pair_weights w = make_weights(h, td->d.direction, kA);
+  // kinetic part of dL/ds from the transforms
+  {
+    gpair_args a{};
+    for (int k = 0; k < M; k++) {
+      a.A[k] = td->Ah[k];
+      a.Y[k] = Y[M + k];
+    }
+    a.L = h.n[0];
+    a.total = h.npoint * nb;
+    for (int e = 0; e < 3; e++) a.bd[e] = w.bd[e];
+    a.kad = w.kad;
+    hipLaunchKernelGGL(gpair_kernel, dim3(OBS_GS_BLOCKS), dim3(256), 0, h.stream, a, E, part);
+    INQAMD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(64), 0, h.stream, (const double *)part,
+                       OBS_GS_BLOCKS, 1, rec);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+  // projector part of dL/ds from sphere moments of A_k and Y_j
+  const projector_tables &t = h.pt;
+  const int nrows = t.nproj * t.max_nlm;
+  if (nrows > 0) {
+    projector_phases(h, slot);
+    double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    lattice_dot(h, td->d.direction, L);
+    sphere_moments_multi(h, A, M, nb, nb, L, false, td->momA, td->mom_stride);
+    sphere_moments_multi(h, Y, M, nb, nb, L, false, td->momY, td->mom_stride);
+    hipLaunchKernelGGL(nl_multi_kernel, dim3(nrows), dim3(64), 0, h.stream,
+                       (const cplx *)td->momA, (const cplx *)td->momY, td->mom_stride,
+                       (const double *)t.coeff, nb, E, part);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(64), 0, h.stream, (const double *)part, nrows,
+                     2, rec + 1);
+  INQAMD_HIP_CHECK(hipGetLastError());
+  grad_fused_args ga;
+  ga.inv_n = w.inv_n;
+  ga.dV = w.dV;
+  ga.t = t_h;
+  ga.omega = td->d.omega;
+  ga.ncoef = td->d.ncoef;
+  ga.scale = 1.0;
+  hipLaunchKernelGGL(grad_fused_kernel, dim3(1), dim3(64), 0, h.stream, (const double *)rec, ga,
+                     td->gcoef);
+  INQAMD_HIP_CHECK(hipGetLastError());
+  std::swap(td->lam, td->lam_in);
 }
 
 // dL/ds and dL/dV contributions of one half step, and lambda <- T(conj z)^dagger lambda'
@@ -817,7 +1145,8 @@ int inqamd_td_create(inqamd_hamiltonian_t h, const inqamd_td_desc *desc, inqamd_
     td->nb = d.band_slice == 0 || d.band_slice > d.nst ? d.nst : d.band_slice;
     td->nseg = (d.nsteps + d.checkpoint_every - 1) / d.checkpoint_every;
     lattice_dot(*h, d.direction, td->dir_cov);
-    td->nstore = d.store_powers ? d.order : 1;
+    td->fused = h->three_sweep && d.store_powers;
+    td->nstore = td->fused ? 2 * d.order : (d.store_powers ? d.order : 1);
     const long N = h->npoint;
     const size_t full = (size_t)N * d.nst * sizeof(cplx);
     const size_t slice = (size_t)N * td->nb * sizeof(cplx);
@@ -829,12 +1158,22 @@ int inqamd_td_create(inqamd_hamiltonian_t h, const inqamd_td_desc *desc, inqamd_
     td->lam = (cplx *)td_alloc(td, slice);
     td->lam_in = (cplx *)td_alloc(td, slice);
     td->hk[0] = (cplx *)td_alloc(td, slice);
-    td->hk[1] = (cplx *)td_alloc(td, slice);
-    for (int j = 0; j < d.order; j++) td->X[j] = (cplx *)td_alloc(td, slice);
-    if (td->nstore == 1)
-      for (int j = 1; j < d.order; j++) td->Ybuf[j] = (cplx *)td_alloc(td, slice);
     td->gA = (cplx *)td_alloc(td, slice);
-    td->gB = (cplx *)td_alloc(td, slice);
+    if (td->fused) {
+      for (int k = 1; k < d.order; k++) td->Ak[k] = (cplx *)td_alloc(td, slice);
+      for (int k = 0; k < d.order; k++) td->Ah[k] = (cplx *)td_alloc(td, slice);
+      if (h->pt.nproj > 0) {
+        td->mom_stride = (long)h->pt.nproj * h->pt.max_nlm * 4 * td->nb;
+        td->momA = (cplx *)td_alloc(td, (size_t)td->mom_stride * d.order * sizeof(cplx));
+        td->momY = (cplx *)td_alloc(td, (size_t)td->mom_stride * d.order * sizeof(cplx));
+      }
+    } else {
+      td->hk[1] = (cplx *)td_alloc(td, slice);
+      for (int j = 0; j < d.order; j++) td->X[j] = (cplx *)td_alloc(td, slice);
+      if (td->nstore == 1)
+        for (int j = 1; j < d.order; j++) td->Ybuf[j] = (cplx *)td_alloc(td, slice);
+      td->gB = (cplx *)td_alloc(td, slice);
+    }
     if (h->pt.nproj > 0)
       td->sph = (cplx *)td_alloc(td, (size_t)h->pt.nproj * h->pt.max_sphere * td->nb * sizeof(cplx));
     td->gcoef = (double *)td_alloc(td, (size_t)(d.ncoef > 0 ? d.ncoef : 1) * sizeof(double));
@@ -939,7 +1278,7 @@ int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double 
             ka_at(td, n * d.dt + half * d.dt, kv);
             const int slot = half ? KA_TDT : KA_T;
             set_slot(*h, slot, kv);
-            cplx *Y[8];
+            cplx *Y[16];
             const size_t base = ((size_t)(n - n0) * 2 + half) * td->nstore;
             for (int j = 0; j < td->nstore; j++) Y[j] = td->tape + (base + j) * slice_elems;
             replay_half(td, td->cur, w, Y, slot);
@@ -952,10 +1291,13 @@ int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double 
             ka_at(td, th, kv);
             const int slot = half ? KA_TDT : KA_T;
             set_slot(*h, slot, kv);
-            cplx *Y[8];
+            cplx *Y[16];
             const size_t base = ((size_t)(n - n0) * 2 + half) * td->nstore;
             for (int j = 0; j < td->nstore; j++) Y[j] = td->tape + (base + j) * slice_elems;
-            adjoint_half(td, w, Y, slot, th, kv, 1.0);
+            if (td->fused)
+              adjoint_half_fused(td, w, Y, slot, th, kv);
+            else
+              adjoint_half(td, w, Y, slot, th, kv, 1.0);
           }
       }
       if (lambda0) copy(*h, td->lam, w, (cplx *)lambda0 + b0, nst, w);

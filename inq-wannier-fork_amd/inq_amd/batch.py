@@ -4,8 +4,8 @@
     outs = batch([phi_0, phi_1, ...])            # one Hpsi per system
 
 Each phase of Hpsi is one persistent work-queue launch over the whole batch
-(include/inq_amd.h, inqamd_batch_*); results equal each system's own
-ks_hamiltonian(phi) bit for bit.  `autotune` picks the tiling knobs by
+(include/inq_amd.h, inqamd_batch_*); results match each system's own
+ks_hamiltonian(phi) to rounding.  `autotune` picks the tiling knobs by
 timing the batch on the device.
 """
 import ctypes

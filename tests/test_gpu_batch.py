@@ -1,9 +1,10 @@
 """GPU tests of the uneven-batch Hpsi.
 
 The batch runs the same tile bodies and projector blocks as the one-system
-Hpsi, so its output must equal each system's own ks_hamiltonian(phi) bit for
-bit, under every tiling choice; one system is also checked against the
-oracle (1e-12 relative).
+Hpsi; its output must match each system's own ks_hamiltonian(phi) to 1e-14
+relative (different kernel instantiations may contract multiply-adds
+differently, so agreement is not bitwise) under every tiling choice, and
+every member must match the oracle to 1e-12 relative.
 """
 import sys
 
@@ -64,7 +65,7 @@ def small_batch(device):
     dict(chunk_bytes=1 << 20),
     dict(chunk_bytes=1 << 20, group_by_grid=True, narrow_remainders=False),
 ])
-def test_batch_equals_per_system_bitwise(device, small_batch, tuning):
+def test_batch_matches_per_system(device, small_batch, tuning):
     say("batch small", tuning)
     sysl, hams, phis, refs = small_batch
     b = batch.hamiltonian_batch(hams)
@@ -73,13 +74,14 @@ def test_batch_equals_per_system_bitwise(device, small_batch, tuning):
     torch.cuda.synchronize()
     info = b.info()
     assert info["fallback"] == 2
-    for i, (o, r) in enumerate(zip(outs, refs)):
-        assert np.array_equal(o.cpu().numpy(), r), (i, tuning)
+    errs = [relerr(o.cpu().numpy(), r) for o, r in zip(outs, refs)]
+    say("batch vs per-system rel err", ["%.1e" % e for e in errs], info)
+    assert max(errs) < 1e-14, (errs, tuning)
     # a second call reuses the work tables
     outs2 = b(phis, outs)
     torch.cuda.synchronize()
     for o, r in zip(outs2, refs):
-        assert np.array_equal(o.cpu().numpy(), r)
+        assert relerr(o.cpu().numpy(), r) < 1e-14
     b.close()
 
 
@@ -89,16 +91,17 @@ def test_batch_member_matches_oracle(device, small_batch):
     b = batch.hamiltonian_batch(hams)
     outs = b(phis)
     torch.cuda.synchronize()
-    for i in (1, 3):
-        s, psi = sysl[i]
+    for i, (s, psi) in enumerate(sysl):
         ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(psi)
-        assert relerr(outs[i].cpu().numpy(), ref) < 1e-12
+        err = relerr(outs[i].cpu().numpy(), ref)
+        say("member", i, s.n, psi.shape[-1], "rel err vs oracle %.2e" % err)
+        assert err < 1e-12, i
     b.close()
 
 
 def test_uneven_batch_config_matches_per_system(device):
-    """The full 24-system uneven batch: batched output equals per-system Hpsi
-    bit for bit; the smallest system against the oracle."""
+    """The full 24-system uneven batch: batched output against per-system
+    Hpsi, and the smallest system against the oracle."""
     lay = systems.uneven_batch_layout()
     say("uneven batch config", len(lay), "systems")
     hams, phis, sysl = [], [], []
@@ -112,8 +115,10 @@ def test_uneven_batch_config_matches_per_system(device):
     b = batch.hamiltonian_batch(hams)
     outs = b(phis)
     torch.cuda.synchronize()
-    for o, r in zip(outs, refs):
-        assert torch.equal(o, r)
+    errs = [float((torch.linalg.vector_norm(o - r) / torch.linalg.vector_norm(r)).item())
+            for o, r in zip(outs, refs)]
+    say("uneven batch vs per-system max rel err %.2e" % max(errs))
+    assert max(errs) < 1e-14
     k = int(np.argmin([L ** 3 * nst for L, nst, _ in lay]))
     s, psi = sysl[k]
     ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(psi)
