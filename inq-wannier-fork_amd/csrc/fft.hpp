@@ -108,6 +108,8 @@ struct fft3_args {
   // optional copy of psi's forward transform (before the kinetic factor),
   // band-major [nb][kz][kx][ky] like W, chunk-offset (nullptr: none)
   cplx *gdump;
+  // hpsi written with non-temporal stores (streamed past the caches)
+  int nt_store;
 };
 
 // True when the three-sweep kernels exist for length L and fit in max_lds.

@@ -10,26 +10,49 @@
 
 #include "fft3_tiles.hpp"
 
+#include <cstdlib>
+
 namespace inqamd {
 
-constexpr int Z_TY1 = 8;  // z tile of the one-system kernels: 8 y x 8 bands
-
-template <int R0, int R1, int R2>
+// Shapes of the one-system kernels: z tiles of TY y x 64/TY bands, x-y
+// planes with plane_threads(L) or twice as many threads.  The defaults (8,
+// 1x) can be overridden through INQAMD_Z_TY (8 or 16) and INQAMD_XY_WIDE (1).
+template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(Z_NT) void z_fwd_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_fwd_tile<R0, R1, R2, Z_TY1>(a, blockIdx.x, lds);
+  z_fwd_tile<R0, R1, R2, TY>(a, blockIdx.x, lds);
 }
 
-template <int R0, int R1, int R2>
-__global__ __launch_bounds__(plane_threads(R0 * R1 * R2)) void xy_kin_kernel(const fft3_args a) {
+constexpr int wide_threads(int L) {
+  return 2 * plane_threads(L) > 1024 ? 1024 : 2 * plane_threads(L);
+}
+
+template <int R0, int R1, int R2, int NT>
+__global__ __launch_bounds__(NT) void xy_kin_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  xy_kin_plane<R0, R1, R2>(a, blockIdx.x, lds);
+  xy_kin_plane<R0, R1, R2, NT>(a, blockIdx.x, lds);
 }
 
-template <int R0, int R1, int R2>
+template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(Z_NT) void z_bwd_epi_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_bwd_epi_tile<R0, R1, R2, Z_TY1>(a, blockIdx.x, lds);
+  z_bwd_epi_tile<R0, R1, R2, TY>(a, blockIdx.x, lds);
+}
+
+static int z_ty_choice() {
+  static const int ty = [] {
+    const char *e = getenv("INQAMD_Z_TY");
+    return e && atoi(e) == 16 ? 16 : 8;
+  }();
+  return ty;
+}
+
+static bool xy_wide_choice() {
+  static const bool w = [] {
+    const char *e = getenv("INQAMD_XY_WIDE");
+    return e && atoi(e) == 1;
+  }();
+  return w;
 }
 
 // ---------------------------------------------------------------------------
@@ -45,31 +68,52 @@ static void ensure_lds(K kernel, size_t bytes) {
 template <int R0, int R1, int R2>
 static void prepare3() {
   constexpr int L = R0 * R1 * R2;
-  ensure_lds(z_fwd_kernel<R0, R1, R2>, (size_t)L * Z_COLS * sizeof(cplx));
-  ensure_lds(xy_kin_kernel<R0, R1, R2>, (size_t)L * (L + 1) * sizeof(cplx));
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2>, (size_t)L * Z_COLS * sizeof(cplx));
+  const size_t zl = (size_t)L * Z_COLS * sizeof(cplx), pl = (size_t)L * (L + 1) * sizeof(cplx);
+  ensure_lds(z_fwd_kernel<R0, R1, R2, 8>, zl);
+  ensure_lds(z_fwd_kernel<R0, R1, R2, 16>, zl);
+  ensure_lds(xy_kin_kernel<R0, R1, R2, plane_threads(L)>, pl);
+  ensure_lds(xy_kin_kernel<R0, R1, R2, wide_threads(L)>, pl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8>, zl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16>, zl);
+}
+
+template <int R0, int R1, int R2, int TY, int NT>
+static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
+  constexpr int L = R0 * R1 * R2;
+  const long ztiles = z_tiles<TY>(L, a.nb);
+  const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
+  const size_t plds = (size_t)L * (L + 1) * sizeof(cplx);
+  if (sweep < 0 || sweep == 0) {
+    hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(Z_NT), zlds,
+                       stream, a);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+  if (sweep < 0 || sweep == 1) {
+    hipLaunchKernelGGL((xy_kin_kernel<R0, R1, R2, NT>), dim3((unsigned)((long)a.nb * L)), dim3(NT),
+                       plds, stream, a);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
+  if (sweep < 0 || sweep == 2) {
+    hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(Z_NT),
+                       zlds, stream, a);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  }
 }
 
 template <int R0, int R1, int R2>
 static void launch3(const fft3_args &a, hipStream_t stream, int sweep) {
   constexpr int L = R0 * R1 * R2;
-  const long ztiles = z_tiles<Z_TY1>(L, a.nb);
-  const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
-  const size_t plds = (size_t)L * (L + 1) * sizeof(cplx);
-  if (sweep < 0 || sweep == 0) {
-    hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2>), dim3((unsigned)ztiles), dim3(Z_NT), zlds,
-                       stream, a);
-    INQAMD_HIP_CHECK(hipGetLastError());
-  }
-  if (sweep < 0 || sweep == 1) {
-    hipLaunchKernelGGL((xy_kin_kernel<R0, R1, R2>), dim3((unsigned)((long)a.nb * L)),
-                       dim3(plane_threads(L)), plds, stream, a);
-    INQAMD_HIP_CHECK(hipGetLastError());
-  }
-  if (sweep < 0 || sweep == 2) {
-    hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2>), dim3((unsigned)ztiles), dim3(Z_NT), zlds,
-                       stream, a);
-    INQAMD_HIP_CHECK(hipGetLastError());
+  const bool wide = xy_wide_choice();
+  if (z_ty_choice() == 16) {
+    if (wide)
+      launch3v<R0, R1, R2, 16, wide_threads(L)>(a, stream, sweep);
+    else
+      launch3v<R0, R1, R2, 16, plane_threads(L)>(a, stream, sweep);
+  } else {
+    if (wide)
+      launch3v<R0, R1, R2, 8, wide_threads(L)>(a, stream, sweep);
+    else
+      launch3v<R0, R1, R2, 8, plane_threads(L)>(a, stream, sweep);
   }
 }
 

@@ -69,11 +69,10 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
 // ---------------------------------------------------------------------------
 // sweep 2: 2-D x-y transform of one plane with the kinetic factor
 // ---------------------------------------------------------------------------
-template <int R0, int R1, int R2>
+template <int R0, int R1, int R2, int NT = plane_threads(R0 * R1 * R2)>
 __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
-  constexpr int NT = plane_threads(L);
   constexpr int PAD = L + 1;  // row stride of the LDS plane (breaks bank aliasing)
   // lds: [x][PAD]; plane = band * L + z
   const int z = (int)(plane % L);
@@ -185,7 +184,13 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
       cplx v = cscale(a.scale, lds[z * Z_COLS + b * Z_TY + y]);
       v.x += pre[k].x;
       v.y += pre[k].y;
-      a.hpsi[p * a.hld + cb] = v;
+      if (a.nt_store) {
+        double *o = reinterpret_cast<double *>(a.hpsi + p * a.hld + cb);
+        __builtin_nontemporal_store(v.x, o);
+        __builtin_nontemporal_store(v.y, o + 1);
+      } else {
+        a.hpsi[p * a.hld + cb] = v;
+      }
       if (a.acc1) {
         cplx *d = a.acc1 + p * a.acc1_ld + cb;
         cplx o = *d;

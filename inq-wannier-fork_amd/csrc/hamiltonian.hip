@@ -236,6 +236,11 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       a.c2im = acc->c2im;
     }
     if (acc && acc->gdump) a.gdump = acc->gdump + (size_t)b0 * h.npoint;
+    static const int nt_store = [] {
+      const char *e = getenv("INQAMD_NT_STORE");
+      return e && atoi(e) == 1 ? 1 : 0;
+    }();
+    a.nt_store = nt_store;
     fft3_local_hamiltonian(a, h.n[0], h.stream, sweep);
     return;
   }

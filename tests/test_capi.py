@@ -75,3 +75,50 @@ def test_sphere_builder_argument_checks(bad):
         rc = L.inqamd_sphere_build(lat, _lib.ivec([8, 8, 8]), _lib.dvec([0, 0, 0]), 1.5, 0, None,
                                    None, None, ctypes.byref(cnt))
         assert rc == _lib.ERR_MEMORY and cnt.value > 0
+
+
+def test_ctypes_structures_match_the_header_layout(tmp_path):
+    """sizeof and field offsets of the descriptor structs, from gcc on the header."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    structs = {"inqamd_system_desc": _lib.SystemDesc,
+               "inqamd_vector_potential_series": _lib.VectorPotentialSeries,
+               "inqamd_td_desc": _lib.TDDesc}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "inq_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, what, val = line.split()
+        py = structs[cname]
+        expect = ctypes.sizeof(py) if what == "size" else getattr(py, what).offset
+        assert int(val) == expect, (cname, what)
+
+
+def test_new_entry_points_reject_null_handles():
+    L = _lib.lib()
+    out = ctypes.c_void_p()
+    d = _lib.TDDesc()
+    assert L.inqamd_td_create(None, ctypes.byref(d), ctypes.byref(out)) == _lib.ERR_ARGUMENT
+    assert L.inqamd_td_forward(None, None, None, 1, None, None) == _lib.ERR_ARGUMENT
+    assert L.inqamd_td_backward(None, 1.0, None, None, None) == _lib.ERR_ARGUMENT
+    assert L.inqamd_td_memory(None) == 0
+    L.inqamd_td_destroy(None)
+    j = (ctypes.c_double * 3)()
+    assert L.inqamd_current(None, None, 1, 1, 2.0, j, j) == _lib.ERR_ARGUMENT
+    assert L.inqamd_batch_create(None, 0, None, ctypes.byref(out)) == _lib.ERR_ARGUMENT
+    assert L.inqamd_batch_apply(None, None, None, None, None) == _lib.ERR_ARGUMENT
+    L.inqamd_batch_destroy(None)
+    ms = (ctypes.c_double * 4)()
+    n = (ctypes.c_int * 4)()
+    assert L.inqamd_hamiltonian_profile(None, None, None, 1, 1, ms, n) == _lib.ERR_ARGUMENT
