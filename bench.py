@@ -130,8 +130,8 @@ def run_uneven_batch(args):
     ref = [o.clone() for o in outs]
     bt = B.hamiltonian_batch(hams)
     t0 = time.perf_counter()
-    best, table = B.autotune(bt, phis, outs, reps=3, chunk_mib=(64, 128, 256), ty_main=(8,),
-                             narrow=(True, False), group=(False, True))
+    best, table = B.autotune(bt, phis, outs, reps=3, chunk_mib=(128, 256, 512), ty_main=(8,),
+                             narrow=(True, False), group=(False, True), nstreams=(1, 2, 4))
     log(f"autotune {time.perf_counter() - t0:.1f} s: best {best}; " +
         ", ".join(f"{k}: {v:.3f} ms" for k, v in sorted(table.items(), key=lambda x: x[1])))
     ms_bat = timed(lambda: bt(phis, outs), args.steps, args.warmup)

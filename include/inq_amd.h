@@ -257,10 +257,11 @@ int inqamd_batch_apply(inqamd_batch_t b, const void *const *psi, void *const *hp
                        const int *nst, const int *ld);
 /* Tiling choices (the autotuner's knobs): W bytes per batch chunk, the y
  * extent of full z tiles (8, 16 or 32 with 64/ty bands), whether band
- * remainders use narrower tiles, and whether each sweep of a chunk is one
- * launch over all its grids (0) or one launch per grid length (1). */
+ * remainders use narrower tiles, whether each sweep of a chunk is one
+ * launch over all its grids (0) or one launch per grid length (1), and over
+ * how many streams (1..4, default priority) the chunks are dealt. */
 int inqamd_batch_set_tuning(inqamd_batch_t b, long chunk_bytes, int ty_main,
-                            int narrow_remainders, int group_by_grid);
+                            int narrow_remainders, int group_by_grid, int nstreams);
 int inqamd_batch_set_stream(inqamd_batch_t b, void *stream);
 /* Cache chunks, kernel launches per apply, band pieces and one-system
  * fallbacks of the last work tables. */

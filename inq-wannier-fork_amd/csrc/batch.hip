@@ -134,7 +134,7 @@ __device__ __forceinline__ void drain_strided(int total, F &&work) {
   }
 }
 
-__global__ __launch_bounds__(256) void batch_phase_kernel(const batch_phase_sys *sys, int nsys,
+__global__ __launch_bounds__(256) void batch_phase_kernel(const batch_phase_sys *__restrict__ sys, int nsys,
                                                           int total) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
@@ -143,19 +143,17 @@ __global__ __launch_bounds__(256) void batch_phase_kernel(const batch_phase_sys 
   phase_item(q.pos, q.uent, q.nsph, q.nent, q.kA, q.phase_sph, q.phase_ent, g - q.item0);
 }
 
-__global__ __launch_bounds__(PE_NT) void batch_project_kernel(const batch_proj_sys *sys, int nsys,
-                                                              unsigned long long *counter,
-                                                              int total) {
-  drain_queue(counter, total, [&](int item) {
-    const int s = uniform(find_segment(nsys, item, [&](int i) { return sys[i].item0; }));
-    const int local = item - sys[s].item0;
-    const int nbt = sys[s].nbt;
-    const project_expand_args a = sys[s].a;
-    project_expand_block(a, local / nbt, (local % nbt) * PE_BANDS);
-  });
+// Projector items (sphere, band tile): one workgroup per item.
+__global__ __launch_bounds__(PE_NT) void batch_project_kernel(
+    const batch_proj_sys *__restrict__ sys, int nsys, unsigned long long *, int) {
+  const int item = blockIdx.x;
+  const int s = find_segment(nsys, item, [&](int i) { return sys[i].item0; });
+  const int local = item - sys[s].item0;
+  const int nbt = sys[s].nbt;
+  project_expand_block(sys[s].a, local / nbt, (local % nbt) * PE_BANDS);
 }
 
-__global__ __launch_bounds__(Z_NT) void batch_z_fwd_kernel(const batch_piece *pc, int np,
+__global__ __launch_bounds__(Z_NT) void batch_z_fwd_kernel(const batch_piece *__restrict__ pc, int np,
                                                            unsigned long long *counter,
                                                            int total) {
   extern __shared__ cplx lds[];
@@ -176,7 +174,7 @@ __global__ __launch_bounds__(Z_NT) void batch_z_fwd_kernel(const batch_piece *pc
   });
 }
 
-__global__ __launch_bounds__(XY_MAX_NT) void batch_xy_kernel(const batch_piece *pc, int np,
+__global__ __launch_bounds__(XY_MAX_NT) void batch_xy_kernel(const batch_piece *__restrict__ pc, int np,
                                                              unsigned long long *counter,
                                                              int total) {
   extern __shared__ cplx lds[];
@@ -194,7 +192,7 @@ __global__ __launch_bounds__(XY_MAX_NT) void batch_xy_kernel(const batch_piece *
   });
 }
 
-__global__ __launch_bounds__(Z_NT) void batch_z_bwd_kernel(const batch_piece *pc, int np,
+__global__ __launch_bounds__(Z_NT) void batch_z_bwd_kernel(const batch_piece *__restrict__ pc, int np,
                                                            unsigned long long *counter,
                                                            int total) {
   extern __shared__ cplx lds[];
@@ -216,40 +214,38 @@ __global__ __launch_bounds__(Z_NT) void batch_z_bwd_kernel(const batch_piece *pc
 }
 
 // One-variant kernels: the pieces of a launch share grid length and tile
-// shape, so their items cost the same and are dealt out in stride.
+// shape, so their items cost the same; one workgroup per item (the item is
+// blockIdx, uniform by construction, so the piece table is read through
+// scalar loads).
 template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(Z_NT) void group_z_fwd_kernel(const batch_piece *pc, int np,
-                                                           unsigned long long *,
-                                                           int total) {
+__global__ __launch_bounds__(Z_NT) void group_z_fwd_kernel(const batch_piece *__restrict__ pc,
+                                                           int np, unsigned long long *, int) {
   extern __shared__ cplx lds[];
-  drain_strided(total, [&](int item) {
-    const int i = uniform(find_segment(np, item, [&](int k) { return pc[k].z0; }));
-    const fft3_args a = pc[i].a;  // a private copy: the tile's stores cannot alias it
-    z_fwd_tile<R0, R1, R2, TY>(a, item - pc[i].z0, lds);
-  });
+  const int item = blockIdx.x;
+  const int i = find_segment(np, item, [&](int k) { return pc[k].z0; });
+  z_fwd_tile<R0, R1, R2, TY>(pc[i].a, item - pc[i].z0, lds);
+}
+
+constexpr int group_xy_threads(int L) {
+  return 2 * plane_threads(L) > 1024 ? 1024 : 2 * plane_threads(L);
 }
 
 template <int R0, int R1, int R2>
-__global__ __launch_bounds__(plane_threads(R0 * R1 * R2)) void group_xy_kernel(
-    const batch_piece *pc, int np, unsigned long long *, int total) {
+__global__ __launch_bounds__(group_xy_threads(R0 * R1 * R2)) void group_xy_kernel(
+    const batch_piece *__restrict__ pc, int np, unsigned long long *, int) {
   extern __shared__ cplx lds[];
-  drain_strided(total, [&](int item) {
-    const int i = uniform(find_segment(np, item, [&](int k) { return pc[k].xy0; }));
-    const fft3_args a = pc[i].a;
-    xy_kin_plane<R0, R1, R2>(a, item - pc[i].xy0, lds);
-  });
+  const int item = blockIdx.x;
+  const int i = find_segment(np, item, [&](int k) { return pc[k].xy0; });
+  xy_kin_plane<R0, R1, R2, group_xy_threads(R0 * R1 * R2)>(pc[i].a, item - pc[i].xy0, lds);
 }
 
 template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(Z_NT) void group_z_bwd_kernel(const batch_piece *pc, int np,
-                                                           unsigned long long *,
-                                                           int total) {
+__global__ __launch_bounds__(Z_NT) void group_z_bwd_kernel(const batch_piece *__restrict__ pc,
+                                                           int np, unsigned long long *, int) {
   extern __shared__ cplx lds[];
-  drain_strided(total, [&](int item) {
-    const int i = uniform(find_segment(np, item, [&](int k) { return pc[k].z0; }));
-    const fft3_args a = pc[i].a;
-    z_bwd_epi_tile<R0, R1, R2, TY>(a, item - pc[i].z0, lds);
-  });
+  const int item = blockIdx.x;
+  const int i = find_segment(np, item, [&](int k) { return pc[k].z0; });
+  z_bwd_epi_tile<R0, R1, R2, TY>(pc[i].a, item - pc[i].z0, lds);
 }
 
 // kernels of one launch group
@@ -326,6 +322,10 @@ struct inqamd_batch {
   int nproj = 0, proj_items = 0;
   batch_piece *d_pieces = nullptr;
   bool group_by_grid = false;      // one launch per grid length instead of per chunk
+  int nstreams = 2;                // streams the chunks are dealt over (after the projector)
+  std::vector<hipStream_t> aux;    // streams 1.. (stream 0 is `stream`)
+  std::vector<hipEvent_t> aux_done;
+  hipEvent_t proj_done = nullptr;
   struct group {  // pieces sharing one launch of each sweep
     int p0, np;
     int L, ty;
@@ -543,11 +543,18 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
     if (g.xylds > 65536)
       INQAMD_HIP_CHECK(hipFuncSetAttribute(g.k.xy, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)g.xylds));
-    g.zgrid_f = resident_grid(g.k.zf, Z_NT, g.zlds, b->cus, g.zitems);
-    g.zgrid_b = resident_grid(g.k.zb, Z_NT, g.zlds, b->cus, g.zitems);
-    g.xygrid = resident_grid(g.k.xy, g.xynt, g.xylds, b->cus, g.xyitems);
+    if (b->group_by_grid) {
+      // one workgroup per item
+      g.zgrid_f = g.zgrid_b = g.zitems;
+      g.xygrid = g.xyitems;
+      g.xynt = group_xy_threads(g.L);
+    } else {
+      g.zgrid_f = resident_grid(g.k.zf, Z_NT, g.zlds, b->cus, g.zitems);
+      g.zgrid_b = resident_grid(g.k.zb, Z_NT, g.zlds, b->cus, g.zitems);
+      g.xygrid = resident_grid(g.k.xy, g.xynt, g.xylds, b->cus, g.xyitems);
+    }
   }
-  b->proj_grid = resident_grid((const void *)batch_project_kernel, PE_NT, 0, b->cus, proj_items);
+  b->proj_grid = std::max(1, proj_items);
   b->d_phase = upload_table(ph);
   b->nphase = (int)ph.size();
   b->phase_items = phase_items;
@@ -588,7 +595,17 @@ void batch_run(inqamd_batch *b) {
                        (const batch_proj_sys *)b->d_proj, b->nproj, b->counters, b->proj_items);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
+  // chunks are independent once the projector has written V_nl psi: deal
+  // them over the streams
+  const int ns = std::min<int>(b->nstreams, 1 + (int)b->aux.size());
+  if (ns > 1) {
+    INQAMD_HIP_CHECK(hipEventRecord(b->proj_done, st));
+    for (int k = 1; k < ns; k++) INQAMD_HIP_CHECK(hipStreamWaitEvent(b->aux[k - 1], b->proj_done, 0));
+  }
+  int ci = 0;
   for (const inqamd_batch::chunk &ch : b->chunks) {
+    const hipStream_t cs = (ci % ns) == 0 ? st : b->aux[(ci % ns) - 1];
+    ci++;
     // z forward of every group of the chunk, then x-y, then z backward
     for (int sweep = 0; sweep < 3; sweep++)
       for (int gi = ch.g0; gi < ch.g0 + ch.ng; gi++) {
@@ -601,8 +618,12 @@ void batch_run(inqamd_batch *b) {
         const size_t lds = sweep == 1 ? g.xylds : g.zlds;
         int np = g.np, total = sweep == 1 ? g.xyitems : g.zitems;
         void *args[] = {(void *)&pc, (void *)&np, (void *)&cnt, (void *)&total};
-        INQAMD_HIP_CHECK(hipLaunchKernel(k, dim3(grid), dim3(nt), args, lds, st));
+        INQAMD_HIP_CHECK(hipLaunchKernel(k, dim3(grid), dim3(nt), args, lds, cs));
       }
+  }
+  for (int k = 1; k < ns; k++) {
+    INQAMD_HIP_CHECK(hipEventRecord(b->aux_done[k - 1], b->aux[k - 1]));
+    INQAMD_HIP_CHECK(hipStreamWaitEvent(st, b->aux_done[k - 1], 0));
   }
 }
 
@@ -628,7 +649,12 @@ void batch_release(inqamd_batch *b) {
   batch_free_tables(b);
   for (hipEvent_t e : b->events)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->aux_done)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t q : b->aux)
+    if (q) (void)hipStreamDestroy(q);
   if (b->done) (void)hipEventDestroy(b->done);
+  if (b->proj_done) (void)hipEventDestroy(b->proj_done);
 }
 
 }  // namespace
@@ -668,6 +694,13 @@ int inqamd_batch_create(inqamd_hamiltonian_t const *handles, int nsys, void *str
     for (int s = 0; s < nsys; s++)
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->events[s], hipEventDisableTiming));
     INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->proj_done, hipEventDisableTiming));
+    b->aux.assign(3, nullptr);
+    b->aux_done.assign(3, nullptr);
+    for (int k = 0; k < 3; k++) {
+      INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&b->aux[k], hipStreamNonBlocking));
+      INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->aux_done[k], hipEventDisableTiming));
+    }
   });
   if (rc != INQAMD_OK) {
     g_batch_create_error() = b->last_error;
@@ -680,15 +713,17 @@ int inqamd_batch_create(inqamd_hamiltonian_t const *handles, int nsys, void *str
 }
 
 int inqamd_batch_set_tuning(inqamd_batch_t b, long chunk_bytes, int ty_main,
-                            int narrow_remainders, int group_by_grid) {
+                            int narrow_remainders, int group_by_grid, int nstreams) {
   if (!b) return INQAMD_ERR_ARGUMENT;
   return batch_guarded(b, [&] {
     INQAMD_REQUIRE(chunk_bytes >= (1L << 20), "chunk_bytes must be at least 1 MiB");
     INQAMD_REQUIRE(ty_main == 8 || ty_main == 16 || ty_main == 32, "ty_main must be 8, 16 or 32");
+    INQAMD_REQUIRE(nstreams >= 1 && nstreams <= 4, "nstreams must be in [1, 4]");
     b->chunk_bytes = chunk_bytes;
     b->ty_main = ty_main;
     b->narrow_remainders = narrow_remainders != 0;
     b->group_by_grid = group_by_grid != 0;
+    b->nstreams = nstreams;
     b->built = false;
   });
 }

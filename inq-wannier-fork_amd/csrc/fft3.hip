@@ -16,7 +16,8 @@ namespace inqamd {
 
 // Shapes of the one-system kernels: z tiles of TY y x 64/TY bands, x-y
 // planes with plane_threads(L) or twice as many threads.  The defaults (8,
-// 1x) can be overridden through INQAMD_Z_TY (8 or 16) and INQAMD_XY_WIDE (1).
+// 2x: measured fastest at 64^3 x 256) can be overridden through INQAMD_Z_TY
+// (8 or 16) and INQAMD_XY_WIDE (0 or 1).
 template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(Z_NT) void z_fwd_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
@@ -50,7 +51,7 @@ static int z_ty_choice() {
 static bool xy_wide_choice() {
   static const bool w = [] {
     const char *e = getenv("INQAMD_XY_WIDE");
-    return e && atoi(e) == 1;
+    return !(e && atoi(e) == 0);
   }();
   return w;
 }

@@ -23,6 +23,12 @@ namespace inqamd {
 constexpr int Z_COLS = 64;  // FFT columns per z tile: TY y values x TB bands
 constexpr int Z_NT = 512;
 
+// LDS slot of column c in a 64-column (1 KB) row of a z tile.  The XOR
+// spreads the transposed (band-fastest) accesses of the psi load and of the
+// epilogue over all 16 bank groups of 16-byte elements, and keeps the
+// FFT stages' contiguous column accesses conflict-free.
+__device__ __forceinline__ int zswz(int c) { return c ^ (((c >> 4) & 3) << 1); }
+
 constexpr int plane_threads(int L) {
   int t = (L * L / 16 + 63) / 64 * 64;
   return t < 64 ? 64 : (t > 1024 ? 1024 : t);
@@ -52,12 +58,12 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
     cplx v = mk(0.0, 0.0);
     if (y0 + y < L && cb0 + b < a.nb)
       v = a.psi[(((long)x * L + y0 + y) * L + z) * a.ld + cb0 + b];
-    lds[z * Z_COLS + b * Z_TY + y] = v;
+    lds[z * Z_COLS + zswz(b * Z_TY + y)] = v;
   }
   __syncthreads();
 
-  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + c]; };
-  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + c] = v; };
+  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + zswz(c)]; };
+  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + zswz(c)] = v; };
   auto gs = [&](int i, int c, cplx v, int, int) {
     const int b = c / Z_TY, y = c % Z_TY;
     if (y0 + y < L && cb0 + b < a.nb)
@@ -158,8 +164,8 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     return (y0 + y < L && cb0 + b < a.nb) ? a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y]
                                           : mk(0.0, 0.0);
   };
-  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + c]; };
-  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + c] = v; };
+  auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + zswz(c)]; };
+  auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + zswz(c)] = v; };
   run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, +1, false, gl, ll, ls, ls, true);
 
   // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
@@ -181,7 +187,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
       const long p = ((long)x * L + y0 + y) * L + z;
       const int cb = cb0 + b;
-      cplx v = cscale(a.scale, lds[z * Z_COLS + b * Z_TY + y]);
+      cplx v = cscale(a.scale, lds[z * Z_COLS + zswz(b * Z_TY + y)]);
       v.x += pre[k].x;
       v.y += pre[k].y;
       if (a.nt_store) {

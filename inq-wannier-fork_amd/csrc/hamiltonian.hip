@@ -203,14 +203,15 @@ void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int ns
 // Taylor accumulations `acc` (offset to b0) in its last sweep; the pencil
 // path leaves both to the caller.
 static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
-                         int slot, int b0, long ldP, const taylor_acc *acc, int sweep = -1) {
+                         int slot, int b0, long ldP, const taylor_acc *acc, int sweep = -1,
+                         hipStream_t stream = nullptr, cplx *W = nullptr) {
   if (h.three_sweep) {
     fft3_args a{};
     a.psi = psi;
     a.ld = ld;
     a.hpsi = hpsi;
     a.hld = hld;
-    a.W = h.work;
+    a.W = W ? W : h.work;
     a.nb = nb;
     a.V = h.V;
     a.kA = h.kA + 3 * slot;
@@ -241,7 +242,7 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       return e && atoi(e) == 1 ? 1 : 0;
     }();
     a.nt_store = nt_store;
-    fft3_local_hamiltonian(a, h.n[0], h.stream, sweep);
+    fft3_local_hamiltonian(a, h.n[0], stream ? stream : h.stream, sweep);
     return;
   }
   cplx *w = h.work;
@@ -267,8 +268,9 @@ static int band_chunk(const handle &h, int nst) {
     return e ? atoi(e) : -1;
   }();
   if (forced >= 0) return forced == 0 ? nst : forced;
-  // about 128 MiB of intermediate per chunk (half the 256 MB Infinity Cache)
-  long c = (128L << 20) / (h.npoint * (long)sizeof(cplx));
+  // about 192 MiB of intermediate per chunk: measured fastest at 64^3 x 256
+  // (48-band chunks; 32 and 64 were 6 % slower)
+  long c = (192L << 20) / (h.npoint * (long)sizeof(cplx));
   c = c < 8 ? 8 : c / 8 * 8;
   return c < nst ? (int)c : nst;
 }
@@ -292,9 +294,29 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
   else
     projector_project_expand(h, psi, ld, nst, nullptr, 0, true);
   const int chunk = band_chunk(h, nst);
-  for (int b0 = 0; b0 < nst; b0 += chunk) {
+  // three-sweep chunks alternate between two streams (and two halves of the
+  // intermediate) so one chunk's launches fill the tails of the other's
+  static const bool pipeline = [] {
+    const char *e = getenv("INQAMD_PIPELINE");
+    return !(e && atoi(e) == 0);
+  }();
+  const bool two = pipeline && h.three_sweep && h.aux && chunk < nst && 2 * chunk <= h.max_bands;
+  if (two) {
+    INQAMD_HIP_CHECK(hipEventRecord(h.ev_fork, h.stream));
+    INQAMD_HIP_CHECK(hipStreamWaitEvent(h.aux, h.ev_fork, 0));
+  }
+  int c = 0;
+  for (int b0 = 0; b0 < nst; b0 += chunk, c++) {
     const int nb = nst - b0 < chunk ? nst - b0 : chunk;
-    local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc);
+    if (two)
+      local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, -1,
+                   (c & 1) ? h.aux : h.stream, h.work + (size_t)(c & 1) * chunk * h.npoint);
+    else
+      local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc);
+  }
+  if (two) {
+    INQAMD_HIP_CHECK(hipEventRecord(h.ev_join, h.aux));
+    INQAMD_HIP_CHECK(hipStreamWaitEvent(h.stream, h.ev_join, 0));
   }
   if (!h.three_sweep) {
     projector_apply_coefficients(h, hpsi, hld, nst, slot);
@@ -408,6 +430,11 @@ static void release(handle &h) {
   h.mom_a = h.mom_b = nullptr;
   h.obs = nullptr;
   h.mom_partial = nullptr;
+  if (h.ev_fork) (void)hipEventDestroy(h.ev_fork);
+  if (h.ev_join) (void)hipEventDestroy(h.ev_join);
+  if (h.aux) (void)hipStreamDestroy(h.aux);
+  h.ev_fork = h.ev_join = nullptr;
+  h.aux = nullptr;
   projector_free(h);
 }
 
@@ -470,7 +497,12 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
     h.three_sweep = !pencil_forced && h.n[0] == h.n[1] && h.n[1] == h.n[2] &&
                     fft3_supported(h.n[0], h.max_lds);
     // kernel attributes are set now so that no later call (or capture) has to
-    if (h.three_sweep) fft3_prepare(h.n[0]);
+    if (h.three_sweep) {
+      fft3_prepare(h.n[0]);
+      INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&h.aux, hipStreamNonBlocking));
+      INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_fork, hipEventDisableTiming));
+      INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_join, hipEventDisableTiming));
+    }
   }
   h.V = (double *)dalloc(h.npoint * sizeof(double));
   INQAMD_HIP_CHECK(hipMemcpy(h.V, d.vloc, h.npoint * sizeof(double), hipMemcpyHostToDevice));

@@ -79,6 +79,8 @@ struct inqamd_hamiltonian {
   double rlat[9] = {};
   double volume = 0.0;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;        // second stream: odd band chunks of Hpsi
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int max_bands = 0;
 
   inqamd::fft_axis_plan axis[3];

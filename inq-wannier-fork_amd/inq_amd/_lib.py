@@ -111,7 +111,7 @@ SIGNATURES = [
     ("inqamd_td_destroy", None, [_vp]),
     ("inqamd_batch_create", _i, [_vp, _i, _vp, ctypes.POINTER(_vp)]),
     ("inqamd_batch_apply", _i, [_vp, _vp, _vp, _vp, _vp]),
-    ("inqamd_batch_set_tuning", _i, [_vp, ctypes.c_long, _i, _i, _i]),
+    ("inqamd_batch_set_tuning", _i, [_vp, ctypes.c_long, _i, _i, _i, _i]),
     ("inqamd_batch_set_stream", _i, [_vp, _vp]),
     ("inqamd_batch_info", _i, [_vp, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     ("inqamd_batch_last_error", ctypes.c_char_p, [_vp]),

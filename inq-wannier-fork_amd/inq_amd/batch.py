@@ -29,20 +29,20 @@ class hamiltonian_batch:
             raise _lib.InqAmdError(rc, (L.inqamd_batch_last_error(None) or b"").decode())
         self._b = b
         self.tuning = dict(chunk_bytes=128 << 20, ty_main=8, narrow_remainders=True,
-                           group_by_grid=False)
+                           group_by_grid=False, nstreams=2)
 
     def _check(self, rc):
         if rc != _lib.OK:
             raise _lib.InqAmdError(rc, (self._lib.inqamd_batch_last_error(self._b) or b"").decode())
 
     def set_tuning(self, chunk_bytes=128 << 20, ty_main=8, narrow_remainders=True,
-                   group_by_grid=False):
+                   group_by_grid=False, nstreams=2):
         self._check(self._lib.inqamd_batch_set_tuning(self._b, int(chunk_bytes), int(ty_main),
                                                       int(bool(narrow_remainders)),
-                                                      int(bool(group_by_grid))))
+                                                      int(bool(group_by_grid)), int(nstreams)))
         self.tuning = dict(chunk_bytes=int(chunk_bytes), ty_main=int(ty_main),
                            narrow_remainders=bool(narrow_remainders),
-                           group_by_grid=bool(group_by_grid))
+                           group_by_grid=bool(group_by_grid), nstreams=int(nstreams))
 
     def __call__(self, phis, outs=None):
         if len(phis) != len(self.hams):
@@ -93,13 +93,13 @@ def _time(fn, reps):
 
 
 def autotune(batch, phis, outs, reps=5, chunk_mib=(32, 64, 128, 256), ty_main=(8, 16),
-             narrow=(True, False), group=(False, True)):
+             narrow=(True, False), group=(False, True), nstreams=(1, 2, 4)):
     """Time every combination of the tiling knobs on the device and keep the
     fastest; returns (best settings, {settings tuple: ms})."""
     results = {}
-    for c, t, nr, g in itertools.product(chunk_mib, ty_main, narrow, group):
-        batch.set_tuning(c << 20, t, nr, g)
-        results[(c, t, nr, g)] = _time(lambda: batch(phis, outs), reps)
+    for c, t, nr, g, ns in itertools.product(chunk_mib, ty_main, narrow, group, nstreams):
+        batch.set_tuning(c << 20, t, nr, g, ns)
+        results[(c, t, nr, g, ns)] = _time(lambda: batch(phis, outs), reps)
     best = min(results, key=results.get)
-    batch.set_tuning(best[0] << 20, best[1], best[2], best[3])
+    batch.set_tuning(best[0] << 20, best[1], best[2], best[3], best[4])
     return dict(batch.tuning), results

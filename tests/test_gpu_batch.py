@@ -64,6 +64,8 @@ def small_batch(device):
     dict(ty_main=32, group_by_grid=True),
     dict(chunk_bytes=1 << 20),
     dict(chunk_bytes=1 << 20, group_by_grid=True, narrow_remainders=False),
+    dict(chunk_bytes=1 << 20, group_by_grid=True, nstreams=4),
+    dict(nstreams=1),
 ])
 def test_batch_matches_per_system(device, small_batch, tuning):
     say("batch small", tuning)

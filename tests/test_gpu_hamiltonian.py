@@ -247,6 +247,33 @@ def test_etrs_graph_replays_match_eager_bitwise(device, cfg0):
     ham_e.close()
 
 
+def test_pipelined_chunks_graph_and_oracle(device):
+    """A 64^3 system whose Hpsi runs two band chunks on two streams (max_bands
+    64, 40 bands, 32-band chunks): Hpsi against the oracle, and a captured
+    ETRS step replayed twice against the eager steps, bitwise."""
+    s = systems.diamond_system(2)
+    psi = systems.random_orbitals(s, 40)
+    say("pipelined chunks 64^3 x 40")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=64)
+    d = to_dev(psi, device)
+    h = to_host(ham(d))
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(psi[..., :3])
+    assert relerr(h[..., :3], ref) < TOL
+    pert = inq_amd.real_time.fourier_series_vector_potential([0.03, -0.02], 0.1)
+    ham_e = inq_amd.ks_hamiltonian(s, max_bands=64)
+    dg, de = d.clone(), d.clone()
+    g = inq_amd.real_time.etrs_graph(ham, dg, 0.02, 4, pert=None)
+    g.pert = pert
+    for step in range(2):
+        g.launch(step * 0.02)
+        inq_amd.real_time.etrs(ham_e, de, step * 0.02, 0.02, pert)
+        torch.cuda.synchronize()
+        assert np.array_equal(to_host(dg).view(np.uint64), to_host(de).view(np.uint64))
+    g.close()
+    ham.close()
+    ham_e.close()
+
+
 def test_etrs_series_graph_matches_oracle(device, cfg0):
     """Series-mode graph: A(t) evaluated on the device from a step counter.
     dt = 0.02 keeps Taylor-4 inside its stability bound on this grid
