@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--write")
     ap.add_argument("--calib-bytes", type=float, default=float(1 << 30))
     ap.add_argument("--per", type=int, default=1, help="Hpsi calls in the traced run")
+    ap.add_argument("--per-pmc", type=int, default=0,
+                    help="Hpsi calls in each counter run (default: --per)")
+    ap.add_argument("--json", help="also write the per-dispatch bytes of each kernel here")
     ap.add_argument("--label", default="")
     args = ap.parse_args()
 
@@ -68,6 +71,7 @@ def main():
               f"factors from a {args.calib_bytes / 2**20:.0f} MiB copyBuffer)")
         print(f"{'kernel':74s} {'grid':>10s} {'read_MB':>9s} {'write_MB':>9s} {'avg_us':>8s} {'GB/s':>8s}")
         tot_r = tot_w = 0.0
+        per_kernel = collections.defaultdict(lambda: [0.0, 0])
         for key in sorted(f, key=lambda k: -sum(x[1] for x in f[k])):
             name, grid = key
             if "copyBuffer" in name or "vectorized_elementwise" in name or key not in w:
@@ -78,10 +82,20 @@ def main():
             n = len(f[key])
             tot_r += rd * n
             tot_w += wr * n
+            base = name.split("(")[0].split("<")[0].split("::")[-1].replace("_kernel", "")
+            per_kernel[base][0] += (rd + wr) * n
+            per_kernel[base][1] += n
             print(f"{short(name):74s} {grid:10d} {rd / 1e6:9.1f} {wr / 1e6:9.1f} {dur / 1e3:8.1f} "
                   f"{(rd + wr) / max(dur, 1):8.0f}")
-        print(f"# total per Hpsi: read {tot_r / args.per / 1e9:.3f} GB, write "
-              f"{tot_w / args.per / 1e9:.3f} GB, traffic {(tot_r + tot_w) / args.per / 1e9:.3f} GB")
+        per = args.per_pmc or args.per
+        print(f"# total per Hpsi: read {tot_r / per / 1e9:.3f} GB, write "
+              f"{tot_w / per / 1e9:.3f} GB, traffic {(tot_r + tot_w) / per / 1e9:.3f} GB")
+        if args.json:
+            import json
+            out = {k: v[0] / max(v[1], 1) for k, v in per_kernel.items()}
+            with open(args.json, "w") as fh:
+                json.dump({"bytes_per_dispatch": out,
+                           "bytes_per_hpsi": (tot_r + tot_w) / per}, fh, indent=1)
 
 
 if __name__ == "__main__":
