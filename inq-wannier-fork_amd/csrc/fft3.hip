@@ -18,10 +18,14 @@ namespace inqamd {
 // planes with plane_threads(L) or twice as many threads.  The defaults (8,
 // 2x: measured fastest at 64^3 x 256) can be overridden through INQAMD_Z_TY
 // (8 or 16) and INQAMD_XY_WIDE (0 or 1).
+// z tiles of the long grids get twice the threads: their 96 KB+ of LDS
+// leaves room for one workgroup per CU
+constexpr int z_threads(int L) { return L >= 80 ? 1024 : Z_NT; }
+
 template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(Z_NT) void z_fwd_kernel(const fft3_args a) {
+__global__ __launch_bounds__(z_threads(R0 * R1 * R2)) void z_fwd_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_fwd_tile<R0, R1, R2, TY>(a, blockIdx.x, lds);
+  z_fwd_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2)>(a, blockIdx.x, lds);
 }
 
 constexpr int wide_threads(int L) {
@@ -35,9 +39,9 @@ __global__ __launch_bounds__(NT) void xy_kin_kernel(const fft3_args a) {
 }
 
 template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(Z_NT) void z_bwd_epi_kernel(const fft3_args a) {
+__global__ __launch_bounds__(z_threads(R0 * R1 * R2)) void z_bwd_epi_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_bwd_epi_tile<R0, R1, R2, TY>(a, blockIdx.x, lds);
+  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2)>(a, blockIdx.x, lds);
 }
 
 static int z_ty_choice() {
@@ -85,7 +89,7 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
   const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
   const size_t plds = (size_t)L * (L + 1) * sizeof(cplx);
   if (sweep < 0 || sweep == 0) {
-    hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(Z_NT), zlds,
+    hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(z_threads(L)), zlds,
                        stream, a);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
@@ -95,7 +99,7 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   if (sweep < 0 || sweep == 2) {
-    hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(Z_NT),
+    hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(z_threads(L)),
                        zlds, stream, a);
     INQAMD_HIP_CHECK(hipGetLastError());
   }

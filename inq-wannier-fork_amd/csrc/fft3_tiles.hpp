@@ -37,7 +37,7 @@ constexpr int plane_threads(int L) {
 // ---------------------------------------------------------------------------
 // sweep 1: z forward, INQ layout -> band-major W
 // ---------------------------------------------------------------------------
-template <int R0, int R1, int R2, int Z_TY>
+template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT>
 __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
@@ -53,7 +53,7 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
 
   // coalesced load: band fastest (TB * 16 B runs of psi)
-  for (int e = threadIdx.x; e < L * Z_COLS; e += Z_NT) {
+  for (int e = threadIdx.x; e < L * Z_COLS; e += NT) {
     const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
     cplx v = mk(0.0, 0.0);
     if (y0 + y < L && cb0 + b < a.nb)
@@ -69,7 +69,7 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
     if (y0 + y < L && cb0 + b < a.nb)
       a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y] = v;
   };
-  run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, -1, true, ll, ll, ls, gs, false);
+  run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, -1, true, ll, ll, ls, gs, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -125,13 +125,13 @@ __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cpl
 // ---------------------------------------------------------------------------
 // sweep 3: z backward, x 1/N + V psi, band-major W -> INQ layout
 // ---------------------------------------------------------------------------
-template <int R0, int R1, int R2, int Z_TY>
+template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT>
 __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
   constexpr int Z_TB = Z_COLS / Z_TY;
-  static_assert(Z_NT % Z_TB == 0, "thread -> band mapping of the epilogue");
-  constexpr int NE = (L * Z_COLS + Z_NT - 1) / Z_NT;  // output elements per thread
+  static_assert(NT % Z_TB == 0, "thread -> band mapping of the epilogue");
+  constexpr int NE = (L * Z_COLS + NT - 1) / NT;  // output elements per thread
 
   const int nyt = (L + Z_TY - 1) / Z_TY;
   const int nbt = (a.nb + Z_TB - 1) / Z_TB;
@@ -146,7 +146,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   int own[NE];
 #pragma unroll
   for (int k = 0; k < NE; k++) {
-    const int e = threadIdx.x + k * Z_NT;
+    const int e = threadIdx.x + k * NT;
     const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
     pre[k] = mk(0.0, 0.0);
     own[k] = -1;
@@ -166,14 +166,14 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   };
   auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + zswz(c)]; };
   auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + zswz(c)] = v; };
-  run_plan<P, R0, R1, R2, Z_COLS, Z_NT>(a.tw, +1, false, gl, ll, ls, ls, true);
+  run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, +1, false, gl, ll, ls, ls, true);
 
   // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
   // points, its ownership entries summed in fixed (sphere, point) order
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     if (own[k] >= 0) {
-      const int e = threadIdx.x + k * Z_NT;
+      const int e = threadIdx.x + k * NT;
       const int cb = cb0 + e % Z_TB;
       const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
       for (int j = 0; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
@@ -182,7 +182,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
 
 #pragma unroll
   for (int k = 0; k < NE; k++) {
-    const int e = threadIdx.x + k * Z_NT;
+    const int e = threadIdx.x + k * NT;
     const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
     if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
       const long p = ((long)x * L + y0 + y) * L + z;

@@ -268,10 +268,11 @@ static int band_chunk(const handle &h, int nst) {
     return e ? atoi(e) : -1;
   }();
   if (forced >= 0) return forced == 0 ? nst : forced;
-  // about 192 MiB of intermediate per chunk: measured fastest at 64^3 x 256
-  // (48-band chunks; 32 and 64 were 6 % slower)
+  // at least 192 MiB of intermediate and 24 bands per chunk: measured
+  // fastest at 64^3 x 256 (48 bands; 32 and 64 were 6 % slower) and at
+  // 96^3 x 512 (24 bands; 8 and 16 were 4-9 % slower)
   long c = (192L << 20) / (h.npoint * (long)sizeof(cplx));
-  c = c < 8 ? 8 : c / 8 * 8;
+  c = c < 24 ? 24 : c / 8 * 8;
   return c < nst ? (int)c : nst;
 }
 

@@ -247,6 +247,21 @@ def test_etrs_graph_replays_match_eager_bitwise(device, cfg0):
     ham_e.close()
 
 
+def test_hpsi_96_matches_oracle(device):
+    """The 3x3x3 supercell grid (96^3, 216 atoms): three-sweep Hpsi with the
+    1024-thread z tiles, 3 bands against the oracle."""
+    s = systems.diamond_system(3)
+    psi = systems.random_orbitals(s, 3)
+    say("hpsi 96^3 x 3")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=3)
+    ham.uniform_vector_potential = (0.02, -0.01, 0.03)
+    h = to_host(ham(to_dev(psi, device)))
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(psi, (0.02, -0.01, 0.03))
+    assert relerr(h, ref) < TOL
+    assert band_relerr(h, ref) < TOL
+    ham.close()
+
+
 def test_pipelined_chunks_graph_and_oracle(device):
     """A 64^3 system whose Hpsi runs two band chunks on two streams (max_bands
     64, 40 bands, 32-band chunks): Hpsi against the oracle, and a captured
