@@ -422,7 +422,7 @@ static void release(handle &h) {
     if (h.axis[a].tw) (void)hipFree(h.axis[a].tw), h.axis[a].tw = nullptr;
   for (void *p : {(void *)h.V, (void *)h.kA, (void *)h.work, (void *)h.hn, (void *)h.hn2,
                   (void *)h.proj, (void *)h.vnl, (void *)h.partial, (void *)h.mom_a,
-                  (void *)h.mom_b, (void *)h.obs, (void *)h.mom_partial})
+                  (void *)h.mom_b, (void *)h.obs, (void *)h.mom_partial, (void *)h.proj_part})
     if (p) (void)hipFree(p);
   h.V = nullptr;
   h.kA = nullptr;
@@ -431,6 +431,7 @@ static void release(handle &h) {
   h.mom_a = h.mom_b = nullptr;
   h.obs = nullptr;
   h.mom_partial = nullptr;
+  h.proj_part = nullptr;
   if (h.ev_fork) (void)hipEventDestroy(h.ev_fork);
   if (h.ev_join) (void)hipEventDestroy(h.ev_join);
   if (h.aux) (void)hipStreamDestroy(h.aux);
@@ -531,6 +532,7 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
     // 64 bands x nlm x 4 weights per workgroup of the moments kernel
     h.mom_partial_elems = MOM_PARTIAL_BLOCKS * 64 * h.pt.max_nlm * 4;
     h.mom_partial = (cplx *)dalloc(h.mom_partial_elems * sizeof(cplx));
+    h.proj_part = (double *)dalloc(PROJ_PART_TILES * 256 * sizeof(double));
   }
   h.obs = (double *)dalloc(observables_scratch_doubles(h) * sizeof(double));
 }

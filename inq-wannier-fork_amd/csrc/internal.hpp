@@ -102,6 +102,7 @@ struct inqamd_hamiltonian {
   inqamd::cplx *mom_a = nullptr; // [nproj][max_nlm][4][max_bands] sphere moments
   inqamd::cplx *mom_b = nullptr; // second operand of pair reductions
   double *obs = nullptr;        // observable partials and result records
+  double *proj_part = nullptr;  // point-chunk GEMM1 tiles of the projector
   inqamd::cplx *mom_partial = nullptr;  // point-chunk partial sums of sphere moments
   long mom_partial_elems = 0;
 
@@ -150,6 +151,8 @@ constexpr int OBS_GS_BLOCKS = 1024;  // blocks (and partial rows) of the G-space
 constexpr int OBS_RECORD = 32;       // doubles per result record
 constexpr int OBS_NRECORDS = 16;
 constexpr int MOM_MAX_CHUNKS = 16;     // point chunks per sphere in the moments kernel
+constexpr int PROJ_MAX_CHUNKS = 16;    // point chunks per sphere in the projector
+constexpr long PROJ_PART_TILES = 4096; // 16 x 16 GEMM1 tiles the chunk buffer holds
 constexpr long MOM_PARTIAL_BLOCKS = 4096;  // workgroups whose partials the buffer holds
 long observables_scratch_doubles(const handle &h);
 // sphere[p][i][b] = sum_lm M[p][lm][i] P[p][lm][b]   (INQ-shaped project output)

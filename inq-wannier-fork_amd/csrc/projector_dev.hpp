@@ -59,13 +59,16 @@ constexpr int PE_UNROLL = 4;  // gather steps whose loads are issued together
 constexpr int PE_NT = 64 * PE_WAVES;
 constexpr int PE_BANDS = 8;
 
-// One workgroup (PE_NT threads) on sphere p and bands [b0, b0 + PE_BANDS).
-__device__ __forceinline__ void project_expand_block(const project_expand_args &a, int p, int b0) {
-  __shared__ double red[PE_WAVES][16][16];
-  __shared__ double Pr[16][16];
+// GEMM1 over the sphere points [ib, ie) of sphere p for bands
+// [b0, b0 + PE_BANDS): rows = lm, k = 4 points, columns = real-doubled
+// bands.  The point indices, phases, psi values and table entries of
+// PE_UNROLL steps are loaded before their MFMAs so the dependent gathers
+// overlap.  The waves' tiles are summed in wave order into out[16][16]
+// (LDS or global), unscaled.
+__device__ __forceinline__ void gemm1_points(const project_expand_args &a, int p, int b0, int ib,
+                                             int ie, double (*red)[16][16], double *out) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int K = a.size[p];
   const int nlm = a.max_nlm;
   const long off = (long)p * a.max_sphere;
   const double *M = a.mat + (long)p * nlm * a.max_sphere;
@@ -73,28 +76,26 @@ __device__ __forceinline__ void project_expand_block(const project_expand_args &
   const int kk = mfma_k(lane);
   const int band = b0 + (col >> 1);
   const bool band_ok = band < a.nst;
-
-  // GEMM1: rows = lm, k = 4 points, columns = real-doubled bands; the point
-  // indices, phases, psi values and table entries of PE_UNROLL steps are
-  // loaded before their MFMAs so the dependent gathers overlap
   v4d acc = {0.0, 0.0, 0.0, 0.0};
   const int lm_a = mfma_row_a(lane);
   constexpr int STEP = PE_WAVES * 4;
-  for (int i0 = wave * 4 + kk; i0 < K; i0 += STEP * PE_UNROLL) {
+  // the loop variable is uniform across the wave (every lane runs every
+  // MFMA); the lane's point i = j0 + kk + u STEP is guarded instead
+  for (int j0 = ib + wave * 4; j0 < ie; j0 += STEP * PE_UNROLL) {
     int idx[PE_UNROLL];
 #pragma unroll
     for (int u = 0; u < PE_UNROLL; u++) {
-      const int i = i0 + u * STEP;
-      idx[u] = i < K ? a.point[off + i] : -1;
+      const int i = j0 + kk + u * STEP;
+      idx[u] = i < ie ? a.point[off + i] : -1;
     }
     cplx ps[PE_UNROLL], ph[PE_UNROLL];
     double av[PE_UNROLL];
 #pragma unroll
     for (int u = 0; u < PE_UNROLL; u++) {
-      const int i = i0 + u * STEP;
+      const int i = j0 + kk + u * STEP;
       ps[u] = (idx[u] >= 0 && band_ok) ? a.psi[(long)idx[u] * a.ld + band] : mk(0.0, 0.0);
-      ph[u] = i < K ? a.phase[off + i] : mk(0.0, 0.0);
-      av[u] = (i < K && lm_a < nlm) ? M[(long)lm_a * a.max_sphere + i] : 0.0;
+      ph[u] = i < ie ? a.phase[off + i] : mk(0.0, 0.0);
+      av[u] = (i < ie && lm_a < nlm) ? M[(long)lm_a * a.max_sphere + i] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < PE_UNROLL; u++) {
@@ -110,29 +111,30 @@ __device__ __forceinline__ void project_expand_block(const project_expand_args &
     const int lm = threadIdx.x >> 4, c = threadIdx.x & 15;  // 16 x 16 tile
     double s = red[0][lm][c];
     for (int w = 1; w < PE_WAVES; w++) s += red[w][lm][c];
-    // GEMM alpha = dV (projector_all.hpp:131-139), then x coeff (:160-164)
-    s = a.dV * s;
-    s = lm < nlm ? s * a.coeff[(long)p * nlm + lm] : 0.0;
-    Pr[lm][c] = s;
+    out[lm * 16 + c] = s;
   }
-  __syncthreads();
-  if (a.P && threadIdx.x < 256) {
-    // P[p][lm][b] for callers that need the coefficients (INQ-shaped project)
-    const int lm = threadIdx.x >> 4, c = threadIdx.x & 15;
-    const int bb = b0 + (c >> 1);
-    if (lm < nlm && bb < a.nst && (c & 1) == 0)
-      a.P[((long)p * nlm + lm) * a.nst + bb] = mk(Pr[lm][c], Pr[lm][c + 1]);
-  }
-  if (!a.Qe) return;
+}
 
-  // GEMM2: rows = 16 points, k = 4 lm per step, columns = real-doubled bands
+// GEMM2 over the sphere points [ib, ie): Q[i][c] = sum_lm M[lm][i] Pr[lm][c],
+// written as Q e^{-i(k+A).r} to the point's ownership-map entry.
+__device__ __forceinline__ void gemm2_points(const project_expand_args &a, int p, int b0, int ib,
+                                             int ie, const double (*Pr)[16]) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nlm = a.max_nlm;
+  const long off = (long)p * a.max_sphere;
+  const double *M = a.mat + (long)p * nlm * a.max_sphere;
+  const int col = mfma_col_b(lane);
+  const int kk = mfma_k(lane);
+  const int band = b0 + (col >> 1);
+  const bool band_ok = band < a.nst;
   const int nkb = (nlm + 3) / 4;
-  for (int i0 = wave * 16; i0 < K; i0 += PE_WAVES * 16) {
+  for (int i0 = ib + wave * 16; i0 < ie; i0 += PE_WAVES * 16) {
     v4d q = {0.0, 0.0, 0.0, 0.0};
     const int ia = i0 + mfma_row_a(lane);
     for (int kb = 0; kb < nkb; kb++) {
       const int lm = kb * 4 + kk;
-      const double av = (ia < K && lm < nlm) ? M[(long)lm * a.max_sphere + ia] : 0.0;
+      const double av = (ia < ie && lm < nlm) ? M[(long)lm * a.max_sphere + ia] : 0.0;
       const double bv = lm < nlm ? Pr[lm][col] : 0.0;
       q = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, q, 0, 0, 0);
     }
@@ -142,7 +144,7 @@ __device__ __forceinline__ void project_expand_block(const project_expand_args &
       // partner lane holds the other component of the same complex value
       const double mine = q[v];
       const double other = __shfl_xor(mine, 1);
-      if (i < K && band_ok) {
+      if (i < ie && band_ok) {
         const cplx ph = a.phase[off + i];  // e^{+i(k+A).r}; apply its conjugate
         const double re = (col & 1) ? other : mine;
         const double im = (col & 1) ? mine : other;
@@ -152,6 +154,71 @@ __device__ __forceinline__ void project_expand_block(const project_expand_args &
       }
     }
   }
+}
+
+// P = dV D (sum of the partial tiles): GEMM alpha = dV (projector_all.hpp:131-139),
+// then x coeff (:160-164); and the INQ-shaped coefficients if requested.
+__device__ __forceinline__ void scale_coefficients(const project_expand_args &a, int p, int b0,
+                                                   double (*Pr)[16]) {
+  const int nlm = a.max_nlm;
+  if (threadIdx.x < 256) {
+    const int lm = threadIdx.x >> 4, c = threadIdx.x & 15;
+    double s = a.dV * Pr[lm][c];
+    Pr[lm][c] = lm < nlm ? s * a.coeff[(long)p * nlm + lm] : 0.0;
+  }
+  __syncthreads();
+  if (a.P && threadIdx.x < 256) {
+    // P[p][lm][b] for callers that need the coefficients (INQ-shaped project)
+    const int lm = threadIdx.x >> 4, c = threadIdx.x & 15;
+    const int bb = b0 + (c >> 1);
+    if (lm < nlm && bb < a.nst && (c & 1) == 0)
+      a.P[((long)p * nlm + lm) * a.nst + bb] = mk(Pr[lm][c], Pr[lm][c + 1]);
+  }
+}
+
+// One workgroup (PE_NT threads) on sphere p and bands [b0, b0 + PE_BANDS).
+__device__ __forceinline__ void project_expand_block(const project_expand_args &a, int p, int b0) {
+  __shared__ double red[PE_WAVES][16][16];
+  __shared__ double Pr[16][16];
+  const int K = a.size[p];
+  gemm1_points(a, p, b0, 0, K, red, &Pr[0][0]);
+  __syncthreads();
+  scale_coefficients(a, p, b0, Pr);
+  if (!a.Qe) return;
+  __syncthreads();
+  gemm2_points(a, p, b0, 0, K, Pr);
+}
+
+// Point-chunked form for few (sphere, band tile) pairs: chunk c of nc covers
+// points [K c / nc, K (c + 1) / nc).  Pass 1 writes each chunk's GEMM1 tile;
+// pass 2 sums the chunk tiles of its (sphere, tile) in chunk order, scales
+// them and expands its own chunk.
+__device__ __forceinline__ int chunk_begin(int K, int c, int nc) {
+  return (int)((long)K * c / nc);
+}
+
+__device__ __forceinline__ void project_chunk_block(const project_expand_args &a, int p, int b0,
+                                                    int c, int nc, double *part) {
+  __shared__ double red[PE_WAVES][16][16];
+  const int K = a.size[p];
+  gemm1_points(a, p, b0, chunk_begin(K, c, nc), chunk_begin(K, c + 1, nc), red, part);
+}
+
+__device__ __forceinline__ void expand_chunk_block(const project_expand_args &a, int p, int b0,
+                                                   int c, int nc, const double *parts) {
+  __shared__ double Pr[16][16];
+  const int K = a.size[p];
+  if (threadIdx.x < 256) {
+    const int e = threadIdx.x;
+    double s = parts[e];
+    for (int k = 1; k < nc; k++) s += parts[k * 256 + e];
+    Pr[e >> 4][e & 15] = s;
+  }
+  __syncthreads();
+  scale_coefficients(a, p, b0, Pr);
+  if (!a.Qe) return;
+  __syncthreads();
+  gemm2_points(a, p, b0, chunk_begin(K, c, nc), chunk_begin(K, c + 1, nc), Pr);
 }
 
 }  // namespace inqamd

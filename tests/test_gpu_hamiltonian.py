@@ -247,6 +247,30 @@ def test_etrs_graph_replays_match_eager_bitwise(device, cfg0):
     ham_e.close()
 
 
+@pytest.mark.parametrize("nst", [4, 64])
+def test_hpsi_48_matches_oracle(device, nst):
+    """The a0 cell on a 48^3 grid (8 atoms, 2501-point spheres): few
+    (sphere, band tile) pairs, so the projector splits each sphere over
+    point chunks."""
+    s = systems.diamond_system(1, grid=48)
+    psi = systems.random_orbitals(s, nst)
+    say("hpsi 48^3 x", nst)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    ham.uniform_vector_potential = (0.02, -0.01, 0.03)
+    d = to_dev(psi, device)
+    h = to_host(ham(d))
+    k = min(nst, 4)
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(psi[..., :k], (0.02, -0.01, 0.03))
+    err = relerr(h[..., :k], ref)
+    vnl = to_host(ham.non_local(d))[..., :k]
+    ref_nl = O.System(s.n, s.lattice, np.zeros(s.n), s.tables).hamiltonian(psi[..., :k], (0.02, -0.01, 0.03)) \
+        - O.System(s.n, s.lattice, np.zeros(s.n)).hamiltonian(psi[..., :k], (0.02, -0.01, 0.03))
+    say("hpsi rel err %.2e, V_nl rel err %.2e" % (err, relerr(vnl, ref_nl)))
+    assert err < TOL
+    assert relerr(vnl, ref_nl) < 1e-11
+    ham.close()
+
+
 def test_hpsi_96_matches_oracle(device):
     """The 3x3x3 supercell grid (96^3, 216 atoms): three-sweep Hpsi with the
     1024-thread z tiles, 3 bands against the oracle."""
