@@ -96,6 +96,11 @@ int inqamd_set_scalar_potential(inqamd_hamiltonian_t h, const double *vloc_devic
  * psi and hpsi must not alias. */
 int inqamd_hamiltonian_apply(inqamd_hamiltonian_t h, const void *psi, void *hpsi, int nst,
                              int ld);
+/* ks_hamiltonian::operator()(orbital_set<fourier_space, complex>)
+ * (ks_hamiltonian.hpp:156-176): phi and hphi in G space (the unnormalised
+ * forward transform of the real-space orbitals), same layout.  Must not alias. */
+int inqamd_hamiltonian_apply_fourier(inqamd_hamiltonian_t h, const void *phi, void *hphi,
+                                     int nst, int ld);
 /* Diagnostic: one Hpsi (as inqamd_hamiltonian_apply, three-sweep path only)
  * with HIP events around each kernel on the handle's stream.  ms[0]: the
  * projector (phases + gather/MFMA/expand), ms[1..3]: the z-forward, x-y and
@@ -132,6 +137,16 @@ int inqamd_projector_apply(inqamd_hamiltonian_t h, const void *sphere, void *hps
  * eps[b] = Re<hpsi_b|psi_b> / <psi_b|psi_b>.  eps is a DEVICE array [nst]. */
 int inqamd_band_energies(inqamd_hamiltonian_t h, const void *hpsi, const void *psi,
                          double *eps, int nst, int ld);
+/* The per-band terms of energy::calculate (energy.hpp:59-87) for one set of
+ * orbitals, with the handle's current k+A:
+ *   eps[b]     = Re<H psi_b|psi_b> / <psi_b|psi_b>             (:74-75)
+ *   normres[b] = dV |H psi_b - eps_b psi_b|^2                  (:76-77)
+ *   nl[b]      = Re<V_nl psi_b|psi_b> / <psi_b|psi_b>          (:81-85)
+ * All three are DEVICE arrays [nst]; the occupation-weighted sums
+ * (occ_sum) are the caller's.  Uses the handle's work buffers, so psi must
+ * not be one of them. */
+int inqamd_energy(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, double *eps,
+                  double *normres, double *nl);
 
 /* ---- propagation --------------------------------------------------------------
  * operations::shift (shift.hpp:33-44): y += (re + i im) x. */

@@ -87,9 +87,9 @@ constexpr long EN_CHUNK = 256;
 
 // Per-chunk partial sums of Re(conj(hpsi) psi) and |psi|^2 for every band
 // (overlap_diagonal_normalized_mult, overlap_diagonal.hpp:118-128).
-__global__ __launch_bounds__(256) void energies_partial_kernel(const cplx *hpsi, const cplx *psi,
-                                                               long ld, int nst, long npoint,
-                                                               double *partial) {
+__global__ __launch_bounds__(256) void energies_partial_kernel(const cplx *hpsi, long hld,
+                                                               const cplx *psi, long ld, int nst,
+                                                               long npoint, double *partial) {
   __shared__ double red[2][EN_LANES][EN_BT];
   const int bb = threadIdx.x % EN_BT;
   const int lane = threadIdx.x / EN_BT;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void energies_partial_kernel(const cplx *hpsi,
   double v = 0.0, nn = 0.0;
   if (b < nst) {
     for (long p = p0 + lane; p < p1; p += EN_LANES) {
-      const cplx h = hpsi[p * ld + b];
+      const cplx h = hpsi[p * hld + b];
       const cplx s = psi[p * ld + b];
       v += h.x * s.x + h.y * s.y;
       nn += s.x * s.x + s.y * s.y;
@@ -116,6 +116,45 @@ __global__ __launch_bounds__(256) void energies_partial_kernel(const cplx *hpsi,
     partial[((long)blockIdx.x * nst + b) * 2 + 0] = v;
     partial[((long)blockIdx.x * nst + b) * 2 + 1] = nn;
   }
+}
+
+// Per-chunk partial sums of |hpsi - eps_b psi|^2 (the residual of
+// energy.hpp:74-77) in column 0 of the partial layout above.
+__global__ __launch_bounds__(256) void residual_partial_kernel(const cplx *hpsi, long hld,
+                                                               const cplx *psi, long ld,
+                                                               const double *eps, int nst,
+                                                               long npoint, double *partial) {
+  __shared__ double red[EN_LANES][EN_BT];
+  const int bb = threadIdx.x % EN_BT;
+  const int lane = threadIdx.x / EN_BT;
+  const int b = blockIdx.y * EN_BT + bb;
+  const long p0 = (long)blockIdx.x * EN_CHUNK;
+  const long p1 = p0 + EN_CHUNK < npoint ? p0 + EN_CHUNK : npoint;
+  double r = 0.0;
+  if (b < nst) {
+    const double e = eps[b];
+    for (long p = p0 + lane; p < p1; p += EN_LANES) {
+      const cplx h = hpsi[p * hld + b];
+      const cplx s = psi[p * ld + b];
+      const double dx = h.x - e * s.x, dy = h.y - e * s.y;
+      r += dx * dx + dy * dy;
+    }
+  }
+  red[lane][bb] = r;
+  __syncthreads();
+  if (lane == 0 && b < nst) {
+    for (int l = 1; l < EN_LANES; l++) r += red[l][bb];
+    partial[((long)blockIdx.x * nst + b) * 2 + 0] = r;
+  }
+}
+
+__global__ void column_sum_kernel(const double *partial, int nchunk, int nst, double scale,
+                                  double *out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nst) return;
+  double v = 0.0;
+  for (int c = 0; c < nchunk; c++) v += partial[((long)c * nst + b) * 2 + 0];
+  out[b] = scale * v;
 }
 
 __global__ void energies_final_kernel(const double *partial, int nchunk, int nst, double *eps) {
@@ -392,12 +431,12 @@ void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double
   }
 }
 
-void band_energies(handle &h, const cplx *hpsi, const cplx *psi, long ld, int nst,
-                          double *eps) {
+void band_energies(handle &h, const cplx *hpsi, long hld, const cplx *psi, long ld, int nst,
+                   double *eps) {
   const int nchunk = (int)((h.npoint + EN_CHUNK - 1) / EN_CHUNK);
   dim3 grid(nchunk, (nst + EN_BT - 1) / EN_BT);
-  hipLaunchKernelGGL(energies_partial_kernel, grid, dim3(256), 0, h.stream, hpsi, psi, ld, nst,
-                     h.npoint, h.partial);
+  hipLaunchKernelGGL(energies_partial_kernel, grid, dim3(256), 0, h.stream, hpsi, hld, psi, ld,
+                     nst, h.npoint, h.partial);
   INQAMD_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(energies_final_kernel, dim3((nst + 63) / 64), dim3(64), 0, h.stream,
                      (const double *)h.partial, nchunk, nst, eps);
@@ -645,6 +684,23 @@ int inqamd_hamiltonian_apply(inqamd_hamiltonian_t h, const void *psi, void *hpsi
   });
 }
 
+int inqamd_hamiltonian_apply_fourier(inqamd_hamiltonian_t h, const void *phi, void *hphi, int nst,
+                                     int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, phi, "phi");
+    check_orbital(*h, hphi, "hphi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(phi != hphi, "phi and hphi must not alias");
+    // ks_hamiltonian.hpp:156-176 is F H F^-1 phi: to_real, the real-space
+    // operator, to_fourier (the kinetic part then passes through the
+    // transforms instead of being added in G space; equal to rounding)
+    fft3d(*h, (const cplx *)phi, ld, h->hn, nst, nst, +1, 1.0 / (double)h->npoint);
+    hamiltonian_apply(*h, h->hn, nst, h->hn2, nst, nst, KA_CURRENT);
+    fft3d(*h, h->hn2, nst, (cplx *)hphi, ld, nst, -1, 1.0);
+  });
+}
+
 int inqamd_non_local(inqamd_hamiltonian_t h, const void *psi, void *vnlpsi, int nst, int ld) {
   if (!h) return INQAMD_ERR_ARGUMENT;
   return guarded(h, [&] {
@@ -716,7 +772,37 @@ int inqamd_band_energies(inqamd_hamiltonian_t h, const void *hpsi, const void *p
     check_orbital(*h, psi, "psi");
     INQAMD_REQUIRE(eps != nullptr, "eps is null");
     check_bands(*h, nst, ld, true);
-    band_energies(*h, (const cplx *)hpsi, (const cplx *)psi, ld, nst, eps);
+    band_energies(*h, (const cplx *)hpsi, ld, (const cplx *)psi, ld, nst, eps);
+  });
+}
+
+int inqamd_energy(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, double *eps,
+                  double *normres, double *nl) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, psi, "psi");
+    check_bands(*h, nst, ld, true);
+    INQAMD_REQUIRE(eps && normres && nl, "outputs are null");
+    const cplx *f = (const cplx *)psi;
+    // residual = H phi; eps = Re<residual|phi>/<phi|phi> (energy.hpp:74-75)
+    hamiltonian_apply(*h, f, ld, h->hn, nst, nst, KA_CURRENT);
+    band_energies(*h, h->hn, nst, f, ld, nst, eps);
+    // normres = dV |residual - eps phi|^2 (:76-77)
+    const int nchunk = (int)((h->npoint + EN_CHUNK - 1) / EN_CHUNK);
+    hipLaunchKernelGGL(residual_partial_kernel, dim3(nchunk, (nst + EN_BT - 1) / EN_BT),
+                       dim3(256), 0, h->stream, (const cplx *)h->hn, (long)nst, f, (long)ld,
+                       (const double *)eps, nst, h->npoint, h->partial);
+    INQAMD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(column_sum_kernel, dim3((nst + 63) / 64), dim3(64), 0, h->stream,
+                       (const double *)h->partial, nchunk, nst, h->volume / (double)h->npoint,
+                       normres);
+    INQAMD_HIP_CHECK(hipGetLastError());
+    // nl = Re<V_nl phi|phi>/<phi|phi> (:81-85)
+    INQAMD_HIP_CHECK(hipMemsetAsync(h->hn2, 0, (size_t)h->npoint * nst * sizeof(cplx), h->stream));
+    projector_phases(*h, KA_CURRENT);
+    projector_project_expand(*h, f, ld, nst, nullptr, 0, true);
+    projector_apply_coefficients(*h, h->hn2, nst, nst, KA_CURRENT);
+    band_energies(*h, h->hn2, nst, f, ld, nst, nl);
   });
 }
 

@@ -88,12 +88,14 @@ SIGNATURES = [
     ("inqamd_set_scalar_potential", _i, [_vp, _vp]),
     ("inqamd_hamiltonian_apply", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_non_local", _i, [_vp, _vp, _vp, _i, _i]),
+    ("inqamd_hamiltonian_apply_fourier", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_hamiltonian_profile", _i, [_vp, _vp, _vp, _i, _i, _c_dbl_p, _c_int_p]),
     ("inqamd_transform_to_fourier", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_transform_to_real", _i, [_vp, _vp, _vp, _i, _i, _i]),
     ("inqamd_projector_project", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_projector_apply", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_band_energies", _i, [_vp, _vp, _vp, _vp, _i, _i]),
+    ("inqamd_energy", _i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     ("inqamd_shift", _i, [_vp, _d, _d, _vp, _vp, _i, _i]),
     ("inqamd_exponential_in_place", _i, [_vp, _d, _d, _i, _vp, _i, _i]),
     ("inqamd_exponential_2_for_1", _i, [_vp, _d, _d, _d, _d, _i, _vp, _vp, _i, _i]),

@@ -156,12 +156,15 @@ class ks_hamiltonian:
         self._h.call("inqamd_set_scalar_potential", ctypes.c_void_p(v.data_ptr()))
         self._vkeep = v
 
-    def __call__(self, phi, out=None):
+    def __call__(self, phi, out=None, fourier=False):
+        """H phi for real-space orbitals (ks_hamiltonian.hpp:132-152) or, with
+        fourier=True, for G-space orbitals (:156-176)."""
         p, nst, ld = _orbital_args(phi)
         if out is None:
             out = torch.empty_like(phi)
         q, _, _ = _orbital_args(out)
-        self._h.call("inqamd_hamiltonian_apply", p, q, nst, ld)
+        name = "inqamd_hamiltonian_apply_fourier" if fourier else "inqamd_hamiltonian_apply"
+        self._h.call(name, p, q, nst, ld)
         return out
 
     def profile(self, phi, out):
@@ -191,6 +194,28 @@ class ks_hamiltonian:
         eps = torch.empty(nst, dtype=torch.float64, device=phi.device)
         self._h.call("inqamd_band_energies", q, p, ctypes.c_void_p(eps.data_ptr()), nst, ld)
         return eps
+
+    def energy(self, phi, occ=None):
+        """The per-band terms of energy::calculate (energy.hpp:59-87).
+
+        Returns device arrays (eigenvalues, normres, nl_me) of length nst:
+        eps_b = Re<H phi_b|phi_b>/<phi_b|phi_b>, normres_b = dV |H phi_b -
+        eps_b phi_b|^2 and nl_b = Re<V_nl phi_b|phi_b>/<phi_b|phi_b>.  With
+        ``occ`` (a sequence of nst occupations) it returns instead the
+        occupation-weighted sums (eigenvalues, non_local) the reference
+        accumulates with occ_sum, plus the normres array.
+        """
+        p, nst, ld = _orbital_args(phi)
+        out = torch.empty(3, nst, dtype=torch.float64, device=phi.device)
+        self._h.call("inqamd_energy", p, nst, ld, ctypes.c_void_p(out[0].data_ptr()),
+                     ctypes.c_void_p(out[1].data_ptr()), ctypes.c_void_p(out[2].data_ptr()))
+        eps, normres, nl = out[0], out[1], out[2]
+        if occ is None:
+            return eps, normres, nl
+        w = torch.as_tensor(occ, dtype=torch.float64, device=phi.device)
+        if w.shape != (nst,):
+            raise ValueError("occ must hold one occupation per band")
+        return float((w * eps).sum()), float((w * nl).sum()), normres
 
     def current(self, phi, occ=2.0):
         """observables::current (current.hpp:25-53) for bands of equal

@@ -303,6 +303,34 @@ static void hamiltonian(const orc_system *s, const double kA_cov[3], int nst, in
   if (s->nproj > 0) apply(s, kA_cov, nst, ld, sphere.data(), hpsi);
 }
 
+/* ks_hamiltonian::operator()(orbital_set<fourier_space>) (ks_hamiltonian.hpp:156-176):
+ * phi_rs = to_real(phi); h_rs = (V + 1/2|k+A|^2) phi_rs + V_nl phi_rs;
+ * h = to_fourier(h_rs); h += laplacian(phi, -1/2, -2 (k+A)^contra) (laplacian_add). */
+static void hamiltonian_fourier(const orc_system *s, const double kA_cov[3], int nst, int ld,
+                                const cplx *phi, cplx *hphi) {
+  cell c(s->lattice);
+  const long N = npoints(s);
+  const long maxK = s->max_sphere;
+  std::vector<cplx> rs((size_t)N * nst), hrs((size_t)N * nst);
+  fft3d(s->n, nst, ld, phi, rs.data(), +1, true);
+  std::vector<cplx> sphere((size_t)std::max<long>(1, s->nproj * maxK * nst));
+  if (s->nproj > 0) project(s, kA_cov, nst, nst, rs.data(), sphere.data());
+  vec3 kA{{kA_cov[0], kA_cov[1], kA_cov[2]}};
+  const double shift = 0.5 * c.norm_cov(kA);
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) hrs[p * nst + b] = (s->vloc[p] + shift) * rs[p * nst + b];
+  if (s->nproj > 0) apply(s, kA_cov, nst, nst, sphere.data(), hrs.data());
+  std::vector<cplx> fs((size_t)N * nst);
+  fft3d(s->n, nst, nst, hrs.data(), fs.data(), -1, false);
+  std::vector<cplx> lap((size_t)N * nst);
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) lap[p * nst + b] = phi[p * ld + b];
+  laplacian_kA(s, kA_cov, nst, lap.data());
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) hphi[p * ld + b] = fs[p * nst + b] + lap[p * nst + b];
+}
+
 /* operations::shift (src/operations/shift.hpp:33-44): phi += factor*shift */
 static void shift_add(long N, int nst, int ld, cplx factor, const cplx *x, cplx *y) {
 #pragma omp parallel for schedule(static)
@@ -614,6 +642,48 @@ int orc_hamiltonian_apply(const orc_system *sys, const double kA_cov[3], int nst
   if (!sys || nst <= 0 || ld < nst) return 1;
   hamiltonian(sys, kA_cov, nst, ld, reinterpret_cast<const cplx *>(psi),
               reinterpret_cast<cplx *>(hpsi));
+  return 0;
+}
+
+/* energy::calculate (src/hamiltonian/energy.hpp:59-107) for one orbital set:
+ * residual = H phi; eps = overlap_diagonal_normalized(residual, phi) (real part);
+ * residual -= eps phi; normres = overlap_diagonal(residual) = dV sum |residual|^2;
+ * nl = overlap_diagonal_normalized(non_local(phi), phi) (real part). */
+int orc_energy(const orc_system *sys, const double kA_cov[3], int nst, int ld, const double *psi_,
+               double *eps, double *normres, double *nl) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  const cplx *psi = reinterpret_cast<const cplx *>(psi_);
+  cell c(sys->lattice);
+  const long N = npoints(sys);
+  const double dV = c.volume / (double)N;
+  std::vector<cplx> res((size_t)N * ld), vnl((size_t)N * ld, 0.0);
+  hamiltonian(sys, kA_cov, nst, ld, psi, res.data());
+  if (sys->nproj > 0) {
+    std::vector<cplx> sphere((size_t)sys->nproj * sys->max_sphere * nst);
+    project(sys, kA_cov, nst, ld, psi, sphere.data());
+    apply(sys, kA_cov, nst, ld, sphere.data(), vnl.data());
+  }
+  for (int b = 0; b < nst; b++) {
+    cplx v = 0.0, nn = 0.0, w = 0.0;
+    for (long p = 0; p < N; p++) {
+      v += std::conj(res[p * ld + b]) * psi[p * ld + b];
+      w += std::conj(vnl[p * ld + b]) * psi[p * ld + b];
+      nn += std::conj(psi[p * ld + b]) * psi[p * ld + b];
+    }
+    eps[b] = std::real(v / nn);
+    nl[b] = std::real(w / nn);
+    double r = 0.0;
+    for (long p = 0; p < N; p++) r += std::norm(res[p * ld + b] - eps[b] * psi[p * ld + b]);
+    normres[b] = dV * r;
+  }
+  return 0;
+}
+
+int orc_hamiltonian_apply_fourier(const orc_system *sys, const double kA_cov[3], int nst, int ld,
+                                  const double *phi, double *hphi) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  hamiltonian_fourier(sys, kA_cov, nst, ld, reinterpret_cast<const cplx *>(phi),
+                      reinterpret_cast<cplx *>(hphi));
   return 0;
 }
 

@@ -27,7 +27,12 @@
  *                       (ks_hamiltonian.hpp:356-397), plane-wave eigenvalues
  *                       with k+A, laplacian plane waves (laplacian.hpp:104-297),
  *                       dense brute-force V_nl, Hermiticity
+ *   orc_hamiltonian_apply_fourier
+ *                       = to_fourier(H to_real(phi)) by the real-space pins;
+ *                       G-space plane-wave eigenvalues with k+A
  *   orc_band_energies   plane-wave eigenvalues, Rayleigh quotient closed form
+ *   orc_energy          eigenstates (zero residual), |H phi|^2 - eps^2 |phi|^2
+ *                       identity, non-local part = H(V=0) - H(V=0, no projectors)
  *   orc_exponential_*   diagonal closed forms in the plane-wave basis
  *                       (exponential.hpp:94-233 pattern), scipy expm
  *   orc_etrs_step       composition of two Taylor-4 half steps; norm drift
@@ -86,6 +91,11 @@ int orc_gaussian_projector_matrix(const double lattice[9], int npoint,
 int orc_hamiltonian_apply(const orc_system *sys, const double kA_cov[3], int nst,
                           int ld, const double *psi, double *hpsi);
 
+/* ks_hamiltonian::operator()(orbital_set<fourier_space, complex>)
+ * (src/hamiltonian/ks_hamiltonian.hpp:156-176) on orbitals in G space. */
+int orc_hamiltonian_apply_fourier(const orc_system *sys, const double kA_cov[3], int nst,
+                                  int ld, const double *phi, double *hphi);
+
 /* projector_all::project / apply (src/hamiltonian/projector_all.hpp:98-232).
  * sphere is [nproj][max_sphere][nst] complex. */
 int orc_projector_project(const orc_system *sys, const double kA_cov[3], int nst,
@@ -97,6 +107,11 @@ int orc_projector_apply(const orc_system *sys, const double kA_cov[3], int nst,
  * (src/hamiltonian/energy.hpp:74-75, overlap_diagonal.hpp:144-171). */
 int orc_band_energies(int npoint, int nst, int ld, const double *hpsi,
                       const double *psi, double *eps);
+
+/* energy::calculate per band (src/hamiltonian/energy.hpp:59-107): eigenvalue,
+ * residual norm dV |H phi - eps phi|^2, non-local expectation value. */
+int orc_energy(const orc_system *sys, const double kA_cov[3], int nst, int ld,
+               const double *psi, double *eps, double *normres, double *nl);
 
 /* operations::exponential_in_place (src/operations/exponential.hpp:21-34). */
 int orc_exponential_in_place(const orc_system *sys, const double kA_cov[3],

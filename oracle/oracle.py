@@ -57,6 +57,10 @@ def lib():
         sysp = ctypes.POINTER(OrcSystem)
         L.orc_hamiltonian_apply.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_hamiltonian_apply_fourier.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_energy.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_projector_project.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_void_p]
         L.orc_projector_apply.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
@@ -223,6 +227,24 @@ class System:
         out = np.empty_like(psi)
         _check(lib().orc_hamiltonian_apply(ctypes.byref(self.s), _dvec(kA), nst, nst, _ptr(psi),
                                            _ptr(out)), "hamiltonian_apply")
+        return out
+
+    def energy(self, psi, kA=(0.0, 0.0, 0.0)):
+        """energy::calculate per band: (eigenvalues, residual norms, non-local)."""
+        psi = np.ascontiguousarray(psi, np.complex128)
+        nst = psi.shape[-1]
+        eps, nr, nl = np.zeros(nst), np.zeros(nst), np.zeros(nst)
+        _check(lib().orc_energy(ctypes.byref(self.s), _dvec(kA), nst, nst, _ptr(psi), _ptr(eps),
+                                _ptr(nr), _ptr(nl)), "energy")
+        return eps, nr, nl
+
+    def hamiltonian_fourier(self, phi, kA=(0.0, 0.0, 0.0)):
+        """ks_hamiltonian::operator() on G-space orbitals (ks_hamiltonian.hpp:156-176)."""
+        phi = np.ascontiguousarray(phi, np.complex128)
+        nst = phi.shape[-1]
+        out = np.zeros_like(phi)
+        _check(lib().orc_hamiltonian_apply_fourier(ctypes.byref(self.s), _dvec(kA), nst, nst,
+                                                   _ptr(phi), _ptr(out)), "hamiltonian_fourier")
         return out
 
     def project(self, psi, kA=(0.0, 0.0, 0.0)):

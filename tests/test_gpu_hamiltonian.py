@@ -271,6 +271,45 @@ def test_hpsi_48_matches_oracle(device, nst):
     ham.close()
 
 
+def test_fourier_space_hpsi_matches_oracle(device, cfg0):
+    """ks_hamiltonian::operator() on G-space orbitals against the oracle's
+    restatement in the reference's order (ks_hamiltonian.hpp:156-176)."""
+    s, psi, osys = cfg0
+    say("fourier-space hpsi")
+    phi_g = O.to_fourier(psi[..., :5])
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    kA = (0.04, -0.02, 0.03)
+    ham.uniform_vector_potential = kA
+    h = to_host(ham(to_dev(phi_g, device), fourier=True))
+    assert relerr(h, osys.hamiltonian_fourier(phi_g, kA)) < TOL
+    ham.close()
+
+
+def test_energy_terms_match_oracle(device, cfg0):
+    """energy::calculate's per-band terms (energy.hpp:59-87): eigenvalues,
+    residual norms and non-local expectation values against the oracle, with
+    a vector potential, and the occupation-weighted sums."""
+    s, psi, osys = cfg0
+    say("energy terms")
+    nst = 7
+    phi = psi[..., :nst]
+    ham = inq_amd.ks_hamiltonian(s, max_bands=16)
+    kA = (0.03, 0.01, -0.02)
+    ham.uniform_vector_potential = kA
+    eps, nr, nl = (to_host(t) for t in ham.energy(to_dev(phi, device)))
+    reps, rnr, rnl = osys.energy(phi, kA)
+    say("energy max abs diffs %.1e %.1e %.1e" % (np.abs(eps - reps).max(),
+                                                 np.abs(nr - rnr).max(), np.abs(nl - rnl).max()))
+    assert relerr(eps, reps) < TOL
+    assert relerr(nr, rnr) < TOL
+    assert relerr(nl, rnl) < TOL
+    occ = np.linspace(2.0, 0.5, nst)
+    e_sum, nl_sum, _ = ham.energy(to_dev(phi, device), occ=occ)
+    assert abs(e_sum - occ @ reps) < TOL * abs(occ @ reps)
+    assert abs(nl_sum - occ @ rnl) < TOL * abs(occ @ rnl)
+    ham.close()
+
+
 def test_hpsi_96_matches_oracle(device):
     """The 3x3x3 supercell grid (96^3, 216 atoms): three-sweep Hpsi with the
     1024-thread z tiles, 3 bands against the oracle."""

@@ -209,3 +209,77 @@ def test_band_energies_rayleigh_quotient():
     pw = np.exp(1j * (r @ G))[..., None]
     e = O.band_energies(free.hamiltonian(pw), pw)
     assert e[0] == pytest.approx(0.5 * G @ G, rel=1e-14)
+
+
+def test_fourier_space_hamiltonian_is_the_conjugated_real_space_one():
+    """ks_hamiltonian.hpp:156-176 = to_fourier(H to_real(phi)): the two
+    orders of the reference's operators agree; with projectors and k+A on a
+    non-orthogonal cell."""
+    lat = np.array([[6.5, 0.0, 0.0], [0.4, 6.0, 0.0], [0.0, -0.3, 7.0]])
+    s = systems.make_system((12, 12, 14), lat, np.array([[0.2, 0.1, -0.3], [3.0, -2.0, 1.5]]),
+                            seed=9, system_id=31)
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    phi_g = systems.random_orbitals(s, 3, ecut=6.0, fourier=True)
+    kA = (0.3, -0.2, 0.45)
+    hg = osys.hamiltonian_fourier(phi_g, kA)
+    ref = O.to_fourier(osys.hamiltonian(O.to_real(phi_g), kA))
+    assert relerr(hg, ref) < 1e-13
+
+
+def test_fourier_space_hamiltonian_plane_wave_eigenvalue():
+    """A plane wave in G space with V = 0 and no projectors: H phi = 1/2|G+k+A|^2 phi."""
+    lat = np.array([[5.0, 0.0, 0.0], [0.5, 6.0, 0.0], [0.0, 0.3, 5.5]])
+    n = (10, 12, 9)
+    osys = O.System(n, lat, np.zeros(n))
+    b = 2 * np.pi * np.linalg.inv(lat).T
+    kA = np.array([0.2, -0.35, 0.15])  # covariant
+    phi = np.zeros(n + (1,), complex)
+    m = (2, -3, 1)
+    phi[m[0] % n[0], m[1] % n[1], m[2] % n[2], 0] = 1.0
+    h = osys.hamiltonian_fourier(phi, kA)
+    G = m[0] * b[0] + m[1] * b[1] + m[2] * b[2]
+    kc = kA @ b / (2 * np.pi)
+    eig = 0.5 * np.sum((G + kc) ** 2)
+    assert abs(h[m[0] % n[0], m[1] % n[1], m[2] % n[2], 0] - eig) < 1e-12 * eig
+    h[m[0] % n[0], m[1] % n[1], m[2] % n[2], 0] = 0
+    assert np.abs(h).max() < 1e-12
+
+
+def test_energy_terms():
+    """energy::calculate (energy.hpp:59-107): plane waves are eigenstates
+    (zero residual, eigenvalue 1/2|G+k+A|^2, no non-local part without
+    projectors); with projectors the residual norm obeys
+    dV|H phi|^2 - eps^2 dV|phi|^2 and the non-local part equals
+    <phi|H(V=0) - H(V=0, no projectors)|phi> / <phi|phi>."""
+    lat = np.diag([6.0, 5.5, 6.5])
+    n = (10, 9, 12)
+    kA = (0.2, -0.1, 0.3)
+    free = O.System(n, lat, np.zeros(n))
+    m = [(1, 0, 0), (0, -2, 1), (2, 1, -1)]
+    pw = np.zeros(n + (3,), complex)
+    b_rec = reciprocal(lat)
+    r = grid_positions(lat, n)
+    for b, mm in enumerate(m):
+        G = sum(mm[d] * b_rec[d] for d in range(3))
+        pw[..., b] = np.exp(1j * (r @ G))
+    eps, nr, nl = free.energy(pw, kA)
+    kc = np.array(kA) @ b_rec / (2 * np.pi)
+    for b, mm in enumerate(m):
+        G = sum(mm[d] * b_rec[d] for d in range(3))
+        assert eps[b] == pytest.approx(0.5 * np.sum((G + kc) ** 2), rel=1e-12)
+    assert np.abs(nr).max() < 1e-20 and np.abs(nl).max() == 0.0
+    s = systems.make_system(n, lat, np.array([[0.3, 0.2, -0.4], [2.0, -1.0, 2.5]]), seed=2,
+                            system_id=41)
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    psi = systems.random_orbitals(s, 3, ecut=5.0)
+    eps, nr, nl = osys.energy(psi, kA)
+    dV = s.volume / s.npoints
+    h = osys.hamiltonian(psi, kA)
+    hn = dV * np.sum(np.abs(h) ** 2, axis=(0, 1, 2))
+    pn = dV * np.sum(np.abs(psi) ** 2, axis=(0, 1, 2))
+    assert np.allclose(nr, hn - eps ** 2 * pn, rtol=1e-9, atol=1e-12)
+    vnl = O.System(s.n, s.lattice, np.zeros(n), s.tables).hamiltonian(psi, kA) - \
+        O.System(s.n, s.lattice, np.zeros(n)).hamiltonian(psi, kA)
+    ref = np.real(np.sum(np.conj(vnl) * psi, axis=(0, 1, 2))) / np.sum(np.abs(psi) ** 2, axis=(0, 1, 2))
+    assert np.allclose(nl, ref, rtol=1e-12, atol=1e-14)
+    assert np.abs(nl).max() > 1e-6
