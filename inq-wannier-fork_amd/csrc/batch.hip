@@ -240,8 +240,9 @@ __global__ __launch_bounds__(group_xy_threads(R0 * R1 * R2)) void group_xy_kerne
 }
 
 template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(Z_NT) void group_z_bwd_kernel(const batch_piece *__restrict__ pc,
-                                                           int np, unsigned long long *, int) {
+__global__ __launch_bounds__(Z_NT)
+__attribute__((amdgpu_waves_per_eu(z_bwd_min_waves(R0 * R1 * R2))))
+void group_z_bwd_kernel(const batch_piece *__restrict__ pc, int np, unsigned long long *, int) {
   extern __shared__ cplx lds[];
   const int item = blockIdx.x;
   const int i = find_segment(np, item, [&](int k) { return pc[k].z0; });

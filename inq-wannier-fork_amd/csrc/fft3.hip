@@ -38,10 +38,12 @@ __global__ __launch_bounds__(NT) void xy_kin_kernel(const fft3_args a) {
   xy_kin_plane<R0, R1, R2, NT>(a, blockIdx.x, lds);
 }
 
-template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(z_threads(R0 * R1 * R2)) void z_bwd_epi_kernel(const fft3_args a) {
+template <int R0, int R1, int R2, int TY, bool ACC>
+__global__ __launch_bounds__(z_threads(R0 * R1 * R2))
+__attribute__((amdgpu_waves_per_eu(z_bwd_min_waves(R0 * R1 * R2))))
+void z_bwd_epi_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2)>(a, blockIdx.x, lds);
+  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), ACC>(a, blockIdx.x, lds);
 }
 
 static int z_ty_choice() {
@@ -78,8 +80,10 @@ static void prepare3() {
   ensure_lds(z_fwd_kernel<R0, R1, R2, 16>, zl);
   ensure_lds(xy_kin_kernel<R0, R1, R2, plane_threads(L)>, pl);
   ensure_lds(xy_kin_kernel<R0, R1, R2, wide_threads(L)>, pl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8>, zl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16>, zl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, false>, zl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, false>, zl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true>, zl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true>, zl);
 }
 
 template <int R0, int R1, int R2, int TY, int NT>
@@ -99,8 +103,12 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   if (sweep < 0 || sweep == 2) {
-    hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(z_threads(L)),
-                       zlds, stream, a);
+    if (a.acc1 || a.acc2)
+      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true>), dim3((unsigned)ztiles),
+                         dim3(z_threads(L)), zlds, stream, a);
+    else
+      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, false>), dim3((unsigned)ztiles),
+                         dim3(z_threads(L)), zlds, stream, a);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
 }

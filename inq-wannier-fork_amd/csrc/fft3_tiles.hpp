@@ -125,7 +125,9 @@ __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cpl
 // ---------------------------------------------------------------------------
 // sweep 3: z backward, x 1/N + V psi, band-major W -> INQ layout
 // ---------------------------------------------------------------------------
-template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT>
+// ACC: the caller may pass Taylor accumulators (acc1/acc2); without it they
+// must be null and the epilogue is a plain store.
+template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool ACC = false>
 __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
@@ -169,51 +171,103 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, +1, false, gl, ll, ls, ls, true);
 
   // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
-  // points, its ownership entries summed in fixed (sphere, point) order
+  // points, its ownership entries summed in fixed (sphere, point) order.
+  // The first entries of all this thread's outputs are loaded back to back
+  // (one memory round trip rather than one per owned output); further
+  // entries (points in overlapping spheres) follow.
+  cplx q[NE];
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    q[k] = mk(0.0, 0.0);
+    if (own[k] >= 0 && (own[k] & OWN_MAX_COUNT) > 0) {
+      const int cb = cb0 + (int)(threadIdx.x + k * NT) % Z_TB;
+      q[k] = a.Qe[(long)(own[k] >> OWN_COUNT_BITS) * a.ldQ + cb];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     if (own[k] >= 0) {
-      const int e = threadIdx.x + k * NT;
-      const int cb = cb0 + e % Z_TB;
+      const int cb = cb0 + (int)(threadIdx.x + k * NT) % Z_TB;
       const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
-      for (int j = 0; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
+      if (cnt > 0) pre[k] = cadd(pre[k], q[k]);
+      for (int j = 1; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
     }
   }
 
+  auto result = [&](int k, int b, int y, int z) {
+    cplx v = cscale(a.scale, lds[z * Z_COLS + zswz(b * Z_TY + y)]);
+    v.x += pre[k].x;
+    v.y += pre[k].y;
+    return v;
+  };
+  auto store = [&](long p, int cb, cplx v) {
+    if (a.nt_store) {
+      double *o = reinterpret_cast<double *>(a.hpsi + p * a.hld + cb);
+      __builtin_nontemporal_store(v.x, o);
+      __builtin_nontemporal_store(v.y, o + 1);
+    } else {
+      a.hpsi[p * a.hld + cb] = v;
+    }
+  };
+  if constexpr (!ACC) {
 #pragma unroll
-  for (int k = 0; k < NE; k++) {
-    const int e = threadIdx.x + k * NT;
-    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-    if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
-      const long p = ((long)x * L + y0 + y) * L + z;
-      const int cb = cb0 + b;
-      cplx v = cscale(a.scale, lds[z * Z_COLS + zswz(b * Z_TY + y)]);
-      v.x += pre[k].x;
-      v.y += pre[k].y;
-      if (a.nt_store) {
-        double *o = reinterpret_cast<double *>(a.hpsi + p * a.hld + cb);
-        __builtin_nontemporal_store(v.x, o);
-        __builtin_nontemporal_store(v.y, o + 1);
-      } else {
-        a.hpsi[p * a.hld + cb] = v;
+    for (int k = 0; k < NE; k++) {
+      const int e = threadIdx.x + k * NT;
+      const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+      if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb)
+        store(((long)x * L + y0 + y) * L + z, cb0 + b, result(k, b, y, z));
+    }
+  } else {
+    // Outputs in groups of G: the group's Taylor accumulators are read
+    // before any of its stores, so their loads overlap instead of each
+    // waiting behind the previous store (the compiler cannot know that the
+    // accumulators alias nothing else here).
+    constexpr int G = NE < 4 ? NE : 4;
+#pragma unroll
+    for (int k0 = 0; k0 < NE; k0 += G) {
+      cplx o1[G], o2[G];
+#pragma unroll
+      for (int g = 0; g < G && k0 + g < NE; g++) {
+        const int e = threadIdx.x + (k0 + g) * NT;
+        const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+        if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
+          const long p = ((long)x * L + y0 + y) * L + z;
+          if (a.acc1) o1[g] = a.acc1[p * a.acc1_ld + cb0 + b];
+          if (a.acc2) o2[g] = a.acc2[p * a.acc2_ld + cb0 + b];
+        }
       }
-      if (a.acc1) {
-        cplx *d = a.acc1 + p * a.acc1_ld + cb;
-        cplx o = *d;
-        o.x += a.c1re * v.x - a.c1im * v.y;
-        o.y += a.c1re * v.y + a.c1im * v.x;
-        *d = o;
-      }
-      if (a.acc2) {
-        cplx *d = a.acc2 + p * a.acc2_ld + cb;
-        cplx o = *d;
-        o.x += a.c2re * v.x - a.c2im * v.y;
-        o.y += a.c2re * v.y + a.c2im * v.x;
-        *d = o;
+#pragma unroll
+      for (int g = 0; g < G && k0 + g < NE; g++) {
+        const int e = threadIdx.x + (k0 + g) * NT;
+        const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+        if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
+          const long p = ((long)x * L + y0 + y) * L + z;
+          const int cb = cb0 + b;
+          const cplx v = result(k0 + g, b, y, z);
+          store(p, cb, v);
+          if (a.acc1) {
+            cplx o = o1[g];
+            o.x += a.c1re * v.x - a.c1im * v.y;
+            o.y += a.c1re * v.y + a.c1im * v.x;
+            a.acc1[p * a.acc1_ld + cb] = o;
+          }
+          if (a.acc2) {
+            cplx o = o2[g];
+            o.x += a.c2re * v.x - a.c2im * v.y;
+            o.y += a.c2re * v.y + a.c2im * v.x;
+            a.acc2[p * a.acc2_ld + cb] = o;
+          }
+        }
       }
     }
   }
 }
+
+// Occupancy floor of the z_bwd_epi kernels (amdgpu_waves_per_eu): at least
+// 4 waves per SIMD (two 512-thread z tiles per CU, or one of 1024) where the
+// tile's registers fit in 128 without spilling; the 72-long tile needs more
+// and keeps 3.
+constexpr int z_bwd_min_waves(int L) { return L == 72 ? 1 : 4; }
 
 // Grid lengths with three-sweep kernels: (L, radices of the plan)
 #define INQAMD_FFT3_TABLE(X) \
