@@ -52,16 +52,14 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
   const int x = (int)(t / nyt);
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
 
-  // coalesced load: band fastest (TB * 16 B runs of psi)
-  for (int e = threadIdx.x; e < L * Z_COLS; e += NT) {
-    const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-    cplx v = mk(0.0, 0.0);
-    if (y0 + y < L && cb0 + b < a.nb)
-      v = a.psi[(((long)x * L + y0 + y) * L + z) * a.ld + cb0 + b];
-    lds[z * Z_COLS + zswz(b * Z_TY + y)] = v;
-  }
-  __syncthreads();
-
+  // The first stage reads psi straight into registers: a wave takes one
+  // butterfly input of all 64 columns (TY y rows x TB bands), i.e. TY runs
+  // of TB * 16 contiguous bytes, each used whole.
+  auto gl = [&](int i, int c) -> cplx {
+    const int b = c / Z_TY, y = c % Z_TY;
+    return (y0 + y < L && cb0 + b < a.nb) ? a.psi[(((long)x * L + y0 + y) * L + i) * a.ld + cb0 + b]
+                                          : mk(0.0, 0.0);
+  };
   auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + zswz(c)]; };
   auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + zswz(c)] = v; };
   auto gs = [&](int i, int c, cplx v, int, int) {
@@ -69,7 +67,7 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
     if (y0 + y < L && cb0 + b < a.nb)
       a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y] = v;
   };
-  run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, -1, true, ll, ll, ls, gs, false);
+  run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, -1, false, gl, ll, ls, gs, false);
 }
 
 // ---------------------------------------------------------------------------
