@@ -7,7 +7,11 @@
 `--pmc WRITE_SIZE` runs (values in KiB per dispatch).  The counters are
 scaled by what they read for the largest __amd_rocclr_copyBuffer dispatch,
 a plain copy of --calib-bytes bytes (read and written once).  --per divides
-totals by the number of Hpsi calls in the run.
+totals by the number of Hpsi calls in the run.  --serial-calls K adds a
+table over the dispatches of the run's last K Hpsi calls only (a bench run
+ends with per-kernel timed calls that launch one kernel at a time on one
+stream, so these durations are the kernels alone, comparable with the
+bench's own event timings).
 """
 import argparse
 import collections
@@ -20,6 +24,25 @@ def kernel_stats(db):
                        "from top_kernels").fetchall()
     con.close()
     return rows
+
+
+def last_calls_stats(db, per, k):
+    """Per kernel: dispatches and mean duration (us) over its dispatches in
+    the last k of `per` Hpsi calls (each kernel launches n/per times per call)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, start, duration from kernels order by start").fetchall()
+    con.close()
+    by = collections.defaultdict(list)
+    for name, start, dur in rows:
+        by[name].append(dur)
+    out = []
+    for name, durs in by.items():
+        if len(durs) % per:
+            continue
+        take = len(durs) // per * k
+        sel = durs[-take:]
+        out.append((name, len(sel), sum(sel) / len(sel) / 1e3))
+    return sorted(out, key=lambda x: -x[1] * x[2])
 
 
 def pmc(db, counter):
@@ -46,6 +69,8 @@ def main():
     ap.add_argument("--per", type=int, default=1, help="Hpsi calls in the traced run")
     ap.add_argument("--per-pmc", type=int, default=0,
                     help="Hpsi calls in each counter run (default: --per)")
+    ap.add_argument("--serial-calls", type=int, default=0,
+                    help="also summarise the dispatches of the last K Hpsi calls of the trace")
     ap.add_argument("--json", help="also write the per-dispatch bytes of each kernel here")
     ap.add_argument("--label", default="")
     args = ap.parse_args()
@@ -55,6 +80,13 @@ def main():
     for name, calls, total, avg, pct in kernel_stats(args.trace):
         # top_kernels reports microseconds
         print(f"{short(name):74s} {calls:6d} {avg:9.2f} {total:10.1f} {pct:6.2f}")
+
+    if args.serial_calls:
+        print()
+        print(f"# the last {args.serial_calls} of {args.per} Hpsi calls (kernels one at a time)")
+        print(f"{'kernel':74s} {'calls':>6s} {'avg_us':>9s}")
+        for name, n, avg in last_calls_stats(args.trace, args.per, args.serial_calls):
+            print(f"{short(name):74s} {n:6d} {avg:9.2f}")
 
     if args.fetch and args.write:
         f = pmc(args.fetch, "FETCH_SIZE")
