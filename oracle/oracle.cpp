@@ -493,6 +493,157 @@ static void current(const orc_system *s, const double kA_cov[3], double occ, int
 /* ===========================================================================
  * C ABI
  * ======================================================================== */
+/* ---------------------------------------------------------------------------
+ * Ground state: laplacian_add, the TPA preconditioner, orthogonalize and the
+ * steepest-descent eigensolver (the GS path of ground_state/calculator.hpp:
+ * 170-180).
+ * ------------------------------------------------------------------------- */
+
+/* operations::laplacian_add, G-space branch (src/operations/laplacian.hpp:24-38):
+ * laplff += factor (-|G|^2 + gradcoeff . G) ff with G covariant = 2 pi m
+ * (fourier_space.hpp:66-81) and gradcoeff contravariant. */
+static void laplacian_add(const int n[3], const cell &c, double factor, const double gradcoeff[3],
+                          int nst, int ld, const cplx *ff, cplx *laplff) {
+#pragma omp parallel for schedule(static)
+  for (int ix = 0; ix < n[0]; ix++)
+    for (int iy = 0; iy < n[1]; iy++)
+      for (int iz = 0; iz < n[2]; iz++) {
+        vec3 g{{2.0 * M_PI * symmetric(ix, n[0]), 2.0 * M_PI * symmetric(iy, n[1]),
+                2.0 * M_PI * symmetric(iz, n[2])}};
+        const double lapl =
+            factor * (-c.norm_cov(g) + (gradcoeff[0] * g[0] + gradcoeff[1] * g[1] + gradcoeff[2] * g[2]));
+        const long p = ((long)ix * n[1] + iy) * n[2] + iz;
+        for (int b = 0; b < nst; b++) laplff[p * ld + b] += lapl * ff[p * ld + b];
+      }
+}
+
+/* operations::overlap_diagonal (src/operations/overlap_diagonal.hpp:25-60):
+ * out[b] = dV sum_p conj(a[p][b]) b[p][b]. */
+static void overlap_diag(long N, int nst, int ld, double dV, const cplx *a, const cplx *bb,
+                         cplx *out) {
+  for (int b = 0; b < nst; b++) {
+    cplx v = 0.0;
+    for (long p = 0; p < N; p++) v += std::conj(a[p * ld + b]) * bb[p * ld + b];
+    out[b] = dV * v;
+  }
+}
+
+/* The Teter-Payne-Allan function of operations::preconditioner
+ * (src/operations/preconditioner.hpp:30-37). */
+static double tpa_k(double x) {
+  const double num = ((8.0 * x + 12.0) * x + 18.0) * x + 27.0;
+  const double den = (((16.0 * x + 8.0) * x + 12.0) * x + 18.0) * x + 27.0;
+  return num / den;
+}
+
+/* operations::preconditioner on fourier_space orbitals (preconditioner.hpp:39-55):
+ * kinetic_b = Re<laplacian(phi, -1/2)|phi> / <phi|phi>, then
+ * phi(G) *= K(1/2 |G|^2 / kinetic_b). */
+static void precondition(const int n[3], const cell &c, int nst, int ld, cplx *phi) {
+  const long N = (long)n[0] * n[1] * n[2];
+  std::vector<double> g2h((size_t)N);
+  for (int ix = 0; ix < n[0]; ix++)
+    for (int iy = 0; iy < n[1]; iy++)
+      for (int iz = 0; iz < n[2]; iz++) {
+        vec3 g{{2.0 * M_PI * symmetric(ix, n[0]), 2.0 * M_PI * symmetric(iy, n[1]),
+                2.0 * M_PI * symmetric(iz, n[2])}};
+        g2h[((long)ix * n[1] + iy) * n[2] + iz] = -0.5 * (-c.norm_cov(g));
+      }
+  std::vector<double> kin((size_t)nst);
+  for (int b = 0; b < nst; b++) {
+    cplx v = 0.0, nn = 0.0;
+    for (long p = 0; p < N; p++) {
+      v += std::conj(g2h[p] * phi[p * ld + b]) * phi[p * ld + b];
+      nn += std::conj(phi[p * ld + b]) * phi[p * ld + b];
+    }
+    kin[b] = std::real(v / nn);
+  }
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) phi[p * ld + b] *= tpa_k(g2h[p] / kin[b]);
+}
+
+/* operations::orthogonalize (src/operations/orthogonalize.hpp:25-32):
+ * S = dV Phi^H Phi (overlap.hpp:26-60); zpotrf on the row-major S gives
+ * S = R R^H with R lower triangular (matrix/cholesky.hpp:29-80);
+ * invert_triangular and rotate_trs (rotate.hpp:147-156) then set
+ * Phi <- Phi R^-H, i.e. Gram-Schmidt in band order. */
+static void orthogonalize(long N, int nst, int ld, double dV, cplx *phi) {
+  std::vector<cplx> S((size_t)nst * nst);
+  for (int i = 0; i < nst; i++)
+    for (int j = 0; j <= i; j++) {
+      cplx v = 0.0;
+      for (long p = 0; p < N; p++) v += std::conj(phi[p * ld + i]) * phi[p * ld + j];
+      S[(size_t)i * nst + j] = dV * v;
+      S[(size_t)j * nst + i] = std::conj(dV * v);
+    }
+  /* S[i][j] = sum_k R[i][k] conj(R[j][k]) */
+  std::vector<cplx> R((size_t)nst * nst, 0.0);
+  for (int j = 0; j < nst; j++) {
+    double d = std::real(S[(size_t)j * nst + j]);
+    for (int k = 0; k < j; k++) d -= std::norm(R[(size_t)j * nst + k]);
+    const double rjj = std::sqrt(d);
+    R[(size_t)j * nst + j] = rjj;
+    for (int i = j + 1; i < nst; i++) {
+      cplx v = S[(size_t)i * nst + j];
+      for (int k = 0; k < j; k++) v -= R[(size_t)i * nst + k] * std::conj(R[(size_t)j * nst + k]);
+      R[(size_t)i * nst + j] = v / rjj;
+    }
+  }
+  /* each point's row x solves x R^H = phi_row by forward substitution */
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < N; p++) {
+    cplx *row = phi + p * ld;
+    for (int j = 0; j < nst; j++) {
+      cplx v = row[j];
+      for (int i = 0; i < j; i++) v -= row[i] * std::conj(R[(size_t)j * nst + i]);
+      row[j] = v / std::real(R[(size_t)j * nst + j]);
+    }
+  }
+}
+
+/* eigensolvers::steepest_descent (src/eigensolvers/steepest_descent.hpp:23-88):
+ * five line minimisations along the (preconditioned) residual, each with the
+ * exact step of the 2x2 Rayleigh-Ritz problem, then orthogonalize. */
+template <class Op, class Prec>
+static void steepest_descent(long N, int nst, int ld, double dV, Op ham, Prec prec, cplx *phi) {
+  const int num_steps = 5;
+  const size_t sz = (size_t)N * ld;
+  std::vector<cplx> hphi(sz), residual(sz), hres(sz);
+  ham(phi, hphi.data());
+  std::vector<cplx> eig(nst), norm(nst), mm0(nst), mm1(nst), mm2(nst), mm3(nst);
+  std::vector<double> lam(nst);
+  for (int istep = 0; istep < num_steps; istep++) {
+    residual = hphi;
+    overlap_diag(N, nst, ld, dV, residual.data(), phi, eig.data());
+    overlap_diag(N, nst, ld, dV, phi, phi, norm.data());
+    std::vector<cplx> evn(nst);
+    for (int b = 0; b < nst; b++) evn[b] = eig[b] / std::real(norm[b]);
+    /* shift(-1, evnorm, phi, residual): residual -= evnorm phi (shift.hpp:19-31) */
+    for (long p = 0; p < N; p++)
+      for (int b = 0; b < nst; b++) residual[p * ld + b] += -1.0 * evn[b] * phi[p * ld + b];
+    prec(residual.data());
+    ham(residual.data(), hres.data());
+    overlap_diag(N, nst, ld, dV, residual.data(), residual.data(), mm0.data());
+    overlap_diag(N, nst, ld, dV, phi, residual.data(), mm1.data());
+    overlap_diag(N, nst, ld, dV, residual.data(), hres.data(), mm2.data());
+    overlap_diag(N, nst, ld, dV, phi, hres.data(), mm3.data());
+    for (int b = 0; b < nst; b++) {
+      const double ca = std::real(mm0[b] * mm3[b] - mm2[b] * mm1[b]);
+      const double cb = std::real(norm[b] * mm2[b] - eig[b] * mm0[b]);
+      const double cc = std::real(eig[b] * mm1[b] - mm3[b] * norm[b]);
+      const double den = cb + std::sqrt(cb * cb - 4.0 * ca * cc);
+      lam[b] = std::fabs(den) < 1e-15 ? 0.0 : 2.0 * cc / den;
+    }
+    for (long p = 0; p < N; p++)
+      for (int b = 0; b < nst; b++) {
+        phi[p * ld + b] += lam[b] * residual[p * ld + b];
+        if (istep != num_steps - 1) hphi[p * ld + b] += lam[b] * hres[p * ld + b];
+      }
+  }
+  orthogonalize(N, nst, ld, dV, phi);
+}
+
 extern "C" {
 
 void orc_set_threads(int nthreads) {
@@ -815,6 +966,66 @@ int orc_etrs_step(const orc_system *sys, const double kA_t[3], const double kA_t
    * half step and apply T(i dt/2, H(t+dt)). */
   exponential_in_place(sys, kA_t, cplx(0.0, dt / 2.0), order, nst, ld, psi);
   exponential_in_place(sys, kA_tdt, cplx(0.0, dt / 2.0), order, nst, ld, psi);
+  return 0;
+}
+
+int orc_laplacian_add(const orc_system *sys, double factor, const double gradcoeff_contra[3],
+                      int nst, int ld, const double *ff, double *laplff) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  cell c(sys->lattice);
+  laplacian_add(sys->n, c, factor, gradcoeff_contra, nst, ld, reinterpret_cast<const cplx *>(ff),
+                reinterpret_cast<cplx *>(laplff));
+  return 0;
+}
+
+int orc_preconditioner(const orc_system *sys, int nst, int ld, double *phi) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  cell c(sys->lattice);
+  precondition(sys->n, c, nst, ld, reinterpret_cast<cplx *>(phi));
+  return 0;
+}
+
+int orc_orthogonalize(long npoint, int nst, int ld, double dV, double *phi) {
+  if (npoint <= 0 || nst <= 0 || ld < nst) return 1;
+  orthogonalize(npoint, nst, ld, dV, reinterpret_cast<cplx *>(phi));
+  return 0;
+}
+
+int orc_steepest_descent(const orc_system *sys, const double kA_cov[3], int precond, int nst,
+                         int ld, double *phi_) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  cell c(sys->lattice);
+  const long N = npoints(sys);
+  /* fourier_space::volume_element (fourier_space.hpp:41-44) */
+  const double dV = c.volume / ((double)N * (double)N);
+  auto ham = [&](const cplx *in, cplx *out) { hamiltonian_fourier(sys, kA_cov, nst, ld, in, out); };
+  auto prec = [&](cplx *r) {
+    if (precond) precondition(sys->n, c, nst, ld, r);
+  };
+  steepest_descent(N, nst, ld, dV, ham, prec, reinterpret_cast<cplx *>(phi_));
+  return 0;
+}
+
+int orc_steepest_descent_dense(int npoint, int nvec, const double *matrix_, double dV, int niter,
+                               double *phi_, double *eigenvalues_) {
+  if (npoint <= 0 || nvec <= 0 || niter < 0) return 1;
+  const cplx *M = reinterpret_cast<const cplx *>(matrix_);
+  cplx *phi = reinterpret_cast<cplx *>(phi_);
+  cplx *ev = reinterpret_cast<cplx *>(eigenvalues_);
+  /* operations::matrix_operator (matrix_operator.hpp:32-43): M phi */
+  auto ham = [&](const cplx *in, cplx *out) {
+    for (int i = 0; i < npoint; i++)
+      for (int b = 0; b < nvec; b++) {
+        cplx v = 0.0;
+        for (int j = 0; j < npoint; j++) v += M[(size_t)i * npoint + j] * in[(size_t)j * nvec + b];
+        out[(size_t)i * nvec + b] = v;
+      }
+  };
+  auto prec = [](cplx *) {};
+  for (int it = 0; it < niter; it++) steepest_descent(npoint, nvec, nvec, dV, ham, prec, phi);
+  std::vector<cplx> mphi((size_t)npoint * nvec);
+  ham(phi, mphi.data());
+  overlap_diag(npoint, nvec, nvec, dV, phi, mphi.data(), ev);
   return 0;
 }
 

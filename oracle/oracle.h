@@ -36,6 +36,14 @@
  *   orc_exponential_*   diagonal closed forms in the plane-wave basis
  *                       (exponential.hpp:94-233 pattern), scipy expm
  *   orc_etrs_step       composition of two Taylor-4 half steps; norm drift
+ *   orc_laplacian_add   plane-wave closed form with factor and gradient term
+ *   orc_preconditioner  TPA closed form on a plane wave (K(1) = 65/81)
+ *   orc_orthogonalize   numpy QR of sqrt(dV) Phi (Q / sqrt(dV), R > 0)
+ *   orc_steepest_descent_dense
+ *                       reference known answers of the 100 x 100 diagonal
+ *                       problem (steepest_descent.hpp:120-166)
+ *   orc_steepest_descent the dense pins above (same template) with the pinned
+ *                       G-space Hamiltonian and preconditioner
  */
 #ifndef INQ_AMD_ORACLE_H
 #define INQ_AMD_ORACLE_H
@@ -151,6 +159,23 @@ int orc_td_current(const orc_system *sys, int ncoef, const double *coef, double 
                    int nst, int ld, double *psi, double *loss);
 
 /* Set the number of OpenMP threads the oracle uses (<=0: library default). */
+/* operations::laplacian_add, G-space (laplacian.hpp:24-38) */
+int orc_laplacian_add(const orc_system *sys, double factor, const double gradcoeff_contra[3],
+                      int nst, int ld, const double *ff, double *laplff);
+/* operations::preconditioner on G-space orbitals (preconditioner.hpp:25-55), in place */
+int orc_preconditioner(const orc_system *sys, int nst, int ld, double *phi);
+/* operations::orthogonalize (orthogonalize.hpp:25-32) with volume element dV */
+int orc_orthogonalize(long npoint, int nst, int ld, double dV, double *phi);
+/* eigensolvers::steepest_descent (steepest_descent.hpp:23-88) on G-space
+ * orbitals with the G-space ks_hamiltonian and (precond != 0) the TPA
+ * preconditioner; in place */
+int orc_steepest_descent(const orc_system *sys, const double kA_cov[3], int precond, int nst,
+                         int ld, double *phi);
+/* niter steepest_descent calls with a dense operator M [npoint][npoint] and no
+ * preconditioner, then eigenvalues[b] = dV <phi_b|M phi_b> (complex) */
+int orc_steepest_descent_dense(int npoint, int nvec, const double *matrix, double dV, int niter,
+                               double *phi, double *eigenvalues);
+
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);
 

@@ -83,6 +83,16 @@ def lib():
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_double)]
+        L.orc_laplacian_add.argtypes = [sysp, ctypes.c_double, _c_dbl_p, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_preconditioner.argtypes = [sysp, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.orc_orthogonalize.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_void_p]
+        L.orc_steepest_descent.argtypes = [sysp, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_void_p]
+        L.orc_steepest_descent_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -238,6 +248,33 @@ class System:
                                 _ptr(nr), _ptr(nl)), "energy")
         return eps, nr, nl
 
+    def laplacian_add(self, ff, laplff, factor=1.0, gradcoeff=(0.0, 0.0, 0.0)):
+        """operations::laplacian_add on G-space arrays (laplacian.hpp:24-38); returns
+        laplff + factor (-|G|^2 + gradcoeff . G) ff (gradcoeff contravariant)."""
+        ff = np.ascontiguousarray(ff, np.complex128)
+        out = np.array(laplff, dtype=np.complex128, order="C", copy=True)
+        nst = ff.shape[-1]
+        _check(lib().orc_laplacian_add(ctypes.byref(self.s), float(factor), _dvec(gradcoeff), nst,
+                                       nst, _ptr(ff), _ptr(out)), "laplacian_add")
+        return out
+
+    def preconditioner(self, phi):
+        """operations::preconditioner on G-space orbitals (preconditioner.hpp:25-55)."""
+        out = np.array(phi, dtype=np.complex128, order="C", copy=True)
+        nst = out.shape[-1]
+        _check(lib().orc_preconditioner(ctypes.byref(self.s), nst, nst, _ptr(out)),
+               "preconditioner")
+        return out
+
+    def steepest_descent(self, phi, kA=(0.0, 0.0, 0.0), precondition=True):
+        """eigensolvers::steepest_descent on G-space orbitals (steepest_descent.hpp:23-88)
+        with the G-space Hamiltonian; returns the new orbitals."""
+        out = np.array(phi, dtype=np.complex128, order="C", copy=True)
+        nst = out.shape[-1]
+        _check(lib().orc_steepest_descent(ctypes.byref(self.s), _dvec(kA), int(bool(precondition)),
+                                          nst, nst, _ptr(out)), "steepest_descent")
+        return out
+
     def hamiltonian_fourier(self, phi, kA=(0.0, 0.0, 0.0)):
         """ks_hamiltonian::operator() on G-space orbitals (ks_hamiltonian.hpp:156-176)."""
         phi = np.ascontiguousarray(phi, np.complex128)
@@ -330,3 +367,24 @@ def band_energies(hpsi, psi):
     _check(lib().orc_band_energies(npoint, nst, nst, _ptr(hpsi), _ptr(psi), _ptr(eps)),
            "band_energies")
     return eps
+
+
+def orthogonalize(phi, dV):
+    """operations::orthogonalize (orthogonalize.hpp:25-32) with volume element dV."""
+    out = np.array(phi, dtype=np.complex128, order="C", copy=True)
+    nst = out.shape[-1]
+    npoint = out.size // nst
+    _check(lib().orc_orthogonalize(npoint, nst, nst, float(dV), _ptr(out)), "orthogonalize")
+    return out
+
+
+def steepest_descent_dense(matrix, phi, dV, niter):
+    """niter eigensolvers::steepest_descent calls with a dense matrix operator and no
+    preconditioner (the reference's unit-test setting); returns (phi, eigenvalues)."""
+    M = np.ascontiguousarray(matrix, np.complex128)
+    out = np.array(phi, dtype=np.complex128, order="C", copy=True)
+    npoint, nvec = out.shape
+    ev = np.zeros(nvec, np.complex128)
+    _check(lib().orc_steepest_descent_dense(npoint, nvec, _ptr(M), float(dV), int(niter),
+                                            _ptr(out), _ptr(ev)), "steepest_descent_dense")
+    return out, ev

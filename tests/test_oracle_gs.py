@@ -1,0 +1,148 @@
+"""Pins the oracle's ground-state pieces: steepest descent, orthogonalize,
+laplacian_add and the TPA preconditioner.
+
+Reference known answers: the eigenvalues after 100 steepest-descent calls on
+the 100 x 100 diagonal problem (src/eigensolvers/steepest_descent.hpp:
+120-166), which exercise the line minimisation, the shift and the Cholesky
+orthogonalisation together.  Mathematics: orthogonalize equals numpy's QR of
+sqrt(dV) Phi (Gram-Schmidt in band order); laplacian_add and the
+preconditioner on plane waves in closed form; the G-space steepest descent
+equals the dense-operator one on the explicit matrix of the same H.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from inq_amd import systems
+from helpers import reciprocal, relerr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KNOWN = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+
+
+def test_steepest_descent_reference_known_answers():
+    k = KNOWN["steepest_descent_diagonal"]
+    npoint, nvec = k["npoint"], k["nvec"]
+    M = np.diag(np.arange(1, npoint + 1).astype(complex))
+    ip = np.arange(npoint)[:, None]
+    iv = np.arange(nvec)[None, :]
+    dV = 1.0 / npoint  # basis::trivial::volume_element
+    phi = O.orthogonalize(np.exp(1j * (ip * iv) * k["phase_step"]), dV)
+    phi, ev = O.steepest_descent_dense(M, phi, dV, k["niter"])
+    # the reference prints 12 decimals
+    assert np.abs(np.abs(ev) - np.array(k["eigenvalues_abs"])).max() < 2e-12
+    assert np.allclose(dV * phi.conj().T @ phi, np.eye(nvec), atol=1e-13)
+
+
+@pytest.mark.parametrize("npoint,nst,dV", [(50, 7, 0.37), (200, 16, 1e-3), (12, 12, 1.0)])
+def test_orthogonalize_is_qr(npoint, nst, dV):
+    rng = np.random.default_rng(npoint + nst)
+    phi = rng.normal(size=(npoint, nst)) + 1j * rng.normal(size=(npoint, nst))
+    out = O.orthogonalize(phi, dV)
+    q, r = np.linalg.qr(np.sqrt(dV) * phi)
+    ph = np.diag(r) / np.abs(np.diag(r))
+    ref = (q * ph[None, :]) / np.sqrt(dV)
+    assert relerr(out, ref) < 1e-12
+    # the first band only changes by its norm (band-order Gram-Schmidt)
+    assert relerr(out[:, 0], phi[:, 0] / np.sqrt(dV * np.vdot(phi[:, 0], phi[:, 0]).real)) < 1e-14
+
+
+def _cell():
+    lat = np.array([[0.0, 3.4, 3.4], [3.4, 0.0, 3.4], [3.4, 3.4, 0.0]])  # FCC
+    return lat, (10, 12, 9)
+
+
+def test_laplacian_add_plane_waves():
+    lat, n = _cell()
+    s = O.System(n, lat, np.zeros(n))
+    b = reciprocal(lat)
+    rng = np.random.default_rng(3)
+    ms = [(1, 0, 0), (0, -2, 1), (3, 1, -2), (-5, 5, 4)]  # inside the symmetric range
+    ff = np.zeros(n + (len(ms),), complex)
+    for j, m in enumerate(ms):
+        ff[m[0] % n[0], m[1] % n[1], m[2] % n[2], j] = 1.0 + 0.5j * j
+    laplff = rng.normal(size=ff.shape) + 1j * rng.normal(size=ff.shape)
+    factor, grad = -0.5, (0.3, -0.1, 0.2)
+    out = O.System(n, lat, np.zeros(n)).laplacian_add(ff, laplff, factor, grad)
+    ref = laplff.copy()
+    for j, m in enumerate(ms):
+        G = m[0] * b[0] + m[1] * b[1] + m[2] * b[2]
+        gdot = 2.0 * np.pi * np.dot(grad, m)  # contravariant . covariant (G_i = 2 pi m_i)
+        ref[m[0] % n[0], m[1] % n[1], m[2] % n[2], j] += factor * (-G @ G + gdot) * ff[
+            m[0] % n[0], m[1] % n[1], m[2] % n[2], j]
+    assert relerr(out - laplff, ref - laplff) < 1e-14
+    # the real-space Laplacian of e^{iG.r} is -|G|^2 e^{iG.r}
+    lap = O.to_real(s.laplacian_add(ff, np.zeros_like(ff), 1.0))
+    pw = O.to_real(ff)
+    for j, m in enumerate(ms):
+        G = m[0] * b[0] + m[1] * b[1] + m[2] * b[2]
+        assert relerr(lap[..., j], -(G @ G) * pw[..., j]) < 1e-12
+
+
+def _tpa(x):
+    return (((8 * x + 12) * x + 18) * x + 27) / ((((16 * x + 8) * x + 12) * x + 18) * x + 27)
+
+
+def test_preconditioner_closed_form():
+    lat, n = _cell()
+    s = O.System(n, lat, np.zeros(n))
+    b = reciprocal(lat)
+    # one plane wave: kinetic = 1/2 |G|^2, so K(1) = 65/81
+    phi = np.zeros(n + (3,), complex)
+    phi[1, 0, 0, 0] = 2.0 - 1.0j
+    ms = [(0, 2, 1), (-3, 1, 2)]
+    amps = [1.0 + 0.5j, -0.3 + 2.0j]
+    for m, a in zip(ms, amps):
+        phi[m[0] % n[0], m[1] % n[1], m[2] % n[2], 1] = a
+    phi[0, 0, 0, 2] = 1.0
+    phi[2, 2, 2, 2] = 0.5
+    out = s.preconditioner(phi)
+    assert out[1, 0, 0, 0] == pytest.approx(65.0 / 81.0 * phi[1, 0, 0, 0], rel=1e-14)
+    assert _tpa(1.0) == pytest.approx(65.0 / 81.0)
+    g2 = [0.5 * np.sum((m[0] * b[0] + m[1] * b[1] + m[2] * b[2]) ** 2) for m in ms]
+    kin = sum(g * abs(a) ** 2 for g, a in zip(g2, amps)) / sum(abs(a) ** 2 for a in amps)
+    for m, a, g in zip(ms, amps, g2):
+        assert out[m[0] % n[0], m[1] % n[1], m[2] % n[2], 1] == pytest.approx(_tpa(g / kin) * a,
+                                                                              rel=1e-13)
+    # G = 0 components are kept (K(0) = 1)
+    assert out[0, 0, 0, 2] == pytest.approx(1.0, rel=1e-15)
+    assert np.count_nonzero(out) == 5
+
+
+def test_steepest_descent_fourier_equals_dense():
+    """The G-space steepest descent (KS operator, no preconditioner) equals the
+    dense-operator one on the explicit matrix of the same H in G space."""
+    lat = np.diag([5.0, 5.5, 4.5])
+    n = (6, 6, 5)
+    s = systems.make_system(n, lat, np.array([[0.4, 0.3, 0.2]]), seed=4, system_id=77)
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    N = int(np.prod(n))
+    kA = (0.1, -0.05, 0.02)
+    # explicit H: columns are H applied to unit vectors of G space
+    eye = np.eye(N, dtype=complex).reshape(n + (N,))
+    H = osys.hamiltonian_fourier(eye, kA).reshape(N, N)
+    assert np.allclose(H, H.conj().T, atol=1e-10 * np.abs(H).max())
+    nst = 4
+    rng = np.random.default_rng(5)
+    phi = rng.normal(size=(N, nst)) + 1j * rng.normal(size=(N, nst))
+    dVG = abs(np.linalg.det(lat)) / N ** 2  # fourier_space::volume_element
+    phi = O.orthogonalize(phi, dVG)
+    a = osys.steepest_descent(phi.reshape(n + (nst,)), kA, precondition=False).reshape(N, nst)
+    b_, _ = O.steepest_descent_dense(H, phi, dVG, 1)
+    assert relerr(a, b_) < 1e-11
+    # and it lowers the Rayleigh quotients' sum, keeping the set orthonormal
+    assert np.allclose(dVG * a.conj().T @ a, np.eye(nst), atol=1e-12)
+    tr0 = np.real(np.trace(phi.conj().T @ H @ phi))
+    tr1 = np.real(np.trace(a.conj().T @ H @ a))
+    assert tr1 < tr0
+    # with the preconditioner it still converges toward the lowest eigenvalues
+    w = np.linalg.eigvalsh(H)[:nst]
+    x = phi.reshape(n + (nst,))
+    for _ in range(40):
+        x = osys.steepest_descent(x, kA, precondition=True)
+    xm = x.reshape(N, nst)
+    ev = np.sort(np.real(np.einsum("pb,pq,qb->b", xm.conj(), H, xm)) * dVG)
+    assert np.abs(ev - w).max() < 1e-6 * np.abs(w).max()
