@@ -68,7 +68,10 @@ typedef struct {
  * Replaces ks_hamiltonian's constructor (ks_hamiltonian.hpp:79-89) and
  * projector_all(projectors) (projector_all.hpp:86-94).  `stream` is a
  * hipStream_t (NULL = default stream).  Allocates every workspace the later
- * calls need, so they allocate nothing. */
+ * calls need, so they allocate nothing -- except the ground-state calls
+ * (orthogonalize, preconditioner, steepest_descent, subspace_diagonalization),
+ * which allocate their own
+ * workspace on first use. */
 int inqamd_hamiltonian_create(const inqamd_system_desc *desc, void *stream,
                               inqamd_hamiltonian_t *out);
 void inqamd_hamiltonian_destroy(inqamd_hamiltonian_t h);
@@ -147,6 +150,35 @@ int inqamd_band_energies(inqamd_hamiltonian_t h, const void *hpsi, const void *p
  * not be one of them. */
 int inqamd_energy(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, double *eps,
                   double *normres, double *nl);
+
+/* ---- ground state (G-space orbitals) --------------------------------------------
+ * operations::laplacian_add, fourier_space branch (laplacian.hpp:24-38):
+ * laplff += factor (-|G|^2 + gradcoeff . G) ff, G covariant = 2 pi m,
+ * gradcoeff contravariant. */
+int inqamd_laplacian_add(inqamd_hamiltonian_t h, const void *ff, void *laplff, int nst, int ld,
+                         double factor, const double gradcoeff_contra[3]);
+/* operations::preconditioner on fourier_space orbitals (preconditioner.hpp:
+ * 25-55, Teter-Payne-Allan): phi(G) *= K(1/2|G|^2 / kinetic_b), in place. */
+int inqamd_preconditioner(inqamd_hamiltonian_t h, void *phi, int nst, int ld);
+/* operations::orthogonalize (orthogonalize.hpp:25-32): Cholesky of
+ * S = dV Phi^H Phi and Phi <- Phi R^-H (Gram-Schmidt in band order), in
+ * place; dV of real space (fourier = 0) or G space (fourier != 0).
+ * Synchronises the handle's stream (the factorisation runs on the host). */
+int inqamd_orthogonalize(inqamd_hamiltonian_t h, void *phi, int nst, int ld, int fourier);
+/* eigensolvers::steepest_descent (steepest_descent.hpp:23-88) on fourier_space
+ * orbitals with ks_hamiltonian's G-space operator (ks_hamiltonian.hpp:156-176)
+ * and, if precondition != 0, operations::preconditioner: five line
+ * minimisations then orthogonalize, in place.  Allocates three [N][max_bands]
+ * work arrays on first use (kept until destroy). */
+int inqamd_steepest_descent(inqamd_hamiltonian_t h, void *phi, int nst, int ld,
+                            int precondition);
+/* ground_state::subspace_diagonalization (subspace_diagonalization.hpp:16-25)
+ * on real-space orbitals: H_sub = dV Phi^H (H Phi), its eigenpairs
+ * (matrix::diagonalize), Phi <- Phi V in place; eigenvalues (ascending) into
+ * the HOST array eigenvalues[nst].  Eigenvector phases (and the basis of a
+ * degenerate eigenspace) are the eigensolver's.  Synchronises. */
+int inqamd_subspace_diagonalization(inqamd_hamiltonian_t h, void *phi, int nst, int ld,
+                                    double *eigenvalues);
 
 /* ---- propagation --------------------------------------------------------------
  * operations::shift (shift.hpp:33-44): y += (re + i im) x. */

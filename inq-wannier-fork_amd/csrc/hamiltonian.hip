@@ -479,6 +479,7 @@ static void *dalloc(size_t bytes) {
 }
 
 static void release(handle &h) {
+  gs_release(h);
   for (int a = 0; a < 3; a++)
     if (h.axis[a].tw) (void)hipFree(h.axis[a].tw), h.axis[a].tw = nullptr;
   for (void *p : {(void *)h.V, (void *)h.kA, (void *)h.work, (void *)h.hn, (void *)h.hn2,
@@ -719,9 +720,7 @@ int inqamd_hamiltonian_apply_fourier(inqamd_hamiltonian_t h, const void *phi, vo
     // ks_hamiltonian.hpp:156-176 is F H F^-1 phi: to_real, the real-space
     // operator, to_fourier (the kinetic part then passes through the
     // transforms instead of being added in G space; equal to rounding)
-    fft3d(*h, (const cplx *)phi, ld, h->hn, nst, nst, +1, 1.0 / (double)h->npoint);
-    hamiltonian_apply(*h, h->hn, nst, h->hn2, nst, nst, KA_CURRENT);
-    fft3d(*h, h->hn2, nst, (cplx *)hphi, ld, nst, -1, 1.0);
+    hamiltonian_fourier(*h, (const cplx *)phi, ld, (cplx *)hphi, ld, nst);
   });
 }
 

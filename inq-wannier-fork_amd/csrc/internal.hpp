@@ -108,6 +108,8 @@ struct inqamd_hamiltonian {
 
   inqamd::projector_tables pt;
 
+  void *gs = nullptr;           // steepest-descent workspace and BLAS handle (gs.hip)
+
   std::string last_error;
 };
 
@@ -146,6 +148,9 @@ void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, 
 void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst);
 void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
                           int nst, int slot);
+// gs.hip
+void hamiltonian_fourier(handle &h, const cplx *phi, long ld, cplx *hphi, long hld, int nst);
+void gs_release(handle &h);
 // td.hip: observables scratch of a handle (sizes from max_bands and tables)
 constexpr int OBS_GS_BLOCKS = 1024;  // blocks (and partial rows) of the G-space reductions
 constexpr int OBS_RECORD = 32;       // doubles per result record

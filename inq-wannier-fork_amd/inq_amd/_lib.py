@@ -96,6 +96,11 @@ SIGNATURES = [
     ("inqamd_projector_apply", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_band_energies", _i, [_vp, _vp, _vp, _vp, _i, _i]),
     ("inqamd_energy", _i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    ("inqamd_laplacian_add", _i, [_vp, _vp, _vp, _i, _i, _d, _c_dbl_p]),
+    ("inqamd_preconditioner", _i, [_vp, _vp, _i, _i]),
+    ("inqamd_orthogonalize", _i, [_vp, _vp, _i, _i, _i]),
+    ("inqamd_steepest_descent", _i, [_vp, _vp, _i, _i, _i]),
+    ("inqamd_subspace_diagonalization", _i, [_vp, _vp, _i, _i, _c_dbl_p]),
     ("inqamd_shift", _i, [_vp, _d, _d, _vp, _vp, _i, _i]),
     ("inqamd_exponential_in_place", _i, [_vp, _d, _d, _i, _vp, _i, _i]),
     ("inqamd_exponential_2_for_1", _i, [_vp, _d, _d, _d, _d, _i, _vp, _vp, _i, _i]),

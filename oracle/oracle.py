@@ -275,6 +275,20 @@ class System:
                                           nst, nst, _ptr(out)), "steepest_descent")
         return out
 
+    def subspace_diagonalization(self, phi, kA=(0.0, 0.0, 0.0)):
+        """ground_state::subspace_diagonalization (subspace_diagonalization.hpp:16-25)
+        on real-space orbitals: H_sub = dV Phi^H (H Phi) (overlap.hpp:26-60), its
+        eigenpairs (numpy eigh, ascending), Phi <- Phi V.  Returns (eigenvalues, phi)."""
+        phi = np.ascontiguousarray(phi, np.complex128)
+        nst = phi.shape[-1]
+        N = int(np.prod(self.n))
+        dV = abs(np.linalg.det(np.asarray(self.lattice, float))) / N
+        hphi = self.hamiltonian(phi, kA).reshape(N, nst)
+        m = phi.reshape(N, nst)
+        hs = dV * (m.conj().T @ hphi)
+        w, v = np.linalg.eigh(0.5 * (hs + hs.conj().T))
+        return w, (m @ v).reshape(phi.shape)
+
     def hamiltonian_fourier(self, phi, kA=(0.0, 0.0, 0.0)):
         """ks_hamiltonian::operator() on G-space orbitals (ks_hamiltonian.hpp:156-176)."""
         phi = np.ascontiguousarray(phi, np.complex128)

@@ -146,3 +146,31 @@ def test_steepest_descent_fourier_equals_dense():
     xm = x.reshape(N, nst)
     ev = np.sort(np.real(np.einsum("pb,pq,qb->b", xm.conj(), H, xm)) * dVG)
     assert np.abs(ev - w).max() < 1e-6 * np.abs(w).max()
+
+
+def test_subspace_diagonalization_plane_waves():
+    """Orthonormal mixtures of plane waves in a free cell: the subspace
+    eigenvalues are the plane waves' 1/2|G+k+A|^2 and the rotated orbitals
+    are those plane waves up to a phase."""
+    lat = np.diag([6.0, 5.0, 7.0])
+    n = (10, 9, 12)
+    s = O.System(n, lat, np.zeros(n))
+    b = reciprocal(lat)
+    kA = (0.2, -0.1, 0.3)
+    ms = [(0, 0, 0), (1, 0, 0), (0, -1, 2), (-2, 1, 1)]
+    from helpers import grid_positions
+    r = grid_positions(lat, n)
+    N = int(np.prod(n))
+    dV = abs(np.linalg.det(lat)) / N
+    pw = np.stack([np.exp(1j * (r @ (m[0] * b[0] + m[1] * b[1] + m[2] * b[2]))) for m in ms], -1)
+    pw /= np.sqrt(dV * N)
+    rng = np.random.default_rng(9)
+    q, _ = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))
+    mix = (pw.reshape(N, 4) @ q).reshape(n + (4,))
+    w, rot = s.subspace_diagonalization(mix, kA)
+    kc = np.array(kA) @ b / (2 * np.pi)
+    ref = sorted(0.5 * np.sum((m[0] * b[0] + m[1] * b[1] + m[2] * b[2] + kc) ** 2) for m in ms)
+    assert np.allclose(w, ref, rtol=1e-12, atol=1e-13)
+    eps, nr, _ = s.energy(rot, kA)
+    assert np.allclose(eps, w, rtol=1e-12, atol=1e-13)
+    assert np.abs(nr).max() < 1e-20
