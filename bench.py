@@ -2,7 +2,7 @@
 one MI355X.  Prints ONE JSON line on stdout; diagnostics go to stderr.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--cpu-seconds 20]
-                    [--workload hpsi_64|uneven_batch|adjoint_48|etrs_96]
+                    [--workload hpsi_64|uneven_batch|adjoint_48|etrs_96|adjoint_96|gs_64]
 
 hpsi_64 (default): band-Hpsi/s at the 64^3 grid x 256 bands FP64 workload
   (2x2x2 diamond supercell, a = 20.52 bohr, 64 atoms x 4 projectors, Gamma);
@@ -13,6 +13,8 @@ adjoint_48: frozen-potential ETRS at 48^3 x 64 bands under a 32-term A(t),
   400 steps with checkpoints every 20; forward + adjoint cost.
 etrs_96: the graph-captured ETRS step at 96^3 x 512 bands (216 atoms), and
   the adjoint of 100 steps with checkpoints every 10.
+gs_64: one frozen-potential ground-state iteration (subspace diagonalisation,
+  preconditioned steepest descent, orthogonalize) at 64^3 x 256 bands.
 """
 import argparse
 import json
@@ -311,10 +313,87 @@ def run_etrs_96(args):
     return out
 
 
+def _gs_bytes(N, nst, sum_k):
+    """Algorithmic bytes of one ground-state iteration (subspace
+    diagonalisation, to_fourier, steepest descent, to_real) on nst bands:
+    seven Hpsi at B_H (one in the subspace step, six G-space ones in steepest
+    descent, each of those with two more transforms at 32N), the orbitals'
+    two transforms, the five line-search passes (256N: 16 B per array read or
+    written) and the two rotations (read + write, 32N each)."""
+    b_h = algorithmic_bytes_per_band(N, nst, sum_k)
+    return nst * (7.0 * b_h + 6 * 64.0 * N + 64.0 * N + 5 * 256.0 * N + 2 * 32.0 * N)
+
+
+def run_gs_64(args):
+    """One iteration of INQ's ground-state loop for a frozen potential
+    (ground_state/calculator.hpp:150-180 without the density update) on the
+    64^3 x 256 system: subspace diagonalisation, to_fourier, steepest descent
+    (five preconditioned line minimisations + orthogonalize), to_real."""
+    import numpy as np
+    import torch
+
+    import inq_amd
+    from inq_amd import ground_state
+    from inq_amd.operations import transform
+
+    s = inq_amd.systems.diamond_system(2)
+    nst = 256
+    psi = inq_amd.systems.random_orbitals(s, nst)
+    d_psi = torch.from_numpy(psi).to("cuda:0")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    fphi = torch.empty_like(d_psi)
+
+    def step():
+        ground_state.subspace_diagonalization(ham, d_psi)
+        transform.to_fourier(d_psi, ham, out=fphi)
+        ground_state.steepest_descent(ham, fphi, True)
+        transform.to_real(fphi, ham, out=d_psi)
+
+    ms = timed(step, args.steps, args.warmup)
+    eps, normres, _ = ham.energy(d_psi)
+    log(f"GS iteration {ms:.2f} ms; eigenvalue sum {float(eps.sum()):.10f}, max normres "
+        f"{float(normres.max()):.3e}")
+    N = s.npoints
+    sum_k = int(np.sum(s.tables["sphere_size"]))
+    it_bytes = _gs_bytes(N, nst, sum_k)
+    achieved = it_bytes / (ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+        nb = 16
+        sample = np.ascontiguousarray(psi[..., :nb])
+        t0 = time.perf_counter()
+        _, x = osys.subspace_diagonalization(sample)
+        osys.steepest_descent(O.to_fourier(x), (0.0, 0.0, 0.0), True)
+        dt = time.perf_counter() - t0
+        cpu = {"value": nb / dt, "unit": "band-iterations/s", "cores": O.get_threads(),
+               "kind": "port",
+               "sample": f"{nb} of {nst} bands of the same 64^3 system, one iteration "
+                         f"(subspace diagonalisation + steepest descent), C++ oracle with "
+                         f"numpy eigh, OpenMP {O.get_threads()} threads"}
+        log(f"cpu oracle: {nb} bands, one iteration in {dt:.2f} s")
+    ham.close()
+    cfg = {"workload": "gs_64: 64^3 grid, 256 bands complex128, 2x2x2 diamond supercell "
+                       "a=20.52 bohr, 64 atoms x 4 projectors; one ground-state iteration "
+                       "(subspace diagonalisation, steepest descent with the TPA "
+                       "preconditioner, orthogonalize) for a frozen potential",
+           "grid": list(s.n), "bands": nst, "atoms": s.natoms, "parallelism": "single GPU"}
+    out = common_fields(args, nst * 1e3 / ms, "band-iterations/s", ms, True, cfg)
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "one ground-state iteration (all launches)",
+                       "algorithmic_bytes_per_step": it_bytes}
+    out["cpu_baseline"] = cpu
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="hpsi_64",
-                    choices=["hpsi_64", "uneven_batch", "adjoint_48", "etrs_96", "adjoint_96"])
+                    choices=["hpsi_64", "uneven_batch", "adjoint_48", "etrs_96", "adjoint_96",
+                             "gs_64"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -346,6 +425,9 @@ def main():
                           "Taylor-4 ETRS, 100 steps of dt=0.03, checkpoints every 10; "
                           "loss J_x(T)", fd_coefs=1)
         print(json.dumps(out), flush=True)
+        return
+    if args.workload == "gs_64":
+        print(json.dumps(run_gs_64(args)))
         return
     if args.workload == "etrs_96":
         print(json.dumps(run_etrs_96(args)), flush=True)

@@ -15,9 +15,10 @@
 //   H residual (G-space Hpsi)
 //   pass 4  <r|r>, <phi|r>, <r|Hr>, <phi|Hr> and the step lambda per band
 //   pass 5  phi += lambda r, hphi += lambda Hr
-// orthogonalize forms S = dV Phi^H Phi with rocBLAS ZHERK, factors it on the
-// host (S = R R^H, R lower) and applies Phi <- Phi R^-H with ZGEMM; the
-// subspace diagonalisation diagonalises dV Phi^H H Phi with rocSOLVER ZHEEVD.
+// orthogonalize forms S = dV Phi^H Phi (a split-K strided-batched ZGEMM),
+// factors it on the host (S = R R^H, R lower) and applies Phi <- Phi R^-H
+// with ZGEMM; the subspace diagonalisation diagonalises dV Phi^H H Phi with
+// rocSOLVER ZHEEVD.
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -35,6 +36,7 @@ namespace {
 constexpr int GS_BT = 64;       // bands per reduction block
 constexpr int GS_LANES = 4;     // point lanes per band
 constexpr long GS_CHUNK = 512;  // grid points per partial sum
+constexpr int IP_BATCHES = 64;  // point chunks of the split-K inner products
 
 struct gs_state {
   cplx *hphi = nullptr, *res = nullptr, *hres = nullptr;  // [N][max_bands] each
@@ -43,6 +45,7 @@ struct gs_state {
   rocblas_handle blas = nullptr;
   cplx *S = nullptr;        // [max_bands]^2 device overlap / rotation
   double *eig = nullptr;    // [2 max_bands] eigenvalues + tridiagonal work
+  cplx *ip_part = nullptr;  // [IP_BATCHES + 1][max_bands]^2 split-K inner-product partials
   rocblas_int *info = nullptr;
   std::vector<std::complex<double>> hostS;
 };
@@ -94,6 +97,8 @@ void ensure_blas(handle &h) {
   if (!g.S) g.S = (cplx *)dalloc((size_t)h.max_bands * h.max_bands * sizeof(cplx));
   if (!g.eig) g.eig = (double *)dalloc((size_t)2 * h.max_bands * sizeof(double));
   if (!g.info) g.info = (rocblas_int *)dalloc(sizeof(rocblas_int));
+  if (!g.ip_part)
+    g.ip_part = (cplx *)dalloc((size_t)(IP_BATCHES + 1) * h.max_bands * h.max_bands * sizeof(cplx));
   GS_BLAS_CHECK(rocblas_set_stream(g.blas, h.stream));
 }
 
@@ -239,19 +244,42 @@ __global__ __launch_bounds__(256) void mm_partial(const cplx *__restrict__ phi, 
   });
 }
 
-__device__ double chunk_sum(const double *part, long nchunk, int nst, int b, int v) {
-  double s = 0.0;
-  for (long c = 0; c < nchunk; c++) s += part[(c * nst + b) * 8 + v];
-  return s;
+// Sums of the NV partials of band b over all point chunks, by one 256-thread
+// workgroup (strided over chunks, then a tree in LDS: a fixed order).
+// Thread 0 gets the totals in `out`.
+template <int NV>
+__device__ __forceinline__ void band_sums(const double *part, long nchunk, int nst, int b,
+                                          double *out) {
+  __shared__ double red[256][NV];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0.0;
+  for (long c = threadIdx.x; c < nchunk; c += 256)
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] += part[(c * nst + b) * 8 + v];
+#pragma unroll
+  for (int v = 0; v < NV; v++) red[threadIdx.x][v] = acc[v];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int v = 0; v < NV; v++) red[threadIdx.x][v] += red[threadIdx.x + w][v];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = red[0][v];
 }
 
 // band record: [0..1] eig, [2] norm, [3..4] eig/norm, [5] kinetic, [6] lambda
-__global__ void eig_norm_final(const double *part, long nchunk, int nst, double dV, double *band) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nst) return;
-  const double er = dV * chunk_sum(part, nchunk, nst, b, 0);
-  const double ei = dV * chunk_sum(part, nchunk, nst, b, 1);
-  const double nn = dV * chunk_sum(part, nchunk, nst, b, 2);
+// (one workgroup of 256 threads per band)
+__global__ __launch_bounds__(256) void eig_norm_final(const double *part, long nchunk, int nst,
+                                                      double dV, double *band) {
+  const int b = blockIdx.x;
+  double t[3];
+  band_sums<3>(part, nchunk, nst, b, t);
+  if (threadIdx.x != 0) return;
+  const double er = dV * t[0], ei = dV * t[1], nn = dV * t[2];
   band[b * 16 + 0] = er;
   band[b * 16 + 1] = ei;
   band[b * 16 + 2] = nn;
@@ -259,20 +287,25 @@ __global__ void eig_norm_final(const double *part, long nchunk, int nst, double 
   band[b * 16 + 4] = ei / nn;
 }
 
-__global__ void kinetic_final(const double *part, long nchunk, int nst, double *band) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nst) return;
+__global__ __launch_bounds__(256) void kinetic_final(const double *part, long nchunk, int nst,
+                                                     double *band) {
+  const int b = blockIdx.x;
+  double t[2];
+  band_sums<2>(part, nchunk, nst, b, t);
   // overlap_diagonal_normalized: the volume element cancels
-  band[b * 16 + 5] = chunk_sum(part, nchunk, nst, b, 0) / chunk_sum(part, nchunk, nst, b, 1);
+  if (threadIdx.x == 0) band[b * 16 + 5] = t[0] / t[1];
 }
 
 // steepest_descent.hpp:56-68
-__global__ void lambda_final(const double *part, long nchunk, int nst, double dV, double *band) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nst) return;
+__global__ __launch_bounds__(256) void lambda_final(const double *part, long nchunk, int nst,
+                                                    double dV, double *band) {
+  const int b = blockIdx.x;
+  double t[8];
+  band_sums<8>(part, nchunk, nst, b, t);
+  if (threadIdx.x != 0) return;
   // m0 = <r|r>, m1 = <phi|r>, m2 = <r|Hr>, m3 = <phi|Hr>, m4 = eig, m5 = norm
   double m[8];
-  for (int v = 0; v < 8; v++) m[v] = dV * chunk_sum(part, nchunk, nst, b, v);
+  for (int v = 0; v < 8; v++) m[v] = dV * t[v];
   const double m4r = band[b * 16 + 0], m4i = band[b * 16 + 1], m5r = band[b * 16 + 2];
   // real parts of the complex products
   auto re = [](double ar, double ai, double br, double bi) { return ar * br - ai * bi; };
@@ -342,12 +375,58 @@ dim3 red_grid(const handle &h, int nst) {
 
 void check_launch() { INQAMD_HIP_CHECK(hipGetLastError()); }
 
+// out[i] = sum over the partial slices of part (fixed order)
+__global__ __launch_bounds__(256) void slice_sum_kernel(const cplx *__restrict__ part, int nslice,
+                                                        long n, cplx *__restrict__ out) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    cplx s = part[i];
+    for (int k = 1; k < nslice; k++) s = cadd(s, part[(long)k * n + i]);
+    out[i] = s;
+  }
+}
+
+// Row-major M[r][c] = alpha sum_p conj(A[p][r]) B[p][c] over all grid points
+// (operations::overlap, overlap.hpp:26-60, as a BLAS gemm H(A) B).  The
+// point sum is long and the output small, so it is split over IP_BATCHES
+// point chunks (a strided-batched ZGEMM; one gemm over all points would keep
+// only (nst/64)^2 workgroups busy) and the chunk results are summed in order.
+void inner_products(handle &h, const cplx *A, long lda, const cplx *B, long ldb, int nst,
+                    double alpha, cplx *out) {
+  ensure_blas(h);
+  gs_state &g = state(h);
+  const long N = h.npoint;
+  const int nb = (int)(N >= 256L * IP_BATCHES ? IP_BATCHES : 1);
+  const long kc = N / nb, rem = N - kc * nb;
+  const rocblas_stride sc = (rocblas_stride)nst * nst;
+  const rocblas_double_complex a = {alpha, 0.0}, zero = {0.0, 0.0};
+  // column-major C_b(c, r) = sum_{p in chunk b} B[p][c] conj(A[p][r]) is the
+  // row-major M_b[r][c]
+  GS_BLAS_CHECK(rocblas_zgemm_strided_batched(
+      g.blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, nst, nst,
+      (rocblas_int)kc, &a, reinterpret_cast<const rocblas_double_complex *>(B), (int)ldb,
+      (rocblas_stride)(kc * ldb), reinterpret_cast<const rocblas_double_complex *>(A), (int)lda,
+      (rocblas_stride)(kc * lda), &zero, reinterpret_cast<rocblas_double_complex *>(g.ip_part),
+      nst, sc, nb));
+  int nslice = nb;
+  if (rem > 0) {
+    GS_BLAS_CHECK(rocblas_zgemm(
+        g.blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, nst, nst,
+        (rocblas_int)rem, &a, reinterpret_cast<const rocblas_double_complex *>(B + kc * nb * ldb),
+        (int)ldb, reinterpret_cast<const rocblas_double_complex *>(A + kc * nb * lda), (int)lda,
+        &zero, reinterpret_cast<rocblas_double_complex *>(g.ip_part + (long)nb * sc), nst));
+    nslice++;
+  }
+  hipLaunchKernelGGL(slice_sum_kernel, dim3(grid1(sc)), dim3(256), 0, h.stream,
+                     (const cplx *)g.ip_part, nslice, (long)sc, out);
+  check_launch();
+}
+
 // The K(1/2|G|^2/kinetic) scaling of operations::preconditioner for
 // fourier_space orbitals (preconditioner.hpp:39-55), given the residual-type
 // kinetic sums already in the band records.
 void precondition_apply(handle &h, cplx *phi, long ld, int nst) {
   gs_state &g = state(h);
-  hipLaunchKernelGGL(kinetic_final, dim3((nst + 63) / 64), dim3(64), 0, h.stream,
+  hipLaunchKernelGGL(kinetic_final, dim3(nst), dim3(256), 0, h.stream,
                      (const double *)g.part, nchunks(h), nst, g.band);
   check_launch();
   hipLaunchKernelGGL(precondition_kernel, dim3(grid1(h.npoint * nst)), dim3(256), 0, h.stream,
@@ -395,14 +474,8 @@ void preconditioner(handle &h, cplx *phi, long ld, int nst) {
 void orthogonalize(handle &h, cplx *phi, long ld, int nst, double dV) {
   ensure_blas(h);
   gs_state &g = state(h);
-  // column-major C = dV A A^H with A = Phi^T (nst x N, lda = ld) holds, read
-  // row-major, S[i][j] = dV sum_p conj(phi_pi) phi_pj; the upper fill gives
-  // its lower triangle
-  const double alpha = dV, beta = 0.0;
-  GS_BLAS_CHECK(rocblas_zherk(g.blas, rocblas_fill_upper, rocblas_operation_none, nst,
-                              (rocblas_int)h.npoint, &alpha,
-                              reinterpret_cast<const rocblas_double_complex *>(phi), (int)ld,
-                              &beta, reinterpret_cast<rocblas_double_complex *>(g.S), nst));
+  // S[i][j] = dV sum_p conj(phi_pi) phi_pj, row-major
+  inner_products(h, phi, ld, phi, ld, nst, dV, g.S);
   g.hostS.resize((size_t)nst * nst);
   INQAMD_HIP_CHECK(hipMemcpyAsync(g.hostS.data(), g.S, (size_t)nst * nst * sizeof(cplx),
                                   hipMemcpyDeviceToHost, h.stream));
@@ -459,7 +532,7 @@ void steepest_descent(handle &h, cplx *phi, long ld, int nst, bool precond) {
   const long nchunk = nchunks(h);
   // fourier_space::volume_element (fourier_space.hpp:41-44)
   const double dV = h.volume / ((double)h.npoint * (double)h.npoint);
-  const dim3 rg = red_grid(h, nst), fg((nst + 63) / 64);
+  const dim3 rg = red_grid(h, nst), fg(nst);
   const unsigned eg = grid1(h.npoint * nst);
   hamiltonian_fourier(h, phi, ld, g.hphi, rl, nst);
   const int num_steps = 5;
@@ -467,7 +540,7 @@ void steepest_descent(handle &h, cplx *phi, long ld, int nst, bool precond) {
     hipLaunchKernelGGL(eig_norm_partial, rg, dim3(256), 0, h.stream, (const cplx *)g.hphi, rl,
                        (const cplx *)phi, ld, nst, h.npoint, g.part);
     check_launch();
-    hipLaunchKernelGGL(eig_norm_final, fg, dim3(64), 0, h.stream, (const double *)g.part, nchunk,
+    hipLaunchKernelGGL(eig_norm_final, fg, dim3(256), 0, h.stream, (const double *)g.part, nchunk,
                        nst, dV, g.band);
     check_launch();
     hipLaunchKernelGGL(residual_partial, rg, dim3(256), 0, h.stream, geometry(h),
@@ -479,7 +552,7 @@ void steepest_descent(handle &h, cplx *phi, long ld, int nst, bool precond) {
     hipLaunchKernelGGL(mm_partial, rg, dim3(256), 0, h.stream, (const cplx *)phi, ld,
                        (const cplx *)g.res, (const cplx *)g.hres, rl, nst, h.npoint, g.part);
     check_launch();
-    hipLaunchKernelGGL(lambda_final, fg, dim3(64), 0, h.stream, (const double *)g.part, nchunk,
+    hipLaunchKernelGGL(lambda_final, fg, dim3(256), 0, h.stream, (const double *)g.part, nchunk,
                        nst, dV, g.band);
     check_launch();
     hipLaunchKernelGGL(sd_update_kernel, dim3(eg), dim3(256), 0, h.stream, phi, ld,
@@ -499,17 +572,10 @@ void subspace_diagonalization(handle &h, cplx *phi, long ld, int nst, double *ei
   ensure_blas(h);
   gs_state &g = state(h);
   hamiltonian_apply(h, phi, ld, g.hphi, nst, nst, KA_CURRENT);
-  const double dV = h.volume / (double)h.npoint;
-  // column-major C(c, r) = dV sum_p hphi[p][c] conj(phi[p][r]) is the
-  // row-major H_sub[r][c] = dV sum_p conj(phi[p][r]) hphi[p][c]; as a
-  // column-major matrix that is conj(H_sub)
-  const rocblas_double_complex a = {dV, 0.0}, zero = {0.0, 0.0}, one = {1.0, 0.0};
-  GS_BLAS_CHECK(rocblas_zgemm(g.blas, rocblas_operation_none,
-                              rocblas_operation_conjugate_transpose, nst, nst,
-                              (rocblas_int)h.npoint, &a,
-                              reinterpret_cast<const rocblas_double_complex *>(g.hphi), nst,
-                              reinterpret_cast<const rocblas_double_complex *>(phi), (int)ld,
-                              &zero, reinterpret_cast<rocblas_double_complex *>(g.S), nst));
+  // H_sub[r][c] = dV sum_p conj(phi[p][r]) hphi[p][c], row-major (dV = V/N);
+  // read as a column-major matrix that is conj(H_sub)
+  inner_products(h, phi, ld, g.hphi, nst, nst, h.volume / (double)h.npoint, g.S);
+  const rocblas_double_complex zero = {0.0, 0.0}, one = {1.0, 0.0};
   // eigenvectors w_k of conj(H_sub) are conj(v_k), v_k those of H_sub
   GS_BLAS_CHECK(rocsolver_zheevd(g.blas, rocblas_evect_original, rocblas_fill_upper, nst,
                                  reinterpret_cast<rocblas_double_complex *>(g.S), nst, g.eig,
@@ -536,7 +602,8 @@ void gs_release(handle &h) {
   if (!h.gs) return;
   gs_state *g = static_cast<gs_state *>(h.gs);
   for (void *p : {(void *)g->hphi, (void *)g->res, (void *)g->hres, (void *)g->part,
-                  (void *)g->band, (void *)g->S, (void *)g->eig, (void *)g->info})
+                  (void *)g->band, (void *)g->S, (void *)g->eig, (void *)g->info,
+                  (void *)g->ip_part})
     if (p) (void)hipFree(p);
   if (g->blas) (void)rocblas_destroy_handle(g->blas);
   delete g;
