@@ -644,6 +644,116 @@ static void steepest_descent(long N, int nst, int ld, double dV, Op ham, Prec pr
   orthogonalize(N, nst, ld, dV, phi);
 }
 
+/* ---------------------------------------------------------------------------
+ * Interacting TDDFT pieces: density, Hartree (periodic Poisson), LDA XC,
+ * the Kohn-Sham potential update and the self-consistent ETRS step.
+ * ------------------------------------------------------------------------- */
+
+/* observables::density::calculate_add (src/observables/density.hpp:23-31),
+ * one spin component: rho[p] += sum_b occ[b] |psi[p][b]|^2 */
+static void density_add(long N, int nst, int ld, const double *occ, const cplx *psi, double *rho) {
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) rho[p] += occ[b] * std::norm(psi[p * ld + b]);
+}
+
+/* solvers::poisson, 3-D periodic (src/solvers/poisson.hpp:29-35, 69-97,
+ * 224-227): to_fourier, V(G) *= (-4 pi / N) * (-1/|G|^2) (0 for |G|^2 < 1e-6),
+ * to_real without normalisation; the real part. */
+static void poisson3d(const int n[3], const cell &c, const double *rho, double *vh) {
+  const long N = (long)n[0] * n[1] * n[2];
+  std::vector<cplx> f((size_t)N), g((size_t)N);
+  for (long p = 0; p < N; p++) f[p] = rho[p];
+  fft3d(n, 1, 1, f.data(), g.data(), -1, false);
+  const double scal = (-4.0 * M_PI) / (double)N;
+#pragma omp parallel for schedule(static)
+  for (int ix = 0; ix < n[0]; ix++)
+    for (int iy = 0; iy < n[1]; iy++)
+      for (int iz = 0; iz < n[2]; iz++) {
+        vec3 gv{{2.0 * M_PI * symmetric(ix, n[0]), 2.0 * M_PI * symmetric(iy, n[1]),
+                 2.0 * M_PI * symmetric(iz, n[2])}};
+        const double g2 = c.norm_cov(gv);
+        const double ker = g2 < 1e-6 ? 0.0 : -1.0 / g2;
+        g[((long)ix * n[1] + iy) * n[2] + iz] *= scal * ker;
+      }
+  fft3d(n, 1, 1, g.data(), f.data(), +1, false);
+  for (long p = 0; p < N; p++) vh[p] = std::real(f[p]);
+}
+
+/* Unpolarised LDA exchange (Dirac/Slater) and Perdew-Zunger 1981
+ * correlation, the functionals libxc calls LDA_X and LDA_C_PZ that
+ * options::theory::lda() selects (src/options/theory.hpp:48-53).  Energy
+ * per particle e and potential v = d(n e)/dn; densities below 1e-15 give 0. */
+static void lda_point(double n, double &e, double &v) {
+  e = v = 0.0;
+  if (n < 1e-15) return;
+  const double cx = std::cbrt(3.0 / M_PI);
+  const double n13 = std::cbrt(n);
+  const double ex = -0.75 * cx * n13;
+  const double vx = -cx * n13;
+  const double rs = std::cbrt(3.0 / (4.0 * M_PI * n));
+  double ec, vc;
+  if (rs >= 1.0) {
+    const double gam = -0.1423, b1 = 1.0529, b2 = 0.3334;
+    const double sq = std::sqrt(rs);
+    const double den = 1.0 + b1 * sq + b2 * rs;
+    ec = gam / den;
+    vc = ec * (1.0 + (7.0 / 6.0) * b1 * sq + (4.0 / 3.0) * b2 * rs) / den;
+  } else {
+    const double A = 0.0311, B = -0.048, C = 0.0020, D = -0.0116;
+    const double lr = std::log(rs);
+    ec = A * lr + B + C * rs * lr + D * rs;
+    vc = A * lr + (B - A / 3.0) + (2.0 / 3.0) * C * rs * lr + (2.0 * D - C) / 3.0 * rs;
+  }
+  e = ex + ec;
+  v = vx + vc;
+}
+
+/* hamiltonian::xc_term::operator() (src/hamiltonian/xc_term.hpp:120-142) for
+ * one spin component and no core density: n = max(rho, 0); vxc[p] = v(n);
+ * exc = dV sum e(n) n; nvxc = dV sum rho vxc. */
+static void lda_xc(long N, double dV, const double *rho, double *vxc, double *exc, double *nvxc) {
+  double es = 0.0, ns = 0.0;
+  for (long p = 0; p < N; p++) {
+    const double n = rho[p] < 0.0 ? 0.0 : rho[p];
+    double e, v;
+    lda_point(n, e, v);
+    vxc[p] = v;
+    es += e * n;
+    ns += rho[p] * v;
+  }
+  *exc = dV * es;
+  *nvxc = dV * ns;
+}
+
+/* self_consistency::update_hamiltonian (src/hamiltonian/self_consistency.hpp:
+ * 88-162) for static ions, one spin, no electric field: V_KS = vion + V_H[rho]
+ * (hartree) + V_xc[rho] (xc).  energies: external, hartree, exc, nvxc. */
+static void ks_potential(const int n[3], const cell &c, const double *vion, const double *rho,
+                         int hartree, int xc, double *vks, double *energies) {
+  const long N = (long)n[0] * n[1] * n[2];
+  const double dV = c.volume / (double)N;
+  double ext = 0.0;
+  for (long p = 0; p < N; p++) ext += rho[p] * vion[p];
+  energies[0] = dV * ext;
+  for (long p = 0; p < N; p++) vks[p] = vion[p];
+  energies[1] = 0.0;
+  if (hartree) {
+    std::vector<double> vh((size_t)N);
+    poisson3d(n, c, rho, vh.data());
+    double eh = 0.0;
+    for (long p = 0; p < N; p++) eh += rho[p] * vh[p];
+    energies[1] = 0.5 * dV * eh;
+    for (long p = 0; p < N; p++) vks[p] += vh[p];
+  }
+  energies[2] = energies[3] = 0.0;
+  if (xc) {
+    std::vector<double> vx((size_t)N);
+    lda_xc(N, dV, rho, vx.data(), &energies[2], &energies[3]);
+    for (long p = 0; p < N; p++) vks[p] += vx[p];
+  }
+}
+
 extern "C" {
 
 void orc_set_threads(int nthreads) {
@@ -1026,6 +1136,86 @@ int orc_steepest_descent_dense(int npoint, int nvec, const double *matrix_, doub
   std::vector<cplx> mphi((size_t)npoint * nvec);
   ham(phi, mphi.data());
   overlap_diag(npoint, nvec, nvec, dV, phi, mphi.data(), ev);
+  return 0;
+}
+
+int orc_density(long npoint, int nst, int ld, const double *occ, const double *psi, double *rho) {
+  if (npoint <= 0 || nst <= 0 || ld < nst) return 1;
+  for (long p = 0; p < npoint; p++) rho[p] = 0.0;
+  density_add(npoint, nst, ld, occ, reinterpret_cast<const cplx *>(psi), rho);
+  return 0;
+}
+
+int orc_poisson(const orc_system *sys, const double *rho, double *vh) {
+  if (!sys) return 1;
+  cell c(sys->lattice);
+  poisson3d(sys->n, c, rho, vh);
+  return 0;
+}
+
+int orc_lda_xc(long npoint, double dV, const double *rho, double *vxc, double *exc, double *nvxc) {
+  if (npoint <= 0) return 1;
+  lda_xc(npoint, dV, rho, vxc, exc, nvxc);
+  return 0;
+}
+
+int orc_ks_potential(const orc_system *sys, const double *vion, const double *rho, int hartree,
+                     int xc, double *vks, double *energies) {
+  if (!sys) return 1;
+  cell c(sys->lattice);
+  ks_potential(sys->n, c, vion, rho, hartree, xc, vks, energies);
+  return 0;
+}
+
+/* real_time::etrs with the self-consistent second half (src/real_time/etrs.hpp:
+ * 22-82), static ions, no induced vector potential.  sys->vloc is V_KS of H(t);
+ * on return vks holds V_KS of H(t+dt) and *nscf the SCF passes run. */
+int orc_etrs_scf(const orc_system *sys, const double *vion, const double *occ, const double kA_t[3],
+                 const double kA_tdt[3], double dt, int order, int hartree, int xc, int nst, int ld,
+                 double *psi_, double *vks, int *nscf) {
+  if (!sys || nst <= 0 || ld < nst) return 1;
+  cplx *psi = reinterpret_cast<cplx *>(psi_);
+  cell c(sys->lattice);
+  const long N = npoints(sys);
+  const double dV = c.volume / (double)N;
+  const int nscf_max = 5;
+  const double scf_threshold = 5e-5;
+  orc_system s = *sys;
+  std::vector<double> V(sys->vloc, sys->vloc + N);
+  s.vloc = V.data();
+  /* :33-38 halfstep_phi = exponential_2_for_1(ham, i dt/2, i dt, phi): save
+   * holds the half step, psi the full step */
+  std::vector<cplx> save((size_t)N * ld);
+  orc_exponential_2_for_1(&s, kA_t, 0.0, dt / 2.0, 0.0, dt, order, nst, ld, psi_,
+                          reinterpret_cast<double *>(save.data()));
+  /* :44 density of the full step */
+  std::vector<double> rho((size_t)N, 0.0), old((size_t)N);
+  density_add(N, nst, ld, occ, psi, rho.data());
+  /* :53 update_hamiltonian(ham, ..., time + dt) */
+  double en[4];
+  ks_potential(sys->n, c, vion, rho.data(), hartree, xc, V.data(), en);
+  /* :56-58 restore the half step */
+  for (long p = 0; p < N; p++)
+    for (int b = 0; b < nst; b++) psi[p * ld + b] = save[p * ld + b];
+  /* :61-80 */
+  int iscf = 0;
+  for (; iscf < nscf_max; iscf++) {
+    if (iscf != 0)
+      for (long p = 0; p < N; p++)
+        for (int b = 0; b < nst; b++) psi[p * ld + b] = save[p * ld + b];
+    exponential_in_place(&s, kA_tdt, cplx(0.0, dt / 2.0), order, nst, ld, psi);
+    old = rho;
+    std::fill(rho.begin(), rho.end(), 0.0);
+    density_add(N, nst, ld, occ, psi, rho.data());
+    double delta = 0.0;
+    for (long p = 0; p < N; p++) delta += std::fabs(old[p] - rho[p]);
+    delta *= dV;
+    const bool done = delta < scf_threshold || iscf == nscf_max - 1;
+    ks_potential(sys->n, c, vion, rho.data(), hartree, xc, V.data(), en);
+    if (done) break;
+  }
+  for (long p = 0; p < N; p++) vks[p] = V[p];
+  *nscf = iscf + 1;
   return 0;
 }
 

@@ -44,6 +44,15 @@
  *                       problem (steepest_descent.hpp:120-166)
  *   orc_steepest_descent the dense pins above (same template) with the pinned
  *                       G-space Hamiltonian and preconditioner
+ *   orc_density         plane waves (rho = sum occ / V), normalisation
+ *   orc_poisson         reference point-charge known answers (poisson.hpp:
+ *                       251-314), plane-wave closed form 4 pi/G^2
+ *   orc_lda_xc          Dirac exchange closed form, Perdew-Zunger published
+ *                       values and continuity at rs = 1, v = d(n e)/dn by
+ *                       finite differences (libxc's own rounding: unpinned)
+ *   orc_ks_potential    sum of the pinned parts; E_H = 1/2 int rho V_H
+ *   orc_etrs_scf        frozen limit = orc_etrs_step; time reversal; energy
+ *                       conservation of the interacting step
  */
 #ifndef INQ_AMD_ORACLE_H
 #define INQ_AMD_ORACLE_H
@@ -175,6 +184,24 @@ int orc_steepest_descent(const orc_system *sys, const double kA_cov[3], int prec
  * preconditioner, then eigenvalues[b] = dV <phi_b|M phi_b> (complex) */
 int orc_steepest_descent_dense(int npoint, int nvec, const double *matrix, double dV, int niter,
                                double *phi, double *eigenvalues);
+
+/* observables::density::calculate_add (density.hpp:23-31), one spin:
+ * rho[p] = sum_b occ[b] |psi[p][b]|^2 */
+int orc_density(long npoint, int nst, int ld, const double *occ, const double *psi, double *rho);
+/* solvers::poisson, 3-D periodic (poisson.hpp:29-97, 224-227) of a real density */
+int orc_poisson(const orc_system *sys, const double *rho, double *vh);
+/* LDA_X + LDA_C_PZ (options::theory::lda) through xc_term::operator()
+ * (xc_term.hpp:120-142): vxc[N], exc = int e n, nvxc = int rho vxc */
+int orc_lda_xc(long npoint, double dV, const double *rho, double *vxc, double *exc, double *nvxc);
+/* self_consistency::update_hamiltonian (self_consistency.hpp:88-162):
+ * vks = vion + V_H (hartree) + V_xc (xc); energies = {external, hartree, exc, nvxc} */
+int orc_ks_potential(const orc_system *sys, const double *vion, const double *rho, int hartree,
+                     int xc, double *vks, double *energies);
+/* real_time::etrs with its SCF re-check (etrs.hpp:22-82); sys->vloc is V_KS
+ * of H(t), vks receives V_KS of H(t+dt), nscf the SCF passes */
+int orc_etrs_scf(const orc_system *sys, const double *vion, const double *occ, const double kA_t[3],
+                 const double kA_tdt[3], double dt, int order, int hartree, int xc, int nst, int ld,
+                 double *psi, double *vks, int *nscf);
 
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -93,6 +93,17 @@ def lib():
         L.orc_steepest_descent_dense.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_void_p]
+        L.orc_density.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_poisson.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_lda_xc.argtypes = [ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.orc_ks_potential.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_etrs_scf.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p, _c_dbl_p, _c_dbl_p,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_int)]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -289,6 +300,38 @@ class System:
         w, v = np.linalg.eigh(0.5 * (hs + hs.conj().T))
         return w, (m @ v).reshape(phi.shape)
 
+    def poisson(self, rho):
+        """solvers::poisson, 3-D periodic, of a real density [nx][ny][nz]."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        vh = np.zeros_like(rho)
+        _check(lib().orc_poisson(ctypes.byref(self.s), _ptr(rho), _ptr(vh)), "poisson")
+        return vh
+
+    def ks_potential(self, vion, rho, hartree=True, xc=True):
+        """self_consistency::update_hamiltonian: (vks, {external, hartree, exc, nvxc})."""
+        vion = np.ascontiguousarray(vion, np.float64)
+        rho = np.ascontiguousarray(rho, np.float64)
+        vks = np.zeros_like(rho)
+        en = np.zeros(4)
+        _check(lib().orc_ks_potential(ctypes.byref(self.s), _ptr(vion), _ptr(rho), int(hartree),
+                                      int(xc), _ptr(vks), _ptr(en)), "ks_potential")
+        return vks, dict(zip(("external", "hartree", "exc", "nvxc"), en))
+
+    def etrs_scf(self, psi, vion, occ, dt, kA_t=(0.0, 0.0, 0.0), kA_tdt=(0.0, 0.0, 0.0), order=4,
+                 hartree=True, xc=True):
+        """real_time::etrs with its SCF re-check; this System's vloc is V_KS(t).
+        Returns (psi(t+dt), V_KS(t+dt), scf passes)."""
+        out = np.array(psi, dtype=np.complex128, order="C", copy=True)
+        nst = out.shape[-1]
+        vion = np.ascontiguousarray(vion, np.float64)
+        occ = np.ascontiguousarray(np.broadcast_to(occ, (nst,)), np.float64)
+        vks = np.zeros(self.n)
+        nscf = ctypes.c_int(0)
+        _check(lib().orc_etrs_scf(ctypes.byref(self.s), _ptr(vion), _ptr(occ), _dvec(kA_t),
+                                  _dvec(kA_tdt), float(dt), int(order), int(hartree), int(xc), nst,
+                                  nst, _ptr(out), _ptr(vks), ctypes.byref(nscf)), "etrs_scf")
+        return out, vks, nscf.value
+
     def hamiltonian_fourier(self, phi, kA=(0.0, 0.0, 0.0)):
         """ks_hamiltonian::operator() on G-space orbitals (ks_hamiltonian.hpp:156-176)."""
         phi = np.ascontiguousarray(phi, np.complex128)
@@ -402,3 +445,24 @@ def steepest_descent_dense(matrix, phi, dV, niter):
     _check(lib().orc_steepest_descent_dense(npoint, nvec, _ptr(M), float(dV), int(niter),
                                             _ptr(out), _ptr(ev)), "steepest_descent_dense")
     return out, ev
+
+
+def density(psi, occ):
+    """observables::density::calculate_add for one spin: sum_b occ_b |psi_b|^2."""
+    psi = np.ascontiguousarray(psi, np.complex128)
+    nst = psi.shape[-1]
+    npoint = psi.size // nst
+    occ = np.ascontiguousarray(np.broadcast_to(occ, (nst,)), np.float64)
+    rho = np.zeros(psi.shape[:-1])
+    _check(lib().orc_density(npoint, nst, nst, _ptr(occ), _ptr(psi), _ptr(rho)), "density")
+    return rho
+
+
+def lda_xc(rho, dV):
+    """LDA_X + LDA_C_PZ through xc_term: (vxc, exc, nvxc)."""
+    rho = np.ascontiguousarray(rho, np.float64)
+    vxc = np.zeros_like(rho)
+    exc, nvxc = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    _check(lib().orc_lda_xc(rho.size, float(dV), _ptr(rho), _ptr(vxc), ctypes.byref(exc),
+                            ctypes.byref(nvxc)), "lda_xc")
+    return vxc, exc.value, nvxc.value
