@@ -69,8 +69,8 @@ typedef struct {
  * projector_all(projectors) (projector_all.hpp:86-94).  `stream` is a
  * hipStream_t (NULL = default stream).  Allocates every workspace the later
  * calls need, so they allocate nothing -- except the ground-state calls
- * (orthogonalize, preconditioner, steepest_descent, subspace_diagonalization),
- * which allocate their own
+ * (orthogonalize, preconditioner, steepest_descent, subspace_diagonalization)
+ * and the interacting TDDFT calls (density ... etrs_scf_step), which allocate their own
  * workspace on first use. */
 int inqamd_hamiltonian_create(const inqamd_system_desc *desc, void *stream,
                               inqamd_hamiltonian_t *out);
@@ -179,6 +179,38 @@ int inqamd_steepest_descent(inqamd_hamiltonian_t h, void *phi, int nst, int ld,
  * degenerate eigenspace) are the eigensolver's.  Synchronises. */
 int inqamd_subspace_diagonalization(inqamd_hamiltonian_t h, void *phi, int nst, int ld,
                                     double *eigenvalues);
+
+/* ---- interacting TDDFT (static ions, one spin, LDA) -------------------------------
+ * Fields are DEVICE arrays [N] of doubles on the handle's grid.
+ * observables::density::calculate_add (density.hpp:23-31):
+ * rho[p] = sum_b occ[b] |psi[p][b]|^2; occ is a HOST array [nst]. */
+int inqamd_density(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, const double *occ,
+                   double *rho);
+/* solvers::poisson, 3-D periodic (poisson.hpp:29-97, 224-227):
+ * V_H(G) = 4 pi rho(G) / |G|^2, zero mean. */
+int inqamd_poisson(inqamd_hamiltonian_t h, const double *rho, double *vh);
+/* The LDA of options::theory::lda (theory.hpp:48-53): Dirac exchange +
+ * Perdew-Zunger 1981 correlation (libxc's LDA_X, LDA_C_PZ, restated from the
+ * published forms) through xc_term::operator() (xc_term.hpp:120-142):
+ * vxc[N], exc = int e(n) n, nvxc = int rho vxc (host scalars).  Synchronises. */
+int inqamd_lda_xc(inqamd_hamiltonian_t h, const double *rho, double *vxc, double *exc,
+                  double *nvxc);
+/* self_consistency::update_hamiltonian (self_consistency.hpp:88-162) for
+ * static ions: the handle's scalar potential <- vion + V_H[rho] (hartree) +
+ * V_xc[rho] (xc); energies[4] (host) = {external, hartree, exc, nvxc}.
+ * Synchronises. */
+int inqamd_ks_potential(inqamd_hamiltonian_t h, const double *vion, const double *rho,
+                        int hartree, int xc, double energies[4]);
+/* real_time::etrs with its self-consistent second half (etrs.hpp:22-82):
+ * 2-for-1 Taylor step with H(t), density, H(t+dt) from it, then up to 5
+ * half steps from the saved half step until int |drho| < 5e-5.  The
+ * handle's scalar potential must be V_KS of H(t) on entry; it holds V_KS of
+ * H(t+dt) on return.  occ is a HOST array [nst]; *nscf receives the SCF
+ * passes.  Allocates an [N][max_bands] half-step array on first use.
+ * Synchronises. */
+int inqamd_etrs_scf_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov[3],
+                         const double A_tdt_cov[3], int order, void *psi, int nst, int ld,
+                         const double *vion, const double *occ, int hartree, int xc, int *nscf);
 
 /* ---- propagation --------------------------------------------------------------
  * operations::shift (shift.hpp:33-44): y += (re + i im) x. */

@@ -40,13 +40,33 @@ __device__ __forceinline__ void stage(const cplx *__restrict__ src, cplx *__rest
   }
 }
 
-// Any radix up to FFT_MAX_GENERIC_RADIX, O(R^2) per butterfly.
+// Any radix, O(R^2) per butterfly: up to FFT_MAX_GENERIC_RADIX through a
+// register array of the twiddled inputs; larger primes read each input from
+// LDS once per output (the same products in the same order).
 template <int B>
 __device__ __noinline__ void stage_generic(const cplx *__restrict__ src, cplx *__restrict__ dst,
                                            int len, int R, int Ns, const cplx *__restrict__ tw,
                                            int sign) {
   const int nb = len / R;
   const int tstride = len / (Ns * R);
+  if (R > FFT_MAX_GENERIC_RADIX) {
+    for (int w = threadIdx.x; w < nb * B; w += blockDim.x) {
+      const int j = w / B;
+      const int bb = w % B;
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+      for (int s = 0; s < R; s++) {
+        cplx acc = src[j * B + bb];
+        for (int q = 1; q < R; q++) {
+          cplx x = src[(j + q * nb) * B + bb];
+          if (k != 0) x = cmul(x, twiddle(tw, q * k * tstride, sign));
+          acc = cadd(acc, cmul(x, twiddle(tw, ((q * s) % R) * nb, sign)));
+        }
+        dst[(base + s * Ns) * B + bb] = acc;
+      }
+    }
+    return;
+  }
   for (int w = threadIdx.x; w < nb * B; w += blockDim.x) {
     const int j = w / B;
     const int bb = w % B;
@@ -176,10 +196,11 @@ bool fft_factor(int len, std::vector<int> &radices) {
   if (n % 2 == 0) { radices.push_back(2); n /= 2; }
   while (n % 3 == 0) { radices.push_back(3); n /= 3; }
   while (n % 5 == 0) { radices.push_back(5); n /= 5; }
-  for (int p = 7; n > 1 && p <= FFT_MAX_GENERIC_RADIX; p += 2) {
+  for (int p = 7; n > 1 && (p <= FFT_MAX_GENERIC_RADIX || p * p <= n); p += 2) {
     while (n % p == 0) { radices.push_back(p); n /= p; }
   }
-  return n == 1 && (int)radices.size() <= FFT_MAX_STAGES;
+  if (n > 1) radices.push_back(n);  // a prime above FFT_MAX_GENERIC_RADIX
+  return (int)radices.size() <= FFT_MAX_STAGES;
 }
 
 std::vector<cplx> fft_twiddles(int len) {

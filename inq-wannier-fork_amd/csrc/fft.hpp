@@ -12,7 +12,7 @@
 namespace inqamd {
 
 constexpr int FFT_MAX_STAGES = 12;
-constexpr int FFT_MAX_GENERIC_RADIX = 31;  // largest prime factor supported
+constexpr int FFT_MAX_GENERIC_RADIX = 31;  // largest radix with a register-resident butterfly
 constexpr int FFT_BLOCK = 256;
 
 enum fft_mode : int { FFT_PLAIN = 0, FFT_KINETIC = 1, FFT_EPILOGUE = 2 };
@@ -57,8 +57,9 @@ struct fft_pass_args {
   long psi_ld;
 };
 
-// Factor len into radices (8, 4, 2, 3, 5, then generic primes <= 31).
-// Returns false if a prime factor exceeds FFT_MAX_GENERIC_RADIX.
+// Factor len into radices (8, 4, 2, 3, 5, then the remaining primes; those
+// above FFT_MAX_GENERIC_RADIX run a direct O(R^2) stage from LDS).
+// Returns false if that takes more than FFT_MAX_STAGES stages.
 bool fft_factor(int len, std::vector<int> &radices);
 
 // Host twiddle table exp(-2 pi i k/len), evaluated in long double.

@@ -480,6 +480,7 @@ static void *dalloc(size_t bytes) {
 
 static void release(handle &h) {
   gs_release(h);
+  td_release(h);
   for (int a = 0; a < 3; a++)
     if (h.axis[a].tw) (void)hipFree(h.axis[a].tw), h.axis[a].tw = nullptr;
   for (void *p : {(void *)h.V, (void *)h.kA, (void *)h.work, (void *)h.hn, (void *)h.hn2,
@@ -539,7 +540,7 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
     std::vector<int> rad;
     if (!fft_factor(h.n[a], rad))
       throw error{ERR_UNSUPPORTED, "grid size " + std::to_string(h.n[a]) +
-                                       " has a prime factor above 31"};
+                                       " has too many prime factors"};
     if (2L * h.n[a] * fft_band_tile(1, h.n[a]) * (long)sizeof(cplx) > 65536)
       throw error{ERR_UNSUPPORTED, "grid size " + std::to_string(h.n[a]) + " is too long"};
     h.axis[a].len = h.n[a];

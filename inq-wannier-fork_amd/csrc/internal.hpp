@@ -109,6 +109,7 @@ struct inqamd_hamiltonian {
   inqamd::projector_tables pt;
 
   void *gs = nullptr;           // steepest-descent workspace and BLAS handle (gs.hip)
+  void *tddft = nullptr;        // density / potential fields and the ETRS half step (tddft.hip)
 
   std::string last_error;
 };
@@ -151,6 +152,8 @@ void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *ps
 // gs.hip
 void hamiltonian_fourier(handle &h, const cplx *phi, long ld, cplx *hphi, long hld, int nst);
 void gs_release(handle &h);
+// tddft.hip
+void td_release(handle &h);
 // td.hip: observables scratch of a handle (sizes from max_bands and tables)
 constexpr int OBS_GS_BLOCKS = 1024;  // blocks (and partial rows) of the G-space reductions
 constexpr int OBS_RECORD = 32;       // doubles per result record

@@ -101,6 +101,12 @@ SIGNATURES = [
     ("inqamd_orthogonalize", _i, [_vp, _vp, _i, _i, _i]),
     ("inqamd_steepest_descent", _i, [_vp, _vp, _i, _i, _i]),
     ("inqamd_subspace_diagonalization", _i, [_vp, _vp, _i, _i, _c_dbl_p]),
+    ("inqamd_density", _i, [_vp, _vp, _i, _i, _c_dbl_p, _vp]),
+    ("inqamd_poisson", _i, [_vp, _vp, _vp]),
+    ("inqamd_lda_xc", _i, [_vp, _vp, _vp, _c_dbl_p, _c_dbl_p]),
+    ("inqamd_ks_potential", _i, [_vp, _vp, _vp, _i, _i, _c_dbl_p]),
+    ("inqamd_etrs_scf_step", _i, [_vp, _d, _c_dbl_p, _c_dbl_p, _i, _vp, _i, _i, _vp, _c_dbl_p,
+                                  _i, _i, ctypes.POINTER(ctypes.c_int)]),
     ("inqamd_shift", _i, [_vp, _d, _d, _vp, _vp, _i, _i]),
     ("inqamd_exponential_in_place", _i, [_vp, _d, _d, _i, _vp, _i, _i]),
     ("inqamd_exponential_2_for_1", _i, [_vp, _d, _d, _d, _d, _i, _vp, _vp, _i, _i]),

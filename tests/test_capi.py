@@ -46,13 +46,13 @@ def test_create_rejects_bad_arguments_without_touching_the_gpu():
     assert rc == _lib.ERR_ARGUMENT
     assert b"max_bands" in L.inqamd_last_error(None)
     import numpy as np
-    v = np.zeros(37 * 64)
+    v = np.zeros(1031 * 64)
     d.vloc = v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     d.max_bands = 4
-    d.n = (ctypes.c_int * 3)(37, 8, 8)  # prime factor 37 > 31
+    d.n = (ctypes.c_int * 3)(1031, 8, 8)  # a prime axis longer than an LDS pencil holds
     rc = L.inqamd_hamiltonian_create(ctypes.byref(d), None, ctypes.byref(h))
     assert rc == _lib.ERR_UNSUPPORTED
-    assert b"prime factor" in L.inqamd_last_error(None)
+    assert b"too long" in L.inqamd_last_error(None)
 
 
 def test_null_handle_calls_fail_cleanly():

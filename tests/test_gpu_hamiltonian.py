@@ -40,7 +40,8 @@ def cfg0():
     return s, psi, O.System(s.n, s.lattice, s.vloc, s.tables)
 
 
-@pytest.mark.parametrize("n", [(32, 32, 32), (24, 30, 36), (17, 10, 12), (8, 9, 5)])
+@pytest.mark.parametrize("n", [(32, 32, 32), (24, 30, 36), (17, 10, 12), (8, 9, 5),
+                               (37, 6, 41)])
 @pytest.mark.parametrize("nst", [1, 5, 16, 37])
 def test_transform_matches_oracle(device, n, nst):
     say("transform", n, nst)

@@ -1,0 +1,140 @@
+"""GPU tests of the interacting TDDFT pieces against the oracle: density,
+Poisson (plus the reference's point-charge known answers, through the GPU's
+prime-length FFT), LDA exchange-correlation, the Kohn-Sham potential update
+and the self-consistent ETRS step on a pencil-FFT and a three-sweep grid.
+Tolerance 1e-12 relative unless stated."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import inq_amd
+from inq_amd import systems, tddft
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+HERE = os.path.dirname(os.path.abspath(__file__))
+KNOWN = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(a, device, dtype=np.complex128):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype)).to(device)
+
+
+@pytest.fixture(scope="module")
+def cfg(device):
+    s = systems.diamond_system(1)
+    psi = systems.random_orbitals(s, 8)
+    return s, psi, O.System(s.n, s.lattice, s.vloc, s.tables)
+
+
+def test_density_matches_oracle(device, cfg):
+    s, psi, _ = cfg
+    say("density")
+    occ = np.linspace(2.0, 0.5, 8)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=8)
+    rho = tddft.density(ham, to_dev(psi, device), occ).cpu().numpy()
+    assert relerr(rho, O.density(psi, occ)) < TOL
+    ham.close()
+
+
+def test_poisson_point_charge_reference_gpu(device):
+    k = KNOWN["poisson_point_charge"]
+    n = tuple(k["grid"])
+    say("poisson point charge", n)
+    s = systems.make_system(n, np.diag(k["cell"]), np.zeros((0, 3)), seed=0, system_id=91,
+                            with_projectors=False)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=1)
+    rho = torch.zeros(n, dtype=torch.float64, device=device)
+    rho[0, 0, 0] = k["charge"]
+    v = tddft.poisson(ham, rho).cpu().numpy()
+    say("sum |V| %.10f, V(0) %.10f" % (np.abs(v).sum(), v[0, 0, 0]))
+    assert np.abs(v).sum() == pytest.approx(k["sum_abs_potential"], abs=5e-10)
+    assert v[0, 0, 0] == pytest.approx(k["potential_at_origin"], abs=5e-11)
+    ham.close()
+
+
+def test_poisson_and_lda_match_oracle(device, cfg):
+    s, psi, osys = cfg
+    say("poisson + lda")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=8)
+    rho_h = O.density(psi, 2.0)
+    rho = to_dev(rho_h, device, np.float64)
+    assert relerr(tddft.poisson(ham, rho).cpu().numpy(), osys.poisson(rho_h)) < TOL
+    vxc, exc, nvxc = tddft.lda_xc(ham, rho)
+    dV = s.volume / np.prod(s.n)
+    rv, re, rn = O.lda_xc(rho_h, dV)
+    assert relerr(vxc.cpu().numpy(), rv) < TOL
+    assert exc == pytest.approx(re, rel=TOL) and nvxc == pytest.approx(rn, rel=TOL)
+    ham.close()
+
+
+def test_ks_potential_matches_oracle(device, cfg):
+    """The update sets the Hamiltonian's potential: Hpsi with it equals the
+    oracle's Hpsi with the oracle's V_KS."""
+    s, psi, osys = cfg
+    say("ks potential")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=8)
+    rho_h = O.density(psi, 2.0)
+    en = tddft.ks_potential(ham, to_dev(s.vloc, device, np.float64),
+                            to_dev(rho_h, device, np.float64))
+    vks, ren = osys.ks_potential(s.vloc, rho_h)
+    for key in ("external", "hartree", "exc", "nvxc"):
+        assert en[key] == pytest.approx(ren[key], rel=TOL), key
+    h = ham(to_dev(psi, device)).cpu().numpy()
+    ref = O.System(s.n, s.lattice, vks, s.tables).hamiltonian(psi)
+    assert relerr(h, ref) < TOL
+    ham.close()
+
+
+@pytest.mark.parametrize("grid", ["pencil", "cubic"])
+def test_etrs_scf_matches_oracle(device, grid):
+    if grid == "pencil":
+        lat = np.diag([6.5, 6.5, 6.5])
+        s = systems.make_system((12, 14, 12), lat, np.array([[0.3, 0.2, 0.1], [3.4, 3.0, 3.5]]),
+                                seed=3, system_id=55)
+        nst, nsteps = 4, 3
+    else:
+        s = systems.diamond_system(1)
+        nst, nsteps = 8, 2
+    say("etrs scf", grid, s.n)
+    psi = systems.random_orbitals(s, nst, ecut=6.0)
+    occ = np.full(nst, 2.0)
+    coef = systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=8)
+    pert = inq_amd.real_time.fourier_series_vector_potential(coef, 0.1)
+    dt = 0.02
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    d = to_dev(psi, device)
+    vion = to_dev(s.vloc, device, np.float64)
+    tddft.ks_potential(ham, vion, tddft.density(ham, d, occ))
+    v, _ = O.System(s.n, s.lattice, s.vloc, s.tables).ks_potential(s.vloc, O.density(psi, occ))
+    x = psi
+    lat = s.lattice
+    for it in range(nsteps):
+        t = it * dt
+        n_gpu = tddft.etrs(ham, d, vion, t, dt, occ, pert)
+        A0 = lat @ pert.uniform_vector_potential(t)
+        A1 = lat @ pert.uniform_vector_potential(t + dt)
+        x, v, n_ref = O.System(s.n, s.lattice, v, s.tables).etrs_scf(x, s.vloc, occ, dt, A0, A1)
+        say("step", it, "scf passes gpu", n_gpu, "oracle", n_ref)
+        assert n_gpu == n_ref
+    err = relerr(d.cpu().numpy(), x)
+    say("etrs scf rel err vs oracle %.2e" % err)
+    assert err < 1e-11
+    # the Hamiltonian now carries V_KS(t + dt): its Hpsi matches the oracle's
+    ham.uniform_vector_potential = lat @ pert.uniform_vector_potential(nsteps * dt)
+    h = ham(d).cpu().numpy()
+    ref = O.System(s.n, s.lattice, v, s.tables).hamiltonian(
+        x, lat @ pert.uniform_vector_potential(nsteps * dt))
+    assert relerr(h, ref) < 1e-11
+    ham.close()
