@@ -2,7 +2,7 @@
 one MI355X.  Prints ONE JSON line on stdout; diagnostics go to stderr.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--cpu-seconds 20]
-                    [--workload hpsi_64|uneven_batch|adjoint_48|etrs_96|adjoint_96|gs_64]
+                    [--workload hpsi_64|uneven_batch|adjoint_48|etrs_96|adjoint_96|gs_64|tddft_64]
 
 hpsi_64 (default): band-Hpsi/s at the 64^3 grid x 256 bands FP64 workload
   (2x2x2 diamond supercell, a = 20.52 bohr, 64 atoms x 4 projectors, Gamma);
@@ -15,6 +15,8 @@ etrs_96: the graph-captured ETRS step at 96^3 x 512 bands (216 atoms), and
   the adjoint of 100 steps with checkpoints every 10.
 gs_64: one frozen-potential ground-state iteration (subspace diagonalisation,
   preconditioned steepest descent, orthogonalize) at 64^3 x 256 bands.
+tddft_64: one interacting ETRS step (Hartree + LDA, self-consistent second
+  half) at 64^3 x 256 bands.
 """
 import argparse
 import json
@@ -389,11 +391,86 @@ def run_gs_64(args):
     return out
 
 
+def _tddft_bytes(N, nst, sum_k, nscf):
+    """Algorithmic bytes of one interacting ETRS step on nst bands: the
+    2-for-1 Taylor-4 step (four Hpsi, two accumulators read and written per
+    order), then per SCF pass a restore of the half step (32N), a Taylor-4 half
+    step (four Hpsi, one accumulator per order) and a density pass (16N),
+    plus the density of the full step (16N)."""
+    b_h = algorithmic_bytes_per_band(N, nst, sum_k)
+    per_band = (4 * b_h + 4 * 64.0 * N) + 16.0 * N + nscf * (32.0 * N + 4 * b_h + 4 * 32.0 * N + 16.0 * N)
+    return nst * per_band
+
+
+def run_tddft_64(args):
+    """One interacting ETRS step (Hartree + LDA, self-consistent second half)
+    at 64^3 x 256 bands, from the KS potential of the initial density."""
+    import numpy as np
+    import torch
+
+    import inq_amd
+    from inq_amd import real_time, tddft
+
+    s = inq_amd.systems.diamond_system(2)
+    nst = 256
+    psi = inq_amd.systems.random_orbitals(s, nst)
+    d_psi = torch.from_numpy(psi).to("cuda:0")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    vion = torch.from_numpy(s.vloc).to("cuda:0")
+    coef = inq_amd.systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=32)
+    pert = real_time.fourier_series_vector_potential(coef, 0.1)
+    dt = 0.015
+    tddft.ks_potential(ham, vion, tddft.density(ham, d_psi, 2.0))
+    state = {"t": 0.0, "nscf": []}
+
+    def step():
+        state["nscf"].append(tddft.etrs(ham, d_psi, vion, state["t"], dt, 2.0, pert))
+        state["t"] += dt
+
+    ms = timed(step, args.steps, args.warmup)
+    nscf = float(np.mean(state["nscf"][args.warmup:])) if len(state["nscf"]) > args.warmup else 1.0
+    log(f"interacting ETRS step {ms:.2f} ms, SCF passes per step {nscf:.2f}")
+    N = s.npoints
+    sum_k = int(np.sum(s.tables["sphere_size"]))
+    step_bytes = _tddft_bytes(N, nst, sum_k, nscf)
+    achieved = step_bytes / (ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        nb = 8
+        sample = np.ascontiguousarray(psi[..., :nb])
+        osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+        v0, _ = osys.ks_potential(s.vloc, O.density(sample, 2.0))
+        t0 = time.perf_counter()
+        _, _, n_ref = O.System(s.n, s.lattice, v0, s.tables).etrs_scf(sample, s.vloc, 2.0, dt)
+        dtc = time.perf_counter() - t0
+        cpu = {"value": nb / dtc, "unit": "band-steps/s", "cores": O.get_threads(), "kind": "port",
+               "sample": f"{nb} of {nst} bands of the same 64^3 system (their own density), one "
+                         f"interacting ETRS step ({n_ref} SCF passes), C++ oracle, OpenMP "
+                         f"{O.get_threads()} threads"}
+        log(f"cpu oracle: {nb} bands, one step in {dtc:.2f} s")
+    ham.close()
+    cfg = {"workload": "tddft_64: 64^3 grid, 256 bands complex128 (occupation 2), 2x2x2 diamond "
+                       "supercell a=20.52 bohr, 64 atoms x 4 projectors; interacting ETRS step "
+                       "(Taylor-4, Hartree + LDA, self-consistent second half) under a 32-term "
+                       "A(t), dt=0.015",
+           "grid": list(s.n), "bands": nst, "atoms": s.natoms, "parallelism": "single GPU",
+           "scf_passes_per_step": nscf}
+    out = common_fields(args, nst * 1e3 / ms, "band-steps/s", ms, True, cfg)
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "one interacting ETRS step (all launches)",
+                       "algorithmic_bytes_per_step": step_bytes}
+    out["cpu_baseline"] = cpu
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="hpsi_64",
                     choices=["hpsi_64", "uneven_batch", "adjoint_48", "etrs_96", "adjoint_96",
-                             "gs_64"])
+                             "gs_64", "tddft_64"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -425,6 +502,9 @@ def main():
                           "Taylor-4 ETRS, 100 steps of dt=0.03, checkpoints every 10; "
                           "loss J_x(T)", fd_coefs=1)
         print(json.dumps(out), flush=True)
+        return
+    if args.workload == "tddft_64":
+        print(json.dumps(run_tddft_64(args)))
         return
     if args.workload == "gs_64":
         print(json.dumps(run_gs_64(args)))

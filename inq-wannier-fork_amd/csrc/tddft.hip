@@ -172,11 +172,15 @@ __device__ __forceinline__ void block_sums4(long n, double *red, F term) {
     for (int v = 0; v < 4; v++) red[blockIdx.x * 4 + v] = sh[0][v];
 }
 
-__global__ void final_sums4(double *red, int nblocks) {
-  if (threadIdx.x >= 4) return;
+// Totals of the block partials: one wave per value, lanes strided over the
+// blocks, then a fixed butterfly.
+__global__ __launch_bounds__(256) void final_sums4(double *red, int nblocks) {
+  const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0.0;
-  for (int b = 0; b < nblocks; b++) s += red[b * 4 + threadIdx.x];
-  red[(long)TD_RED_BLOCKS * 4 + threadIdx.x] = s;
+  for (int b = lane; b < nblocks; b += 64) s += red[b * 4 + v];
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) s += __shfl_xor(s, w, 64);
+  if (lane == 0) red[(long)TD_RED_BLOCKS * 4 + v] = s;
 }
 
 // xc_term.hpp:120-142 for one spin and no core density: n = max(rho, 0),
@@ -232,7 +236,7 @@ void reduce4(handle &h, long n, double out[4], K kernel, A... args) {
   const unsigned nb = red_blocks(n);
   hipLaunchKernelGGL(kernel, dim3(nb), dim3(256), 0, h.stream, args..., t.red);
   check_launch();
-  hipLaunchKernelGGL(final_sums4, dim3(1), dim3(64), 0, h.stream, t.red, (int)nb);
+  hipLaunchKernelGGL(final_sums4, dim3(1), dim3(256), 0, h.stream, t.red, (int)nb);
   check_launch();
   INQAMD_HIP_CHECK(hipMemcpyAsync(out, t.red + (long)TD_RED_BLOCKS * 4, 4 * sizeof(double),
                                   hipMemcpyDeviceToHost, h.stream));
