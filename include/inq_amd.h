@@ -212,6 +212,18 @@ int inqamd_etrs_scf_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov
                          const double A_tdt_cov[3], int order, void *psi, int nst, int ld,
                          const double *vion, const double *occ, int hartree, int xc, int *nscf);
 
+/* ---- checkpoint files ------------------------------------------------------------
+ * operations::io::save / load of an orbital_set (io.hpp:163-197): directory
+ * `dirname` with one file per band, "%010d_%010d" (first_band + b, spinor),
+ * each the band's N complex doubles in grid order, raw native-endian binary
+ * (one process's basis part: the whole grid).  save creates the directory;
+ * load fails (non-zero status) if a file is missing or short, as INQ's
+ * load returns false.  Both synchronise. */
+int inqamd_save_states(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, long first_band,
+                       int spinor, const char *dirname);
+int inqamd_load_states(inqamd_hamiltonian_t h, void *psi, int nst, int ld, long first_band,
+                       int spinor, const char *dirname);
+
 /* ---- propagation --------------------------------------------------------------
  * operations::shift (shift.hpp:33-44): y += (re + i im) x. */
 int inqamd_shift(inqamd_hamiltonian_t h, double re, double im, const void *x, void *y, int nst,

@@ -7,8 +7,8 @@ runs the HIP library and fails loudly if it is absent.
 """
 from . import _lib, geometry, systems  # noqa: F401
 from .hamiltonian import ks_hamiltonian, projector_all  # noqa: F401
-from . import operations, real_time, observables, adjoint, batch, ground_state, tddft  # noqa: F401
+from . import operations, real_time, observables, adjoint, batch, ground_state, tddft, checkpoint  # noqa: F401
 from .operations import transform, space, fourier_plan  # noqa: F401
 
 __all__ = ["ks_hamiltonian", "projector_all", "operations", "real_time", "transform", "space",
-           "fourier_plan", "geometry", "systems", "observables", "adjoint", "batch", "ground_state", "tddft"]
+           "fourier_plan", "geometry", "systems", "observables", "adjoint", "batch", "ground_state", "tddft", "checkpoint"]

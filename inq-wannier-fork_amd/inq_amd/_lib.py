@@ -102,6 +102,8 @@ SIGNATURES = [
     ("inqamd_steepest_descent", _i, [_vp, _vp, _i, _i, _i]),
     ("inqamd_subspace_diagonalization", _i, [_vp, _vp, _i, _i, _c_dbl_p]),
     ("inqamd_density", _i, [_vp, _vp, _i, _i, _c_dbl_p, _vp]),
+    ("inqamd_save_states", _i, [_vp, _vp, _i, _i, ctypes.c_long, _i, ctypes.c_char_p]),
+    ("inqamd_load_states", _i, [_vp, _vp, _i, _i, ctypes.c_long, _i, ctypes.c_char_p]),
     ("inqamd_poisson", _i, [_vp, _vp, _vp]),
     ("inqamd_lda_xc", _i, [_vp, _vp, _vp, _c_dbl_p, _c_dbl_p]),
     ("inqamd_ks_potential", _i, [_vp, _vp, _vp, _i, _i, _c_dbl_p]),
