@@ -186,6 +186,9 @@ int inqamd_subspace_diagonalization(inqamd_hamiltonian_t h, void *phi, int nst, 
  * rho[p] = sum_b occ[b] |psi[p][b]|^2; occ is a HOST array [nst]. */
 int inqamd_density(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, const double *occ,
                    double *rho);
+/* observables::dipole of a density (dipole.hpp:24-40): dV sum_p r_cart rho,
+ * the positions in the symmetric range; host result.  Synchronises. */
+int inqamd_dipole(inqamd_hamiltonian_t h, const double *rho, double dipole_cart[3]);
 /* solvers::poisson, 3-D periodic (poisson.hpp:29-97, 224-227):
  * V_H(G) = 4 pi rho(G) / |G|^2, zero mean. */
 int inqamd_poisson(inqamd_hamiltonian_t h, const double *rho, double *vh);

@@ -224,6 +224,27 @@ __global__ __launch_bounds__(256) void absdiff_kernel(const double *__restrict__
   block_sums4(n, red, [&](long i, double *s) { s[0] += fabs(a[i] - b[i]); });
 }
 
+// observables::dipole of a density (dipole.hpp:24-40): sums of r_cart rho,
+// r from the symmetric-range contravariant coordinates (real_space.hpp:76-95)
+__global__ __launch_bounds__(256) void dipole_kernel(const double *__restrict__ rho, int n0, int n1,
+                                                     int n2, double a0, double a1, double a2,
+                                                     double a3, double a4, double a5, double a6,
+                                                     double a7, double a8, double *red) {
+  const long N = (long)n0 * n1 * n2;
+  block_sums4(N, red, [&](long p, double *s) {
+    const int iz = (int)(p % n2);
+    const long t = p / n2;
+    const int iy = (int)(t % n1), ix = (int)(t / n1);
+    const double c0 = (double)(ix >= (n0 + 1) / 2 ? ix - n0 : ix) / n0;
+    const double c1 = (double)(iy >= (n1 + 1) / 2 ? iy - n1 : iy) / n1;
+    const double c2 = (double)(iz >= (n2 + 1) / 2 ? iz - n2 : iz) / n2;
+    const double v = rho[p];
+    s[0] += ((c0 * a0 + c1 * a3) + c2 * a6) * v;
+    s[1] += ((c0 * a1 + c1 * a4) + c2 * a7) * v;
+    s[2] += ((c0 * a2 + c1 * a5) + c2 * a8) * v;
+  });
+}
+
 unsigned red_blocks(long n) {
   long b = (n + 255) / 256;
   return (unsigned)(b > TD_RED_BLOCKS ? TD_RED_BLOCKS : b);
@@ -266,6 +287,16 @@ void poisson(handle &h, const double *rho, double *vh) {
   hipLaunchKernelGGL(complex_real_part, dim3(grid1(h.npoint)), dim3(256), 0, h.stream,
                      (const cplx *)t.cf, h.npoint, vh);
   check_launch();
+}
+
+void dipole(handle &h, const double *rho, double out[3]) {
+  ensure_fields(h);
+  const double dV = h.volume / (double)h.npoint;
+  const double *a = h.lattice;
+  double s[4];
+  reduce4(h, h.npoint, s, dipole_kernel, rho, h.n[0], h.n[1], h.n[2], a[0], a[1], a[2], a[3], a[4],
+          a[5], a[6], a[7], a[8]);
+  for (int d = 0; d < 3; d++) out[d] = dV * s[d];
 }
 
 void lda_xc(handle &h, const double *rho, double *vxc, double *exc, double *nvxc) {
@@ -378,6 +409,14 @@ int inqamd_poisson(inqamd_hamiltonian_t h, const double *rho, double *vh) {
   return guarded(h, [&] {
     INQAMD_REQUIRE(rho && vh, "rho and vh are required");
     poisson(*h, rho, vh);
+  });
+}
+
+int inqamd_dipole(inqamd_hamiltonian_t h, const double *rho, double dipole_cart[3]) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    INQAMD_REQUIRE(rho && dipole_cart, "rho and dipole are required");
+    dipole(*h, rho, dipole_cart);
   });
 }
 

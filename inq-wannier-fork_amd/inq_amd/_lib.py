@@ -105,6 +105,7 @@ SIGNATURES = [
     ("inqamd_save_states", _i, [_vp, _vp, _i, _i, ctypes.c_long, _i, ctypes.c_char_p]),
     ("inqamd_load_states", _i, [_vp, _vp, _i, _i, ctypes.c_long, _i, ctypes.c_char_p]),
     ("inqamd_poisson", _i, [_vp, _vp, _vp]),
+    ("inqamd_dipole", _i, [_vp, _vp, _c_dbl_p]),
     ("inqamd_lda_xc", _i, [_vp, _vp, _vp, _c_dbl_p, _c_dbl_p]),
     ("inqamd_ks_potential", _i, [_vp, _vp, _vp, _i, _i, _c_dbl_p]),
     ("inqamd_etrs_scf_step", _i, [_vp, _d, _c_dbl_p, _c_dbl_p, _i, _vp, _i, _i, _vp, _c_dbl_p,

@@ -39,6 +39,13 @@ def density(ham, psi, occ=2.0):
     return rho
 
 
+def dipole(ham, rho):
+    """observables::dipole of a density (cartesian, the electronic part)."""
+    out = np.zeros(3)
+    ham.handle.call("inqamd_dipole", _field(rho, ham), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
 def poisson(ham, rho):
     """The periodic Hartree potential of rho (zero mean)."""
     vh = torch.empty_like(rho)

@@ -726,6 +726,25 @@ static void lda_xc(long N, double dV, const double *rho, double *vxc, double *ex
   *nvxc = dV * ns;
 }
 
+/* observables::dipole of a density (src/observables/dipole.hpp:24-40):
+ * dV sum_p r_cart(p) rho[p], r from the symmetric-range contravariant point
+ * coordinates (real_space.hpp:76-95). */
+static void dipole(const int n[3], const cell &c, const double *rho, double out[3]) {
+  const long N = (long)n[0] * n[1] * n[2];
+  const double dV = c.volume / (double)N;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int ix = 0; ix < n[0]; ix++)
+    for (int iy = 0; iy < n[1]; iy++)
+      for (int iz = 0; iz < n[2]; iz++) {
+        vec3 rc{{(double)symmetric(ix, n[0]) / n[0], (double)symmetric(iy, n[1]) / n[1],
+                 (double)symmetric(iz, n[2]) / n[2]}};
+        vec3 r = c.contra_to_cart(rc);
+        const double v = rho[((long)ix * n[1] + iy) * n[2] + iz];
+        for (int d = 0; d < 3; d++) s[d] += r[d] * v;
+      }
+  for (int d = 0; d < 3; d++) out[d] = dV * s[d];
+}
+
 /* self_consistency::update_hamiltonian (src/hamiltonian/self_consistency.hpp:
  * 88-162) for static ions, one spin, no electric field: V_KS = vion + V_H[rho]
  * (hartree) + V_xc[rho] (xc).  energies: external, hartree, exc, nvxc. */
@@ -1156,6 +1175,13 @@ int orc_poisson(const orc_system *sys, const double *rho, double *vh) {
 int orc_lda_xc(long npoint, double dV, const double *rho, double *vxc, double *exc, double *nvxc) {
   if (npoint <= 0) return 1;
   lda_xc(npoint, dV, rho, vxc, exc, nvxc);
+  return 0;
+}
+
+int orc_dipole(const orc_system *sys, const double *rho, double *out) {
+  if (!sys) return 1;
+  cell c(sys->lattice);
+  dipole(sys->n, c, rho, out);
   return 0;
 }
 

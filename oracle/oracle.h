@@ -50,6 +50,7 @@
  *   orc_lda_xc          Dirac exchange closed form, Perdew-Zunger published
  *                       values and continuity at rs = 1, v = d(n e)/dn by
  *                       finite differences (libxc's own rounding: unpinned)
+ *   orc_dipole          reference known answers of dipole.hpp:60-115
  *   orc_ks_potential    sum of the pinned parts; E_H = 1/2 int rho V_H
  *   orc_etrs_scf        frozen limit = orc_etrs_step; time reversal; energy
  *                       conservation of the interacting step
@@ -193,6 +194,8 @@ int orc_poisson(const orc_system *sys, const double *rho, double *vh);
 /* LDA_X + LDA_C_PZ (options::theory::lda) through xc_term::operator()
  * (xc_term.hpp:120-142): vxc[N], exc = int e n, nvxc = int rho vxc */
 int orc_lda_xc(long npoint, double dV, const double *rho, double *vxc, double *exc, double *nvxc);
+/* observables::dipole of a density (dipole.hpp:24-40): dV sum r_cart rho */
+int orc_dipole(const orc_system *sys, const double *rho, double *out);
 /* self_consistency::update_hamiltonian (self_consistency.hpp:88-162):
  * vks = vion + V_H (hartree) + V_xc (xc); energies = {external, hartree, exc, nvxc} */
 int orc_ks_potential(const orc_system *sys, const double *vion, const double *rho, int hartree,

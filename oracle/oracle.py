@@ -98,6 +98,7 @@ def lib():
         L.orc_poisson.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_lda_xc.argtypes = [ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.orc_dipole.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_ks_potential.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_etrs_scf.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p, _c_dbl_p, _c_dbl_p,
@@ -306,6 +307,13 @@ class System:
         vh = np.zeros_like(rho)
         _check(lib().orc_poisson(ctypes.byref(self.s), _ptr(rho), _ptr(vh)), "poisson")
         return vh
+
+    def dipole(self, rho):
+        """observables::dipole of a density: dV sum r_cart rho."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        out = np.zeros(3)
+        _check(lib().orc_dipole(ctypes.byref(self.s), _ptr(rho), _ptr(out)), "dipole")
+        return out
 
     def ks_potential(self, vion, rho, hartree=True, xc=True):
         """self_consistency::update_hamiltonian: (vks, {external, hartree, exc, nvxc})."""

@@ -138,3 +138,24 @@ def test_etrs_scf_matches_oracle(device, grid):
         x, lat @ pert.uniform_vector_potential(nsteps * dt))
     assert relerr(h, ref) < 1e-11
     ham.close()
+
+
+def test_dipole_reference_known_answers_gpu(device):
+    k = KNOWN["dipole"]
+    lat = np.diag(k["cell"])
+    n = tuple(int(round(L / k["spacing"])) for L in k["cell"])
+    say("dipole", n)
+    s = systems.make_system(n, lat, np.zeros((0, 3)), seed=0, system_id=92, with_projectors=False)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=1)
+    from helpers import grid_positions
+    r = grid_positions(lat, n)
+    e = np.exp(-(r * r).sum(-1))
+    x, y, z = r[..., 0], r[..., 1], r[..., 2]
+    d = tddft.dipole(ham, to_dev(x * e, device, np.float64))
+    assert d[0] == pytest.approx(k["x"][0], rel=1e-9) and np.abs(d[1:]).max() < 1e-14
+    d = tddft.dipole(ham, to_dev((np.sin(y) + x * z) * e, device, np.float64))
+    assert d[0] == pytest.approx(k["xz"][0], abs=1e-10) and d[1] == pytest.approx(k["xz"][1], rel=1e-9)
+    rho = to_dev((y + 2.0 * z) * e, device, np.float64)
+    assert np.allclose(tddft.dipole(ham, rho), O.System(n, lat, np.zeros(n)).dipole(
+        (y + 2.0 * z) * e), rtol=1e-13, atol=1e-14)
+    ham.close()

@@ -167,3 +167,22 @@ def test_etrs_scf_conserves_energy_and_reverses():
     for _ in range(5):
         y, v, _ = O.System(s.n, s.lattice, v, s.tables).etrs_scf(y, s.vloc, occ, -dt)
     assert relerr(y, psi) < 1e-5
+
+
+def test_dipole_reference_known_answers():
+    k = KNOWN["dipole"]
+    lat = np.diag(k["cell"])
+    n = tuple(int(round(L / k["spacing"])) for L in k["cell"])  # real_space.hpp:183-194
+    assert n == (11, 9, 16)
+    s = O.System(n, lat, np.zeros(n))
+    r = grid_positions(lat, n)
+    e = np.exp(-(r * r).sum(-1))
+    x, y, z = r[..., 0], r[..., 1], r[..., 2]
+    d = s.dipole(x * e)
+    assert d[0] == pytest.approx(k["x"][0], rel=1e-9) and np.abs(d[1:]).max() < 1e-14
+    d = s.dipole((y + 2.0 * z) * e)
+    assert abs(d[0]) < 1e-14
+    assert d[1] == pytest.approx(k["yz"][1], rel=1e-6) and d[2] == pytest.approx(k["yz"][2], rel=1e-6)
+    d = s.dipole((np.sin(y) + x * z) * e)
+    assert d[0] == pytest.approx(k["xz"][0], abs=1e-10)
+    assert d[1] == pytest.approx(k["xz"][1], rel=1e-9) and abs(d[2]) < 1e-14
