@@ -73,9 +73,54 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
 // ---------------------------------------------------------------------------
 // sweep 2: 2-D x-y transform of one plane with the kinetic factor
 // ---------------------------------------------------------------------------
-template <int R0, int R1, int R2, int NT = plane_threads(R0 * R1 * R2)>
-__device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cplx *lds) {
-  using P = plan3<R0, R1, R2>;
+// Radices of the x-y plane transforms: the grid's plan (R0, R1, R2), except
+// at 96 where the plane runs two stages (8 x 12: fewer LDS passes and
+// barriers per plane) and the z tiles keep three (their radix-12 stage costs
+// more in the transposing epilogue than it saves).
+constexpr int xy_radix(int i, int R0, int R1, int R2) {
+  const int L = R0 * R1 * R2;
+  const int r[3] = {R0, R1, R2};
+  const int r96[3] = {8, 12, 1};
+  return L == 96 ? r96[i] : r[i];
+}
+
+template <int R0, int R1, int R2>
+struct xy_plan {
+  static constexpr int X0 = xy_radix(0, R0, R1, R2), X1 = xy_radix(1, R0, R1, R2),
+                       X2 = xy_radix(2, R0, R1, R2);
+  using P = plan3<X0, X1, X2>;
+  static_assert(P::LEN == R0 * R1 * R2, "x-y radices must multiply to the grid length");
+};
+
+// Registers of the plane's first stage (the x-forward inputs)
+template <int R0, int R1, int R2, int NT>
+using xy_regs = stage_regs<xy_plan<R0, R1, R2>::X0, R0 * R1 * R2, R0 * R1 * R2, NT>;
+
+template <int R0, int R1, int R2, int NT>
+__device__ __forceinline__ void xy_load(const fft3_args &a, long plane, xy_regs<R0, R1, R2, NT> &in) {
+  constexpr int L = R0 * R1 * R2;
+  const cplx *W = a.W + plane * L * L;
+  pload<xy_plan<R0, R1, R2>::X0, L, L, NT>(in, [&](int i, int c) -> cplx { return W[i * L + c]; });
+}
+
+// Cartesian k + A (the covariant vector through the reciprocal lattice / 2 pi)
+__device__ __forceinline__ void xy_kc(const fft3_args &a, double kc[3]) {
+  const double inv2pi = 0.15915494309189533576888376337251;
+  const double k0 = a.kA[0], k1 = a.kA[1], k2 = a.kA[2];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    kc[d] = (a.rlat[0 * 3 + d] * k0 + a.rlat[1 * 3 + d] * k1 + a.rlat[2 * 3 + d] * k2) * inv2pi;
+}
+
+// The plane transform from its first-stage inputs already in registers
+template <int R0, int R1, int R2, int NT>
+__device__ __forceinline__ void xy_kin_plane_from(const fft3_args &a, long plane, cplx *lds,
+                                                  xy_regs<R0, R1, R2, NT> &in,
+                                                  const cplx *__restrict__ tw,
+                                                  const double kc[3]) {
+  using XP = xy_plan<R0, R1, R2>;
+  using P = typename XP::P;
+  constexpr int X0 = XP::X0, X1 = XP::X1, X2 = XP::X2;
   constexpr int L = P::LEN;
   constexpr int PAD = L + 1;  // row stride of the LDS plane (breaks bank aliasing)
   // lds: [x][PAD]; plane = band * L + z
@@ -83,7 +128,6 @@ __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cpl
   cplx *W = a.W + plane * L * L;
 
   // x-axis accessors: element (i = x, c = y)
-  auto gx = [&](int i, int c) -> cplx { return W[i * L + c]; };
   auto lx = [&](int i, int c) -> cplx { return lds[i * PAD + c]; };
   auto sx = [&](int i, int c, cplx v, int, int) { lds[i * PAD + c] = v; };
   auto wx = [&](int i, int c, cplx v, int, int) { W[i * L + c] = v; };
@@ -92,15 +136,11 @@ __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cpl
   auto sy = [&](int i, int c, cplx v, int, int) { lds[c * PAD + i] = v; };
 
   // 1/2 |G + k + A|^2 with G = m_x b0 + m_y b1 + m_z b2 (cartesian)
-  const double inv2pi = 0.15915494309189533576888376337251;
   const int mz = symmetric_index(z, L);
   double gz[3], b0[3], b1[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const double kc =
-        (a.rlat[0 * 3 + d] * a.kA[0] + a.rlat[1 * 3 + d] * a.kA[1] + a.rlat[2 * 3 + d] * a.kA[2]) *
-        inv2pi;
-    gz[d] = mz * a.rlat[2 * 3 + d] + kc;
+    gz[d] = mz * a.rlat[2 * 3 + d] + kc[d];
     b0[d] = a.rlat[0 * 3 + d];
     b1[d] = a.rlat[1 * 3 + d];
   }
@@ -114,10 +154,19 @@ __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cpl
     lds[c * PAD + i] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), v);
   };
 
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, -1, false, gx, lx, sx, sx, true);  // x forward
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, -1, true, ly, ly, sy, ky, true);   // y forward, x T
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, +1, true, ly, ly, sy, sy, true);   // y backward
-  run_plan<P, R0, R1, R2, L, NT>(a.tw, +1, true, lx, lx, sx, wx, false);  // x backward
+  run_plan_from<P, X0, X1, X2, L, NT>(in, tw, -1, false, lx, sx, sx, true);  // x forward
+  run_plan<P, X0, X1, X2, L, NT>(tw, -1, true, ly, ly, sy, ky, true);   // y forward, x T
+  run_plan<P, X0, X1, X2, L, NT>(tw, +1, true, ly, ly, sy, sy, true);   // y backward
+  run_plan<P, X0, X1, X2, L, NT>(tw, +1, true, lx, lx, sx, wx, false);  // x backward
+}
+
+template <int R0, int R1, int R2, int NT = plane_threads(R0 * R1 * R2)>
+__device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cplx *lds) {
+  xy_regs<R0, R1, R2, NT> in;
+  xy_load<R0, R1, R2, NT>(a, plane, in);
+  double kc[3];
+  xy_kc(a, kc);
+  xy_kin_plane_from<R0, R1, R2, NT>(a, plane, lds, in, a.tw, kc);
 }
 
 // ---------------------------------------------------------------------------
@@ -270,14 +319,14 @@ constexpr int z_bwd_min_waves(int L) { return L == 72 ? 1 : 4; }
 // Grid lengths with three-sweep kernels: (L, radices of the plan)
 #define INQAMD_FFT3_TABLE(X) \
   X(24, 8, 3, 1)             \
-  X(30, 2, 3, 5)             \
+  X(30, 6, 5, 1)             \
   X(32, 8, 4, 1)             \
-  X(36, 4, 3, 3)             \
+  X(36, 6, 6, 1)             \
   X(40, 8, 5, 1)             \
-  X(48, 8, 2, 3)             \
-  X(60, 4, 3, 5)             \
+  X(48, 8, 6, 1)             \
+  X(60, 12, 5, 1)            \
   X(64, 8, 8, 1)             \
-  X(72, 8, 3, 3)             \
+  X(72, 8, 9, 1)             \
   X(80, 8, 2, 5)             \
   X(96, 8, 4, 3)
 

@@ -85,6 +85,81 @@ __device__ __forceinline__ void dft<8>(cplx *v, int sign) {
   }
 }
 
+// cos and sin of 2 pi m / N for the composite butterflies (N = 6, 9, 12),
+// the correctly rounded doubles
+__host__ __device__ constexpr double unit_cos(int N, int m) {
+  constexpr double c30 = 0.86602540378443860;  // cos(pi/6)
+  constexpr double c12[12] = {1.0, c30, 0.5, 0.0, -0.5, -c30, -1.0, -c30, -0.5, 0.0, 0.5, c30};
+  constexpr double c9[9] = {1.0, 0.76604444311897801, 0.17364817766693036, -0.5,
+                            -0.93969262078590843, -0.93969262078590843, -0.5,
+                            0.17364817766693036, 0.76604444311897801};
+  return N == 9 ? c9[m % 9] : c12[(m * (12 / N)) % 12];
+}
+
+__host__ __device__ constexpr double unit_sin(int N, int m) {
+  constexpr double c30 = 0.86602540378443860;  // sin(pi/3)
+  constexpr double s12[12] = {0.0, 0.5, c30, 1.0, c30, 0.5, 0.0, -0.5, -c30, -1.0, -c30, -0.5};
+  constexpr double s9[9] = {0.0, 0.64278760968653936, 0.98480775301220802, c30,
+                            0.34202014332566871, -0.34202014332566871, -c30,
+                            -0.98480775301220802, -0.64278760968653936};
+  return N == 9 ? s9[m % 9] : s12[(m * (12 / N)) % 12];
+}
+
+// Radix P*Q butterfly on registers by one Cooley-Tukey split: inputs
+// n = Q n1 + n2, radix-P transforms over n1, twiddles W_N^(n2 k1), radix-Q
+// transforms over n2, outputs k = k1 + P k2.
+template <int P, int Q>
+__device__ __forceinline__ void dft_split(cplx *v, int sign) {
+  constexpr int N = P * Q;
+  cplx a[Q][P];
+#pragma unroll
+  for (int n2 = 0; n2 < Q; n2++) {
+#pragma unroll
+    for (int n1 = 0; n1 < P; n1++) a[n2][n1] = v[Q * n1 + n2];
+    dft<P>(a[n2], sign);
+  }
+#pragma unroll
+  for (int n2 = 1; n2 < Q; n2++) {
+#pragma unroll
+    for (int k1 = 1; k1 < P; k1++) {
+      const int m = (n2 * k1) % N;
+      if (4 * m == N) {
+        a[n2][k1] = mul_i(a[n2][k1], sign);
+      } else if (2 * m == N) {
+        a[n2][k1] = mk(-a[n2][k1].x, -a[n2][k1].y);
+      } else if (4 * m == 3 * N) {
+        a[n2][k1] = mul_i(a[n2][k1], -sign);
+      } else {
+        a[n2][k1] = cmul(a[n2][k1], mk(unit_cos(N, m), sign * unit_sin(N, m)));
+      }
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < P; k1++) {
+    cplx t[Q];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; n2++) t[n2] = a[n2][k1];
+    dft<Q>(t, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; k2++) v[k1 + P * k2] = t[k2];
+  }
+}
+
+template <>
+__device__ __forceinline__ void dft<6>(cplx *v, int sign) {
+  dft_split<3, 2>(v, sign);
+}
+
+template <>
+__device__ __forceinline__ void dft<9>(cplx *v, int sign) {
+  dft_split<3, 3>(v, sign);
+}
+
+template <>
+__device__ __forceinline__ void dft<12>(cplx *v, int sign) {
+  dft_split<4, 3>(v, sign);
+}
+
 __device__ __forceinline__ cplx twiddle(const cplx *__restrict__ tw, int idx, int sign) {
   cplx t = tw[idx];
   if (sign > 0) t.y = -t.y;

@@ -23,26 +23,49 @@ constexpr int plan_threads(int len, int B) {
   return t < 64 ? 64 : (t > 256 ? 256 : t);
 }
 
-// One stage with compile-time radix R and span NS over a LEN x B tile.
-template <int R, int NS, int LEN, int B, int NT, class LD, class ST>
-__device__ __forceinline__ void pstage(const cplx *__restrict__ tw, int sign, LD ld, ST st,
-                                       bool sync_mid) {
-  constexpr int NB = LEN / R;
-  constexpr int NBF = (NB * B + NT - 1) / NT;
-  constexpr int TSTRIDE = LEN / (NS * R);
+// Inputs of one stage held in registers: the R inputs of each of the
+// thread's NBF butterflies.
+template <int R, int LEN, int B, int NT>
+struct stage_regs {
+  static constexpr int NBF = (LEN / R * B + NT - 1) / NT;
   cplx v[NBF][R];
+};
+
+// Loads a stage's inputs (butterfly j = w / B of column w % B takes inputs
+// j + r*LEN/R) through `ld(i, c)`.
+template <int R, int LEN, int B, int NT, class LD>
+__device__ __forceinline__ void pload(stage_regs<R, LEN, B, NT> &in, LD ld) {
+  constexpr int NB = LEN / R;
+  constexpr int NBF = stage_regs<R, LEN, B, NT>::NBF;
 #pragma unroll
   for (int q = 0; q < NBF; q++) {
     const int w = threadIdx.x + q * NT;
     if ((int)threadIdx.x < NT && (NBF * NT == NB * B || w < NB * B)) {
-      const int j = w / B, bb = w % B, k = j % NS;
+      const int j = w / B, bb = w % B;
 #pragma unroll
-      for (int r = 0; r < R; r++) v[q][r] = ld(j + r * NB, bb);
+      for (int r = 0; r < R; r++) in.v[q][r] = ld(j + r * NB, bb);
+    }
+  }
+}
+
+// Twiddles, butterflies and stores of one stage with compile-time radix R
+// and span NS over a LEN x B tile, from inputs already in registers.
+template <int R, int NS, int LEN, int B, int NT, class ST>
+__device__ __forceinline__ void pfinish(stage_regs<R, LEN, B, NT> &in, const cplx *__restrict__ tw,
+                                        int sign, ST st, bool sync_mid) {
+  constexpr int NB = LEN / R;
+  constexpr int NBF = stage_regs<R, LEN, B, NT>::NBF;
+  constexpr int TSTRIDE = LEN / (NS * R);
+#pragma unroll
+  for (int q = 0; q < NBF; q++) {
+    const int w = threadIdx.x + q * NT;
+    if ((int)threadIdx.x < NT && (NBF * NT == NB * B || w < NB * B)) {
+      const int j = w / B, k = j % NS;
       if (NS > 1 && k != 0) {
 #pragma unroll
-        for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], twiddle(tw, r * k * TSTRIDE, sign));
+        for (int r = 1; r < R; r++) in.v[q][r] = cmul(in.v[q][r], twiddle(tw, r * k * TSTRIDE, sign));
       }
-      dft<R>(v[q], sign);
+      dft<R>(in.v[q], sign);
     }
   }
   if (sync_mid) __syncthreads();
@@ -53,24 +76,34 @@ __device__ __forceinline__ void pstage(const cplx *__restrict__ tw, int sign, LD
       const int j = w / B, bb = w % B, k = j % NS;
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) st(base + r * NS, bb, v[q][r], q, r);
+      for (int r = 0; r < R; r++) st(base + r * NS, bb, in.v[q][r], q, r);
     }
   }
 }
 
-// Runs the stages of plan P (R0, R1, R2; radix 1 = absent).  The first
-// stage reads with `first_ld`, the last writes with `last_st`; stages in
-// between go through LDS in place.
-template <class P, int R0, int R1, int R2, int B, int NT, class FL, class LL, class LS, class LST>
-__device__ __forceinline__ void run_plan(const cplx *tw, int sign, bool first_from_lds, FL first_ld,
-                                         LL ll, LS ls, LST last_st, bool last_to_lds) {
+// One stage: inputs through `ld(i, c)`, outputs to (j/NS)*NS*R + j%NS + r*NS
+// through `st(i, c, v, q, r)`.
+template <int R, int NS, int LEN, int B, int NT, class LD, class ST>
+__device__ __forceinline__ void pstage(const cplx *__restrict__ tw, int sign, LD ld, ST st,
+                                       bool sync_mid) {
+  stage_regs<R, LEN, B, NT> in;
+  pload<R, LEN, B, NT>(in, ld);
+  pfinish<R, NS, LEN, B, NT>(in, tw, sign, st, sync_mid);
+}
+
+// Runs the stages of plan P (R0, R1, R2; radix 1 = absent) from the first
+// stage's inputs already in registers (pload with the plan's R0); the last
+// stage writes with `last_st`, stages in between go through LDS in place.
+template <class P, int R0, int R1, int R2, int B, int NT, class LL, class LS, class LST>
+__device__ __forceinline__ void run_plan_from(stage_regs<R0, P::LEN, B, NT> &in, const cplx *tw,
+                                              int sign, bool first_from_lds, LL ll, LS ls,
+                                              LST last_st, bool last_to_lds) {
   constexpr int LEN = P::LEN;
   constexpr int NSTAGE = P::NSTAGE;
-  // stage 0
   if constexpr (NSTAGE == 1) {
-    pstage<R0, 1, LEN, B, NT>(tw, sign, first_ld, last_st, first_from_lds && last_to_lds);
+    pfinish<R0, 1, LEN, B, NT>(in, tw, sign, last_st, first_from_lds && last_to_lds);
   } else {
-    pstage<R0, 1, LEN, B, NT>(tw, sign, first_ld, ls, first_from_lds);
+    pfinish<R0, 1, LEN, B, NT>(in, tw, sign, ls, first_from_lds);
     __syncthreads();
     if constexpr (NSTAGE == 2) {
       pstage<R1, R0, LEN, B, NT>(tw, sign, ll, last_st, last_to_lds);
@@ -81,6 +114,15 @@ __device__ __forceinline__ void run_plan(const cplx *tw, int sign, bool first_fr
     }
   }
   if (last_to_lds) __syncthreads();
+}
+
+// Runs the stages of plan P; the first stage reads with `first_ld`.
+template <class P, int R0, int R1, int R2, int B, int NT, class FL, class LL, class LS, class LST>
+__device__ __forceinline__ void run_plan(const cplx *tw, int sign, bool first_from_lds, FL first_ld,
+                                         LL ll, LS ls, LST last_st, bool last_to_lds) {
+  stage_regs<R0, P::LEN, B, NT> in;
+  pload<R0, P::LEN, B, NT>(in, first_ld);
+  run_plan_from<P, R0, R1, R2, B, NT>(in, tw, sign, first_from_lds, ll, ls, last_st, last_to_lds);
 }
 
 }  // namespace inqamd

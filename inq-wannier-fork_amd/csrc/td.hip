@@ -58,6 +58,7 @@ struct moments_args {
 
 constexpr int MOM_BT = 64;
 constexpr int MOM_G = 4;
+constexpr int MOM_U = 4;
 
 __global__ __launch_bounds__(MOM_BT *MOM_G) void moments_kernel(const moments_args a) {
   __shared__ cplx red[MOM_G - 1][4][4][MOM_BT];
@@ -80,25 +81,45 @@ __global__ __launch_bounds__(MOM_BT *MOM_G) void moments_kernel(const moments_ar
 #pragma unroll
       for (int c = 0; c < 4; c++) acc[l][c] = mk(0.0, 0.0);
     if (b < a.nst) {
-      for (int i = i0 + g; i < i1; i += MOM_G) {
-        const cplx x = cmul(a.phase[off + i], f[(long)a.point[off + i] * a.ld + b]);
-        const float *ps = a.pos + (off + i) * 3;
-        const double p0 = ps[0], p1 = ps[1], p2 = ps[2];
-        double w[4];
-        w[0] = 1.0;
+      // MOM_U points per pass, all their loads issued before the sums (a
+      // point past the chunk end reloads point i and adds zero), so the
+      // gathers overlap; the summation order stays i, i + MOM_G, ...
+      for (int i = i0 + g; i < i1; i += MOM_G * MOM_U) {
+        int pt[MOM_U];
+        bool ok[MOM_U];
 #pragma unroll
-        for (int c = 1; c < 4; c++)
-          w[c] = p0 * a.L[(c - 1) * 3 + 0] + p1 * a.L[(c - 1) * 3 + 1] + p2 * a.L[(c - 1) * 3 + 2];
-        if (a.square2) w[2] = w[1] * w[1];
+        for (int u = 0; u < MOM_U; u++) {
+          ok[u] = i + u * MOM_G < i1;
+          pt[u] = ok[u] ? i + u * MOM_G : i;
+        }
+        cplx x[MOM_U];
+        double w[MOM_U][4], m[MOM_U][4];
 #pragma unroll
-        for (int l = 0; l < 4; l++) {
-          if (lm0 + l < nlm) {
-            const double m = M[(long)(lm0 + l) * a.max_sphere + i];
+        for (int u = 0; u < MOM_U; u++) {
+          const long q = off + pt[u];
+          x[u] = f[(long)a.point[q] * a.ld + b];
+          const cplx ph = a.phase[q];
+          const float *ps = a.pos + q * 3;
+          const double p0 = ps[0], p1 = ps[1], p2 = ps[2];
+          w[u][0] = 1.0;
+#pragma unroll
+          for (int c = 1; c < 4; c++)
+            w[u][c] = p0 * a.L[(c - 1) * 3 + 0] + p1 * a.L[(c - 1) * 3 + 1] + p2 * a.L[(c - 1) * 3 + 2];
+          if (a.square2) w[u][2] = w[u][1] * w[u][1];
+#pragma unroll
+          for (int l = 0; l < 4; l++)
+            m[u][l] = M[(long)(lm0 + l < nlm ? lm0 + l : nlm - 1) * a.max_sphere + pt[u]];
+          x[u] = ok[u] ? cmul(ph, x[u]) : mk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < MOM_U; u++) {
+#pragma unroll
+          for (int l = 0; l < 4; l++) {
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-              const double mw = m * w[c];
-              acc[l][c].x += mw * x.x;
-              acc[l][c].y += mw * x.y;
+              const double mw = m[u][l] * w[u][c];
+              acc[l][c].x += mw * x[u].x;
+              acc[l][c].y += mw * x[u].y;
             }
           }
         }
@@ -224,13 +245,25 @@ __global__ __launch_bounds__(64) void nl_pair_kernel(const cplx *mX, const cplx 
   if (threadIdx.x < 18) partial[(long)row * 18 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out[k] = sum over rows of partial[row][k] in row order (k < width).
-__global__ void sum_rows_kernel(const double *partial, int nrows, int width, double *out) {
-  const int k = threadIdx.x;
-  if (k >= width) return;
-  double s = 0.0;
-  for (int r = 0; r < nrows; r++) s += partial[(long)r * width + k];
-  out[k] = s;
+// out[k] = sum over rows of partial[row][k] (k < width), one 256-thread
+// block: thread t adds rows t, t + 256, ..., then a tree over the threads, a
+// fixed order.
+constexpr int SUM_ROWS_NT = 256;
+__global__ __launch_bounds__(SUM_ROWS_NT) void sum_rows_kernel(const double *partial, int nrows,
+                                                               int width, double *out) {
+  __shared__ double red[SUM_ROWS_NT];
+  for (int k = 0; k < width; k++) {
+    double s = 0.0;
+    for (int r = threadIdx.x; r < nrows; r += SUM_ROWS_NT) s += partial[(long)r * width + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = SUM_ROWS_NT / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
+  }
 }
 
 // Record layout (OBS_RECORD doubles): [0..7] G-space S_0..S_3 (complex),
@@ -648,8 +681,8 @@ static void gspace_record(handle &h, const cplx *Xh, const cplx *Yh, long ld, in
   hipLaunchKernelGGL(gspace_moments_kernel, dim3(OBS_GS_BLOCKS), dim3(256), 0, h.stream, Xh, Yh,
                      ld, nb, h.n[0], h.n[1], h.n[2], part);
   INQAMD_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(64), 0, h.stream, (const double *)part,
-                     OBS_GS_BLOCKS, 8, rec);
+  hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(SUM_ROWS_NT), 0, h.stream,
+                     (const double *)part, OBS_GS_BLOCKS, 8, rec);
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 
@@ -663,8 +696,8 @@ static void nl_record(handle &h, const cplx *mX, const cplx *mY, int nb, double 
                        (const double *)t.coeff, t.max_nlm, nb, part);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(64), 0, h.stream, (const double *)part, nrows,
-                     18, rec + 8);
+  hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(SUM_ROWS_NT), 0, h.stream,
+                     (const double *)part, nrows, 18, rec + 8);
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 
@@ -895,8 +928,8 @@ void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double 
     a.kad = w.kad;
     hipLaunchKernelGGL(gpair_kernel, dim3(OBS_GS_BLOCKS), dim3(256), 0, h.stream, a, E, part);
     INQAMD_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(64), 0, h.stream, (const double *)part,
-                       OBS_GS_BLOCKS, 1, rec);
+    hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(SUM_ROWS_NT), 0, h.stream,
+                       (const double *)part, OBS_GS_BLOCKS, 1, rec);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   // projector part of dL/ds from sphere moments of A_k and Y_j
@@ -913,8 +946,8 @@ void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double 
                        (const double *)t.coeff, nb, E, part);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(64), 0, h.stream, (const double *)part, nrows,
-                     2, rec + 1);
+  hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(SUM_ROWS_NT), 0, h.stream,
+                     (const double *)part, nrows, 2, rec + 1);
   INQAMD_HIP_CHECK(hipGetLastError());
   grad_fused_args ga;
   ga.inv_n = w.inv_n;
