@@ -56,17 +56,18 @@ struct batch_piece {
   int xy0;     // first x-y item of the piece in its chunk
 };
 
-// Grid lengths the batch kernels cover (larger grids run the one-system Hpsi)
+// Grid lengths the batch kernels cover (larger grids run the one-system
+// Hpsi), with the plans of INQAMD_FFT3_TABLE
 #define INQAMD_BATCH_TABLE(X) \
   X(24, 8, 3, 1)              \
-  X(30, 2, 3, 5)              \
+  X(30, 6, 5, 1)              \
   X(32, 8, 4, 1)              \
-  X(36, 4, 3, 3)              \
+  X(36, 6, 6, 1)              \
   X(40, 8, 5, 1)              \
-  X(48, 8, 2, 3)              \
-  X(60, 4, 3, 5)              \
+  X(48, 8, 6, 1)              \
+  X(60, 12, 5, 1)             \
   X(64, 8, 8, 1)              \
-  X(72, 8, 3, 3)
+  X(72, 8, 9, 1)
 
 constexpr int XY_MAX_NT = 384;  // plane_threads(72)
 
