@@ -328,6 +328,7 @@ struct inqamd_batch {
   std::vector<hipStream_t> aux;    // streams 1.. (stream 0 is `stream`)
   std::vector<hipEvent_t> aux_done;
   hipEvent_t proj_done = nullptr;
+  hipEvent_t start = nullptr;      // counters cleared: the aux streams may begin
   struct group {  // pieces sharing one launch of each sweep
     int p0, np;
     int L, ty;
@@ -587,6 +588,15 @@ void batch_run(inqamd_batch *b) {
   hipStream_t st = b->stream;
   hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(256), 0, st, b->counters, b->ncounters);
   INQAMD_HIP_CHECK(hipGetLastError());
+  // Chunks are dealt over the streams, the first to an aux stream: the z
+  // forward and x-y sweeps need no V_nl psi, so those of the aux streams'
+  // chunks run beside the projector, and each aux stream waits for the
+  // projector only before its first z backward sweep.
+  const int ns = std::min<int>(b->nstreams, 1 + (int)b->aux.size());
+  if (ns > 1) {
+    INQAMD_HIP_CHECK(hipEventRecord(b->start, st));
+    for (int k = 1; k < ns; k++) INQAMD_HIP_CHECK(hipStreamWaitEvent(b->aux[k - 1], b->start, 0));
+  }
   if (b->phase_items > 0) {
     hipLaunchKernelGGL(batch_phase_kernel, dim3((b->phase_items + 255) / 256), dim3(256), 0, st,
                        (const batch_phase_sys *)b->d_phase, b->nphase, b->phase_items);
@@ -597,19 +607,19 @@ void batch_run(inqamd_batch *b) {
                        (const batch_proj_sys *)b->d_proj, b->nproj, b->counters, b->proj_items);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
-  // chunks are independent once the projector has written V_nl psi: deal
-  // them over the streams
-  const int ns = std::min<int>(b->nstreams, 1 + (int)b->aux.size());
-  if (ns > 1) {
-    INQAMD_HIP_CHECK(hipEventRecord(b->proj_done, st));
-    for (int k = 1; k < ns; k++) INQAMD_HIP_CHECK(hipStreamWaitEvent(b->aux[k - 1], b->proj_done, 0));
-  }
+  if (ns > 1) INQAMD_HIP_CHECK(hipEventRecord(b->proj_done, st));
+  bool waited[4] = {true, false, false, false};  // stream 0 runs after the projector
   int ci = 0;
   for (const inqamd_batch::chunk &ch : b->chunks) {
-    const hipStream_t cs = (ci % ns) == 0 ? st : b->aux[(ci % ns) - 1];
+    const int si = (ci + 1) % ns;
+    const hipStream_t cs = si == 0 ? st : b->aux[si - 1];
     ci++;
     // z forward of every group of the chunk, then x-y, then z backward
-    for (int sweep = 0; sweep < 3; sweep++)
+    for (int sweep = 0; sweep < 3; sweep++) {
+      if (sweep == 2 && !waited[si]) {
+        INQAMD_HIP_CHECK(hipStreamWaitEvent(cs, b->proj_done, 0));
+        waited[si] = true;
+      }
       for (int gi = ch.g0; gi < ch.g0 + ch.ng; gi++) {
         const inqamd_batch::group &g = b->groups[gi];
         const batch_piece *pc = b->d_pieces + g.p0;
@@ -622,8 +632,11 @@ void batch_run(inqamd_batch *b) {
         void *args[] = {(void *)&pc, (void *)&np, (void *)&cnt, (void *)&total};
         INQAMD_HIP_CHECK(hipLaunchKernel(k, dim3(grid), dim3(nt), args, lds, cs));
       }
+    }
   }
   for (int k = 1; k < ns; k++) {
+    // an aux stream that got no z backward sweep still joins after the projector
+    if (!waited[k]) INQAMD_HIP_CHECK(hipStreamWaitEvent(b->aux[k - 1], b->proj_done, 0));
     INQAMD_HIP_CHECK(hipEventRecord(b->aux_done[k - 1], b->aux[k - 1]));
     INQAMD_HIP_CHECK(hipStreamWaitEvent(st, b->aux_done[k - 1], 0));
   }
@@ -657,6 +670,7 @@ void batch_release(inqamd_batch *b) {
     if (q) (void)hipStreamDestroy(q);
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->proj_done) (void)hipEventDestroy(b->proj_done);
+  if (b->start) (void)hipEventDestroy(b->start);
 }
 
 }  // namespace
@@ -697,6 +711,7 @@ int inqamd_batch_create(inqamd_hamiltonian_t const *handles, int nsys, void *str
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->events[s], hipEventDisableTiming));
     INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->proj_done, hipEventDisableTiming));
+    INQAMD_HIP_CHECK(hipEventCreateWithFlags(&b->start, hipEventDisableTiming));
     b->aux.assign(3, nullptr);
     b->aux_done.assign(3, nullptr);
     for (int k = 0; k < 3; k++) {
