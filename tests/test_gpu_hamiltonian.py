@@ -326,6 +326,41 @@ def test_hpsi_96_matches_oracle(device):
     ham.close()
 
 
+@pytest.mark.parametrize("L", [60, 72])
+def test_hpsi_60_72_match_oracle(device, L):
+    """Grids whose z tiles and x-y planes run radix-12 (60 = 12 x 5) and
+    radix-9 (72 = 8 x 9) butterflies, the 72-long tiles with 768 threads:
+    Hpsi of 11 bands with a vector potential, the batch kernels on the same
+    system, and a Taylor-4 exponential (the accumulating epilogue) against
+    the oracle."""
+    from inq_amd import batch
+    s = systems.uneven_batch_system(7, L, 6)
+    psi = systems.random_orbitals(s, 11, seed=3, system_id=L)
+    kA = (0.02, -0.01, 0.03)
+    say("hpsi %d^3 x 11" % L)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=11)
+    ham.uniform_vector_potential = kA
+    d = to_dev(psi, device)
+    h = to_host(ham(d))
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    ref = osys.hamiltonian(psi, kA)
+    assert relerr(h, ref) < TOL
+    assert band_relerr(h, ref) < TOL
+    b = batch.hamiltonian_batch([ham])
+    for group in (False, True):
+        say("batch kernels %d^3, one launch per grid length: %s" % (L, group))
+        b.set_tuning(group_by_grid=group)
+        out = b([d])[0]
+        torch.cuda.synchronize()
+        assert relerr(to_host(out), ref) < TOL
+    b.close()
+    say("exponential %d^3 x 3" % L)
+    e = to_dev(psi[..., :3], device)
+    operations.exponential_in_place(ham, 0.025j, e, 4)
+    assert relerr(to_host(e), osys.exponential_in_place(psi[..., :3], 0.025j, 4, kA)) < TOL
+    ham.close()
+
+
 def test_pipelined_chunks_graph_and_oracle(device):
     """A 64^3 system whose Hpsi runs two band chunks on two streams (max_bands
     64, 40 bands, 32-band chunks): Hpsi against the oracle, and a captured
