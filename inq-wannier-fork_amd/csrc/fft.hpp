@@ -96,6 +96,7 @@ struct fft3_args {
   double scale;      // 1/N
   // projector apply fused into the last sweep (own == nullptr: not fused):
   // hpsi(p, b) += sum_{j < cnt} Qe[e0 + j][b] with own[p] = (e0 << OWN_COUNT_BITS) | cnt
+  // (own[p] = -1 where no sphere reaches)
   const int *own;
   const cplx *Qe;        // [nent][ldQ], offset to the chunk's first band
   long ldQ;
@@ -111,7 +112,18 @@ struct fft3_args {
   cplx *gdump;
   // hpsi written with non-temporal stores (streamed past the caches)
   int nt_store;
+  // G-space Hpsi sweeps: G-space orbitals whose 1/2 |G+k+A|^2 multiple the
+  // last sweep adds (fft3_fourier_hamiltonian)
+  const cplx *kin_in;
+  long kin_ld;
 };
+
+// G-space Hpsi on the chunk's bands from psi = IFFT(phi) (INQ layout, `psi`)
+// and phi itself (`kin_in`): hpsi = FFT(V psi + V_nl psi) + 1/2 |G+k+A|^2 phi,
+// V_nl psi from the ownership entries Qe, in three sweeps (z forward of the
+// potential terms, x-y forward, a transposing write that adds the kinetic
+// term).
+void fft3_fourier_hamiltonian(const fft3_args &a, int L, hipStream_t stream);
 
 // True when the three-sweep kernels exist for length L and fit in max_lds.
 bool fft3_supported(int L, size_t max_lds);

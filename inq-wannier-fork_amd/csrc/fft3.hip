@@ -51,6 +51,27 @@ void z_bwd_epi_kernel(const fft3_args a) {
   z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), ACC>(a, blockIdx.x, lds);
 }
 
+// G-space Hpsi sweeps
+template <int R0, int R1, int R2, int TY>
+__global__ __launch_bounds__(z_threads(R0 * R1 * R2)) void z_pot_kernel(const fft3_args a) {
+  extern __shared__ cplx lds[];
+  z_fwd_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), true>(a, blockIdx.x, lds);
+}
+
+template <int R0, int R1, int R2, int NT>
+__global__ __launch_bounds__(NT) void xy_fwd_kernel(const fft3_args a) {
+  extern __shared__ cplx lds[];
+  xy_fwd_plane<R0, R1, R2, NT>(a, blockIdx.x, lds);
+}
+
+template <int R0, int R1, int R2, int TY>
+__global__ __launch_bounds__(z_threads(R0 * R1 * R2))
+__attribute__((amdgpu_waves_per_eu(z_bwd_waves(R0 * R1 * R2))))
+void z_kin_kernel(const fft3_args a) {
+  extern __shared__ cplx lds[];
+  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), false, true>(a, blockIdx.x, lds);
+}
+
 static int z_ty_choice() {
   static const int ty = [] {
     const char *e = getenv("INQAMD_Z_TY");
@@ -85,6 +106,9 @@ static void prepare3() {
   ensure_lds(z_fwd_kernel<R0, R1, R2, 16>, zl);
   ensure_lds(xy_kin_kernel<R0, R1, R2, plane_threads(L)>, pl);
   ensure_lds(xy_kin_kernel<R0, R1, R2, wide_threads(L)>, pl);
+  ensure_lds(z_pot_kernel<R0, R1, R2, 8>, zl);
+  ensure_lds(z_kin_kernel<R0, R1, R2, 8>, zl);
+  ensure_lds(xy_fwd_kernel<R0, R1, R2, wide_threads(L)>, pl);
   ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, false>, zl);
   ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, false>, zl);
   ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true>, zl);
@@ -132,6 +156,33 @@ static void launch3(const fft3_args &a, hipStream_t stream, int sweep) {
       launch3v<R0, R1, R2, 8, wide_threads(L)>(a, stream, sweep);
     else
       launch3v<R0, R1, R2, 8, plane_threads(L)>(a, stream, sweep);
+  }
+}
+
+template <int R0, int R1, int R2>
+static void launch_fourier(const fft3_args &a, hipStream_t stream) {
+  constexpr int L = R0 * R1 * R2;
+  const long ztiles = z_tiles<8>(L, a.nb);
+  const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
+  const size_t plds = (size_t)L * (L + 1) * sizeof(cplx);
+  hipLaunchKernelGGL((z_pot_kernel<R0, R1, R2, 8>), dim3((unsigned)ztiles), dim3(z_threads(L)), zlds,
+                     stream, a);
+  INQAMD_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((xy_fwd_kernel<R0, R1, R2, wide_threads(L)>), dim3((unsigned)((long)a.nb * L)),
+                     dim3(wide_threads(L)), plds, stream, a);
+  INQAMD_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((z_kin_kernel<R0, R1, R2, 8>), dim3((unsigned)ztiles), dim3(z_threads(L)), zlds,
+                     stream, a);
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+void fft3_fourier_hamiltonian(const fft3_args &a, int L, hipStream_t stream) {
+  switch (L) {
+#define INQAMD_FFT3_CASE(LL, R0, R1, R2) \
+  case LL: launch_fourier<R0, R1, R2>(a, stream); return;
+    INQAMD_FFT3_TABLE(INQAMD_FFT3_CASE)
+#undef INQAMD_FFT3_CASE
+    default: throw error{ERR_UNSUPPORTED, "G-space three-sweep Hpsi: unsupported grid length"};
   }
 }
 

@@ -37,7 +37,9 @@ constexpr int plane_threads(int L) {
 // ---------------------------------------------------------------------------
 // sweep 1: z forward, INQ layout -> band-major W
 // ---------------------------------------------------------------------------
-template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT>
+// POT: the tile transforms V psi + V_nl psi (ownership entries Qe) instead
+// of psi, formed as the first stage loads psi.
+template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool POT = false>
 __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
@@ -57,8 +59,20 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
   // of TB * 16 contiguous bytes, each used whole.
   auto gl = [&](int i, int c) -> cplx {
     const int b = c / Z_TY, y = c % Z_TY;
-    return (y0 + y < L && cb0 + b < a.nb) ? a.psi[(((long)x * L + y0 + y) * L + i) * a.ld + cb0 + b]
-                                          : mk(0.0, 0.0);
+    if (!(y0 + y < L && cb0 + b < a.nb)) return mk(0.0, 0.0);
+    const long p = ((long)x * L + y0 + y) * L + i;
+    cplx v = a.psi[p * a.ld + cb0 + b];
+    if constexpr (POT) {
+      const double vp = a.V[p];
+      v = mk(vp * v.x, vp * v.y);
+      // own[p] = -1 outside every sphere
+      const int w = a.own ? a.own[p] : -1;
+      if (w >= 0) {
+        const int e0 = w >> OWN_COUNT_BITS, cnt = w & OWN_MAX_COUNT;
+        for (int j = 0; j < cnt; j++) v = cadd(v, a.Qe[(long)(e0 + j) * a.ldQ + cb0 + b]);
+      }
+    }
+    return v;
   };
   auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + zswz(c)]; };
   auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + zswz(c)] = v; };
@@ -169,12 +183,34 @@ __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cpl
   xy_kin_plane_from<R0, R1, R2, NT>(a, plane, lds, in, a.tw, kc);
 }
 
+// The forward x-y transform of one plane (G-space Hpsi), in place in W
+template <int R0, int R1, int R2, int NT = plane_threads(R0 * R1 * R2)>
+__device__ __forceinline__ void xy_fwd_plane(const fft3_args &a, long plane, cplx *lds) {
+  using XP = xy_plan<R0, R1, R2>;
+  using P = typename XP::P;
+  constexpr int X0 = XP::X0, X1 = XP::X1, X2 = XP::X2;
+  constexpr int L = P::LEN;
+  constexpr int PAD = L + 1;
+  cplx *W = a.W + plane * L * L;
+  auto gx = [&](int i, int c) -> cplx { return W[i * L + c]; };          // (i = x, c = y)
+  auto lx = [&](int i, int c) -> cplx { return lds[i * PAD + c]; };
+  auto sx = [&](int i, int c, cplx v, int, int) { lds[i * PAD + c] = v; };
+  auto ly = [&](int i, int c) -> cplx { return lds[c * PAD + i]; };    // (i = y, c = x)
+  auto sy = [&](int i, int c, cplx v, int, int) { lds[c * PAD + i] = v; };
+  auto wy = [&](int i, int c, cplx v, int, int) { W[c * L + i] = v; };
+  run_plan<P, X0, X1, X2, L, NT>(a.tw, -1, false, gx, lx, sx, sx, true);  // x forward
+  run_plan<P, X0, X1, X2, L, NT>(a.tw, -1, true, ly, ly, sy, wy, false);  // y forward
+}
+
 // ---------------------------------------------------------------------------
 // sweep 3: z backward, x 1/N + V psi, band-major W -> INQ layout
 // ---------------------------------------------------------------------------
 // ACC: the caller may pass Taylor accumulators (acc1/acc2); without it they
 // must be null and the epilogue is a plain store.
-template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool ACC = false>
+// GS (G-space Hpsi): no z transform (the first sweep made it), only the
+// transposing write, whose epilogue adds 1/2 |G+k+A|^2 kin_in instead of
+// V psi (no ownership entries, no scale).
+template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool ACC = false, bool GS = false>
 __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
@@ -190,9 +226,18 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   const int x = (int)(t / nyt);
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
 
-  // prefetch V psi (and the ownership entries) of this thread's outputs
+  // prefetch V psi (and the ownership entries) of this thread's outputs,
+  // or 1/2 |G+k+A|^2 kin_in
   cplx pre[NE];
   int own[NE];
+  [[maybe_unused]] double gx[3];
+  if constexpr (GS) {
+    double kc[3];
+    xy_kc(a, kc);
+    const int mx = symmetric_index(x, L);
+#pragma unroll
+    for (int d = 0; d < 3; d++) gx[d] = mx * a.rlat[0 * 3 + d] + kc[d];
+  }
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     const int e = threadIdx.x + k * NT;
@@ -201,10 +246,18 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     own[k] = -1;
     if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
       const long p = ((long)x * L + y0 + y) * L + z;
-      const cplx ps = a.psi[p * a.ld + cb0 + b];
-      const double vp = a.V[p];
-      pre[k] = mk(vp * ps.x, vp * ps.y);
-      if (a.own) own[k] = a.own[p];
+      if constexpr (GS) {
+        const int my = symmetric_index(y0 + y, L), mz = symmetric_index(z, L);
+        const double q0 = gx[0] + my * a.rlat[3] + mz * a.rlat[6];
+        const double q1 = gx[1] + my * a.rlat[4] + mz * a.rlat[7];
+        const double q2 = gx[2] + my * a.rlat[5] + mz * a.rlat[8];
+        pre[k] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), a.kin_in[p * a.kin_ld + cb0 + b]);
+      } else {
+        const cplx ps = a.psi[p * a.ld + cb0 + b];
+        const double vp = a.V[p];
+        pre[k] = mk(vp * ps.x, vp * ps.y);
+        if (a.own) own[k] = a.own[p];
+      }
     }
   }
 
@@ -215,7 +268,16 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   };
   auto ll = [&](int i, int c) -> cplx { return lds[i * Z_COLS + zswz(c)]; };
   auto ls = [&](int i, int c, cplx v, int, int) { lds[i * Z_COLS + zswz(c)] = v; };
-  run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, +1, false, gl, ll, ls, ls, true);
+  if constexpr (GS) {
+    // the first sweep transformed z already: this sweep only transposes
+    for (int e = threadIdx.x; e < L * Z_COLS; e += NT) {
+      const int i = e / Z_COLS, c = e % Z_COLS;
+      lds[i * Z_COLS + zswz(c)] = gl(i, c);
+    }
+    __syncthreads();
+  } else {
+    run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, +1, false, gl, ll, ls, ls, true);
+  }
 
   // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
   // points, its ownership entries summed in fixed (sphere, point) order.

@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <vector>
 
 #include "internal.hpp"
@@ -454,6 +455,14 @@ __global__ __launch_bounds__(256) void kinetic_partial(gspace geo, const cplx *_
 // ks_hamiltonian::operator() on fourier_space orbitals (ks_hamiltonian.hpp:
 // 156-176): to_real, the real-space operator, to_fourier.  Uses hn and hn2.
 void hamiltonian_fourier(handle &h, const cplx *phi, long ld, cplx *hphi, long hld, int nst) {
+  static const bool sweeps = [] {
+    const char *e = getenv("INQAMD_GSPACE_SWEEPS");
+    return !(e && atoi(e) == 0);
+  }();
+  if (h.three_sweep && sweeps) {
+    hamiltonian_fourier_sweeps(h, phi, ld, hphi, hld, nst);
+    return;
+  }
   fft3d(h, phi, ld, h.hn, nst, nst, +1, 1.0 / (double)h.npoint);
   hamiltonian_apply(h, h.hn, nst, h.hn2, nst, nst, KA_CURRENT);
   fft3d(h, h.hn2, nst, hphi, hld, nst, -1, 1.0);

@@ -387,6 +387,44 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
   }
 }
 
+// ks_hamiltonian::operator() on G-space orbitals (ks_hamiltonian.hpp:156-176)
+// for three-sweep grids: psi = to_real(phi), the projector on psi, then
+// H phi = to_fourier(V psi + V_nl psi) + 1/2 |G+k+A|^2 phi in three sweeps
+// per band chunk (z forward, x-y forward, transposing write with the kinetic
+// term).  The kinetic term acts on phi directly: the real-space
+// Hpsi followed by to_fourier would transform psi forward and back again.
+void hamiltonian_fourier_sweeps(handle &h, const cplx *phi, long ld, cplx *hphi, long hld,
+                                int nst) {
+  fft3d(h, phi, ld, h.hn, nst, nst, +1, 1.0 / (double)h.npoint);
+  projector_phases(h, KA_CURRENT);
+  projector_project_expand(h, h.hn, nst, nst, h.vnl, nst, false);
+  const int chunk = band_chunk(h, nst);
+  for (int b0 = 0; b0 < nst; b0 += chunk) {
+    const int nb = nst - b0 < chunk ? nst - b0 : chunk;
+    fft3_args a{};
+    a.psi = h.hn + b0;
+    a.ld = nst;
+    a.hpsi = hphi + b0;
+    a.hld = hld;
+    a.W = h.work;
+    a.nb = nb;
+    a.V = h.V;
+    a.kA = h.kA + 3 * KA_CURRENT;
+    for (int d = 0; d < 9; d++) a.rlat[d] = h.rlat[d];
+    a.tw = h.axis[0].tw;
+    a.scale = 1.0;
+    const projector_tables &t = h.pt;
+    if (t.nproj > 0 && t.nunique > 0) {
+      a.own = t.own;
+      a.Qe = h.vnl + b0;
+      a.ldQ = nst;
+    }
+    a.kin_in = phi + b0;
+    a.kin_ld = ld;
+    fft3_fourier_hamiltonian(a, h.n[0], h.stream);
+  }
+}
+
 void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
                   int nst) {
   const long total = h.npoint * nst;

@@ -140,6 +140,10 @@ void mfma_selftest(double *out_device, hipStream_t stream);
 // hamiltonian.hip
 void set_slot(handle &h, int slot, const double v[3]);
 void update_current_slot(handle &h);
+// G-space Hpsi on a three-sweep grid (phi and hphi distinct from the
+// handle's work arrays)
+void hamiltonian_fourier_sweeps(handle &h, const cplx *phi, long ld, cplx *hphi, long hld,
+                                int nst);
 void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int nst, int sign,
            double scale);
 void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,

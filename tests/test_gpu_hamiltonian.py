@@ -286,6 +286,28 @@ def test_fourier_space_hpsi_matches_oracle(device, cfg0):
     ham.close()
 
 
+@pytest.mark.parametrize("L", [48, 64])
+def test_fourier_space_hpsi_sweeps_match_oracle(device, L):
+    """The G-space Hpsi's three-sweep form (to_real, projector, then
+    to_fourier of V psi + V_nl psi with 1/2 |G+k+A|^2 phi added in the last
+    sweep) on a radix-6 and a radix-8 grid, with a vector potential, two
+    band chunks at 64, against the oracle's restatement."""
+    s = systems.diamond_system(1, grid=48) if L == 48 else systems.diamond_system(2)
+    nst = 24 if L == 48 else 72
+    psi = systems.random_orbitals(s, nst, seed=2)
+    phi_g = O.to_fourier(psi)
+    kA = (0.03, -0.02, 0.05)
+    say("fourier-space hpsi sweeps %d^3 x %d" % (L, nst))
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    ham.uniform_vector_potential = kA
+    h = to_host(ham(to_dev(phi_g, device), fourier=True))
+    k = 5
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian_fourier(
+        np.ascontiguousarray(phi_g[..., -k:]), kA)
+    assert relerr(h[..., -k:], ref) < TOL
+    ham.close()
+
+
 def test_energy_terms_match_oracle(device, cfg0):
     """energy::calculate's per-band terms (energy.hpp:59-87): eigenvalues,
     residual norms and non-local expectation values against the oracle, with
