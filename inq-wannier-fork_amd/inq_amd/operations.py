@@ -47,7 +47,7 @@ class transform:
         return out
 
 
-# BASELINE's name for the same operations
+# operations::space: the same transforms under their other name
 space = transform
 
 
