@@ -104,6 +104,7 @@ struct fft3_args {
   cplx *acc1;
   long acc1_ld;
   double c1re, c1im;
+  int acc1_init;  // acc1 = psi + c1 * hpsi instead of acc1 += c1 * hpsi
   cplx *acc2;
   long acc2_ld;
   double c2re, c2im;

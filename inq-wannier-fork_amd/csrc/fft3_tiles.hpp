@@ -341,7 +341,8 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
         const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
         if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
           const long p = ((long)x * L + y0 + y) * L + z;
-          if (a.acc1) o1[g] = a.acc1[p * a.acc1_ld + cb0 + b];
+          if (a.acc1)
+            o1[g] = a.acc1_init ? a.psi[p * a.ld + cb0 + b] : a.acc1[p * a.acc1_ld + cb0 + b];
           if (a.acc2) o2[g] = a.acc2[p * a.acc2_ld + cb0 + b];
         }
       }

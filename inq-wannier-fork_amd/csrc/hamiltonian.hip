@@ -268,6 +268,7 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       a.acc1_ld = acc->ld1;
       a.c1re = acc->c1re;
       a.c1im = acc->c1im;
+      a.acc1_init = acc->init1 ? 1 : 0;
     }
     if (acc && acc->p2) {
       a.acc2 = acc->p2 + b0;
@@ -382,6 +383,7 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
   }
   if (!h.three_sweep) {
     projector_apply_coefficients(h, hpsi, hld, nst, slot);
+    if (acc && acc->p1 && acc->init1) copy(h, psi, ld, acc->p1, acc->ld1, nst);
     if (acc && acc->p1) shift(h, {acc->c1re, acc->c1im}, hpsi, hld, acc->p1, acc->ld1, nst);
     if (acc && acc->p2) shift(h, {acc->c2re, acc->c2im}, hpsi, hld, acc->p2, acc->ld2, nst);
   }

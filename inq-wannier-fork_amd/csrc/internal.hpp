@@ -62,6 +62,7 @@ struct taylor_acc {
   cplx *p1 = nullptr;
   long ld1 = 0;
   double c1re = 0.0, c1im = 0.0;
+  bool init1 = false;  // p1 = psi + c1 hpsi (p1's old values unused)
   cplx *p2 = nullptr;
   long ld2 = 0;
   double c2re = 0.0, c2im = 0.0;

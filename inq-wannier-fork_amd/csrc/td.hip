@@ -834,16 +834,19 @@ void slice_copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nb) {
   copy(h, x, xld, y, yld, nb);
 }
 
-// One Taylor half step of `nb` bands in place on cur (ld = nb), writing
-// H^j phi (j < nstore) into the tape slots Y[j] (Y[0] = phi).  The
-// arithmetic is exponential_in_place's: each H application accumulates its
-// term into cur in its last sweep.
-void replay_half(inqamd_td *td, cplx *cur, int nb, cplx *const *Y, int slot) {
+// One Taylor half step of `nb` bands (ld = nb), writing H^j phi (j <
+// nstore) into the tape slots Y[j] (Y[0] = phi).  The arithmetic is
+// exponential_in_place's: each H application adds its term to the new
+// state in its last sweep.  With one slot (nstore == 1) the step runs in
+// place on cur after copying phi to Y[0]; with the powers stored, phi is
+// already in Y[0] and the new state goes to `next` (the first H
+// application writes phi + z H phi there, so nothing is copied).
+void replay_half(inqamd_td *td, cplx *cur, int nb, cplx *const *Y, cplx *next, int slot) {
   handle &h = *td->h;
   const int order = td->d.order;
   const std::complex<double> z(0.0, td->d.dt / 2.0);
-  copy(h, cur, nb, Y[0], nb, nb);
   if (td->nstore == 1) {
+    copy(h, cur, nb, Y[0], nb, nb);
     exponential_in_place(h, z, order, cur, nb, nb, slot);
     return;
   }
@@ -851,10 +854,11 @@ void replay_half(inqamd_td *td, cplx *cur, int nb, cplx *const *Y, int slot) {
   for (int it = 1; it <= order; it++) {
     coeff *= z / double(it);
     taylor_acc acc;
-    acc.p1 = cur;
+    acc.p1 = next;
     acc.ld1 = nb;
     acc.c1re = coeff.real();
     acc.c1im = coeff.imag();
+    acc.init1 = it == 1;
     if (td->fused) acc.gdump = Y[order + it - 1];  // transform of Y_{it-1}
     cplx *out = it < order ? Y[it] : td->hk[0];
     hamiltonian_apply(h, Y[it - 1], nb, out, nb, nb, slot, &acc);
@@ -868,8 +872,8 @@ void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double 
   const int M = td->d.order;
   const std::complex<double> z(0.0, td->d.dt / 2.0);
   const std::complex<double> zc = std::conj(z);
-  // lambda_in = sum_k conj(z)^k/k! H^k lambda' (the k >= 1 terms fused)
-  copy(h, td->lam, nb, td->lam_in, nb, nb);
+  // lambda_in = sum_k conj(z)^k/k! H^k lambda' (the k >= 1 terms fused; the
+  // first H application writes lambda' + conj(z) H lambda')
   const cplx *A[TD_MAXM];
   A[0] = td->lam;
   std::complex<double> ck = 1.0;
@@ -878,6 +882,7 @@ void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double 
     taylor_acc acc;
     acc.p1 = td->lam_in;
     acc.ld1 = nb;
+    acc.init1 = k == 1;
     acc.c1re = ck.real();
     acc.c1im = ck.imag();
     acc.gdump = td->Ah[k - 1];
@@ -1303,8 +1308,10 @@ int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double 
       adjoint_start(td, td->cur, w, KA_T, kT, T, dloss);
       for (int s = td->nseg - 1; s >= 0; s--) {
         const int n0 = s * K, n1 = n0 + K < d.nsteps ? n0 + K : d.nsteps;
-        // replay the segment, keeping each half step's input (and powers)
-        slice_copy(*h, td->ckpt + (size_t)s * N * nst + b0, nst, td->cur, w, w);
+        // replay the segment, keeping each half step's input (and powers);
+        // with the powers stored the state moves from tape slot to tape slot
+        cplx *first = td->nstore == 1 ? td->cur : td->tape;
+        slice_copy(*h, td->ckpt + (size_t)s * N * nst + b0, nst, first, w, w);
         for (int n = n0; n < n1; n++)
           for (int half = 0; half < 2; half++) {
             double kv[3];
@@ -1314,7 +1321,9 @@ int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double 
             cplx *Y[16];
             const size_t base = ((size_t)(n - n0) * 2 + half) * td->nstore;
             for (int j = 0; j < td->nstore; j++) Y[j] = td->tape + (base + j) * slice_elems;
-            replay_half(td, td->cur, w, Y, slot);
+            const bool last = n == n1 - 1 && half == 1;
+            cplx *next = last ? td->cur : td->tape + (base + td->nstore) * slice_elems;
+            replay_half(td, td->cur, w, Y, next, slot);
           }
         // back through the segment
         for (int n = n1 - 1; n >= n0; n--)
