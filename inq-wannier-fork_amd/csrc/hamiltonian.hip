@@ -465,10 +465,34 @@ void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *ps
   }
 }
 
-// exponential_2_for_1 (exponential.hpp:50-69)
+// out = T(z, H) in (exponential_in_place on a copy of in, without the
+// copy: the first H application writes in + z H in to out)
+void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, cplx *out,
+                      long ld, int nst, int slot) {
+  cplx *hn = h.hn, *hn2 = h.hn2;
+  const long tl = nst;
+  std::complex<double> coeff = 1.0;
+  for (int iter = 1; iter <= order; iter++) {
+    coeff *= z / double(iter);
+    taylor_acc acc;
+    acc.p1 = out;
+    acc.ld1 = ld;
+    acc.c1re = coeff.real();
+    acc.c1im = coeff.imag();
+    acc.init1 = iter == 1;
+    if (iter == 1) {
+      hamiltonian_apply(h, in, ld, hn, tl, nst, slot, &acc);
+    } else {
+      hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot, &acc);
+      std::swap(hn, hn2);
+    }
+  }
+}
+
+// exponential_2_for_1 (exponential.hpp:50-69); expphi starts as phi + z1 H phi
+// from the first H application (no copy of phi)
 void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2,
                                 int order, cplx *phi, cplx *expphi, long ld, int nst, int slot) {
-  copy(h, phi, ld, expphi, ld, nst);
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
   std::complex<double> c1 = 1.0, c2 = 1.0;
@@ -480,6 +504,7 @@ void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double
     acc.ld1 = ld;
     acc.c1re = c1.real();
     acc.c1im = c1.imag();
+    acc.init1 = iter == 1;
     acc.p2 = phi;
     acc.ld2 = ld;
     acc.c2re = c2.real();

@@ -20,6 +20,8 @@ namespace inqamd {
 
 void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2, int order,
                          cplx *phi, cplx *expphi, long ld, int nst, int slot);
+void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, cplx *out,
+                      long ld, int nst, int slot);
 
 namespace {
 
@@ -359,11 +361,10 @@ int etrs_scf(handle &h, double dt, const double A_t[3], const double A_tdt[3], i
   for (int d = 0; d < 3; d++) h.A_cov[d] = A_tdt[d];
   update_current_slot(h);
   // :56-58 restore the half step; :61-80 the second half, self-consistently
-  copy(h, t.save, ld, psi, ld, nst);
+  // (each pass psi = T(i dt/2, H(t+dt)) save, written from save directly)
   int iscf = 0;
   for (; iscf < nscf; iscf++) {
-    if (iscf != 0) copy(h, t.save, ld, psi, ld, nst);
-    exponential_in_place(h, std::complex<double>(0.0, dt / 2.0), order, psi, ld, nst, KA_TDT);
+    exponential_from(h, std::complex<double>(0.0, dt / 2.0), order, t.save, psi, ld, nst, KA_TDT);
     std::swap(t.rho, t.rho_old);
     density(h, psi, ld, nst, t.occ, t.rho);
     double s[4];
