@@ -393,12 +393,15 @@ def run_gs_64(args):
 
 def _tddft_bytes(N, nst, sum_k, nscf):
     """Algorithmic bytes of one interacting ETRS step on nst bands: the
-    2-for-1 Taylor-4 step (four Hpsi, two accumulators read and written per
-    order), then per SCF pass a restore of the half step (32N), a Taylor-4 half
-    step (four Hpsi, one accumulator per order) and a density pass (16N),
+    2-for-1 Taylor-4 step (four Hpsi; two accumulators read and written per
+    order, except the first order's exp(phi), which is only written), then
+    per SCF pass a Taylor-4 half step from the saved state (four Hpsi, one
+    accumulator per order, the first only written) and a density pass (16N),
     plus the density of the full step (16N)."""
     b_h = algorithmic_bytes_per_band(N, nst, sum_k)
-    per_band = (4 * b_h + 4 * 64.0 * N) + 16.0 * N + nscf * (32.0 * N + 4 * b_h + 4 * 32.0 * N + 16.0 * N)
+    two_for_one = 4 * b_h + 48.0 * N + 3 * 64.0 * N
+    half = 4 * b_h + 16.0 * N + 3 * 32.0 * N
+    per_band = two_for_one + 16.0 * N + nscf * (half + 16.0 * N)
     return nst * per_band
 
 
