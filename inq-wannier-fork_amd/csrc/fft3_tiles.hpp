@@ -169,8 +169,44 @@ __device__ __forceinline__ void xy_kin_plane_from(const fft3_args &a, long plane
   };
 
   run_plan_from<P, X0, X1, X2, L, NT>(in, tw, -1, false, lx, sx, sx, true);  // x forward
-  run_plan<P, X0, X1, X2, L, NT>(tw, -1, true, ly, ly, sy, ky, true);   // y forward, x T
-  run_plan<P, X0, X1, X2, L, NT>(tw, +1, true, ly, ly, sy, sy, true);   // y backward
+  if constexpr (P::NSTAGE == 2) {
+    // y forward: its last stage stays in registers, where butterfly j < X0
+    // holds outputs j + r X0 (r < X1); after the kinetic factor they are
+    // exactly the inputs of the first stage of the inverse run with the
+    // radices in reverse order (X1, X0), which therefore starts from the
+    // same registers (one LDS round trip and a barrier fewer)
+    constexpr int NB = L / X1;  // = X0
+    constexpr int NBF = stage_regs<X1, L, L, NT>::NBF;
+    pstage<X0, 1, L, L, NT>(tw, -1, ly, sy, true);
+    __syncthreads();
+    stage_regs<X1, L, L, NT> r;
+    pload<X1, L, L, NT>(r, ly);
+    pcompute<X1, X0, L, L, NT>(r, tw, -1);
+#pragma unroll
+    for (int q = 0; q < NBF; q++) {
+      const int w = threadIdx.x + q * NT;
+      if ((int)threadIdx.x < NT && (NBF * NT == NB * L || w < NB * L)) {
+        const int j = w / L, c = w % L;
+#pragma unroll
+        for (int rr = 0; rr < X1; rr++) {
+          const int i = j + rr * X0;
+          if (G) G[c * L + i] = r.v[q][rr];
+          const int my = symmetric_index(i, L), mx = symmetric_index(c, L);
+          const double q0 = mx * b0[0] + my * b1[0] + gz[0];
+          const double q1 = mx * b0[1] + my * b1[1] + gz[1];
+          const double q2 = mx * b0[2] + my * b1[2] + gz[2];
+          r.v[q][rr] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), r.v[q][rr]);
+        }
+      }
+    }
+    pfinish<X1, 1, L, L, NT>(r, tw, +1, sy, true);  // y backward, plan (X1, X0)
+    __syncthreads();
+    pstage<X0, X1, L, L, NT>(tw, +1, ly, sy, true);
+    __syncthreads();
+  } else {
+    run_plan<P, X0, X1, X2, L, NT>(tw, -1, true, ly, ly, sy, ky, true);  // y forward, x T
+    run_plan<P, X0, X1, X2, L, NT>(tw, +1, true, ly, ly, sy, sy, true);  // y backward
+  }
   run_plan<P, X0, X1, X2, L, NT>(tw, +1, true, lx, lx, sx, wx, false);  // x backward
 }
 

@@ -48,11 +48,12 @@ __device__ __forceinline__ void pload(stage_regs<R, LEN, B, NT> &in, LD ld) {
   }
 }
 
-// Twiddles, butterflies and stores of one stage with compile-time radix R
-// and span NS over a LEN x B tile, from inputs already in registers.
-template <int R, int NS, int LEN, int B, int NT, class ST>
-__device__ __forceinline__ void pfinish(stage_regs<R, LEN, B, NT> &in, const cplx *__restrict__ tw,
-                                        int sign, ST st, bool sync_mid) {
+// Twiddles and butterflies of one stage with compile-time radix R and span
+// NS over a LEN x B tile, in registers: slot (q, r) ends up holding output
+// (j/NS)*NS*R + j%NS + r*NS of column w % B.
+template <int R, int NS, int LEN, int B, int NT>
+__device__ __forceinline__ void pcompute(stage_regs<R, LEN, B, NT> &in, const cplx *__restrict__ tw,
+                                         int sign) {
   constexpr int NB = LEN / R;
   constexpr int NBF = stage_regs<R, LEN, B, NT>::NBF;
   constexpr int TSTRIDE = LEN / (NS * R);
@@ -68,6 +69,16 @@ __device__ __forceinline__ void pfinish(stage_regs<R, LEN, B, NT> &in, const cpl
       dft<R>(in.v[q], sign);
     }
   }
+}
+
+// Twiddles, butterflies and stores of one stage with compile-time radix R
+// and span NS over a LEN x B tile, from inputs already in registers.
+template <int R, int NS, int LEN, int B, int NT, class ST>
+__device__ __forceinline__ void pfinish(stage_regs<R, LEN, B, NT> &in, const cplx *__restrict__ tw,
+                                        int sign, ST st, bool sync_mid) {
+  constexpr int NB = LEN / R;
+  constexpr int NBF = stage_regs<R, LEN, B, NT>::NBF;
+  pcompute<R, NS, LEN, B, NT>(in, tw, sign);
   if (sync_mid) __syncthreads();
 #pragma unroll
   for (int q = 0; q < NBF; q++) {
