@@ -684,13 +684,20 @@ static void poisson3d(const int n[3], const cell &c, const double *rho, double *
  * correlation, the functionals libxc calls LDA_X and LDA_C_PZ that
  * options::theory::lda() selects (src/options/theory.hpp:48-53).  Energy
  * per particle e and potential v = d(n e)/dn; densities below 1e-15 give 0. */
-static void lda_point(double n, double &e, double &v) {
+static void lda_x_point(double n, double &e, double &v) {
   e = v = 0.0;
   if (n < 1e-15) return;
   const double cx = std::cbrt(3.0 / M_PI);
   const double n13 = std::cbrt(n);
-  const double ex = -0.75 * cx * n13;
-  const double vx = -cx * n13;
+  e = -0.75 * cx * n13;
+  v = -cx * n13;
+}
+
+static void lda_point(double n, double &e, double &v) {
+  e = v = 0.0;
+  if (n < 1e-15) return;
+  double ex, vx;
+  lda_x_point(n, ex, vx);
   const double rs = std::cbrt(3.0 / (4.0 * M_PI * n));
   double ec, vc;
   if (rs >= 1.0) {
@@ -707,6 +714,131 @@ static void lda_point(double n, double &e, double &v) {
   }
   e = ex + ec;
   v = vx + vc;
+}
+
+/* Unpolarised PBE (Perdew, Burke, Ernzerhof, PRL 77, 3865 (1996)), the
+ * functionals libxc calls GGA_X_PBE and GGA_C_PBE that options::theory's
+ * default selects (src/options/theory.hpp:40-45, 73-77).  For a point with
+ * density n and sigma = |grad n|^2: energy per particle e and the partial
+ * derivatives vn = d(n e)/dn, vs = d(n e)/dsigma.  The uniform-gas
+ * correlation is Perdew-Wang 1992 (PRB 45, 13244) with the parameters of
+ * libxc's LDA_C_PW_MOD (A = 0.0310907), as GGA_C_PBE uses it. */
+static void pw92_point(double rs, double &ec, double &dec) {
+  const double A = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382,
+               b4 = 0.49294;
+  const double sq = std::sqrt(rs);
+  const double q0 = -2.0 * A * (1.0 + a1 * rs);
+  const double q1 = 2.0 * A * (b1 * sq + b2 * rs + b3 * rs * sq + b4 * rs * rs);
+  const double q1p = A * (b1 / sq + 2.0 * b2 + 3.0 * b3 * sq + 4.0 * b4 * rs);
+  const double lg = std::log(1.0 + 1.0 / q1);
+  ec = q0 * lg;
+  dec = -2.0 * A * a1 * lg - q0 * q1p / (q1 * q1 + q1);
+}
+
+static void pbe_x_point(double n, double sigma, double &e, double &vn, double &vs) {
+  e = vn = vs = 0.0;
+  if (n < 1e-15) return;
+  const double kappa = 0.8040, mu = 0.2195149727645171;
+  const double ax = -0.75 * std::cbrt(3.0 / M_PI);
+  const double n13 = std::cbrt(n);
+  const double c = 4.0 * std::pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+  const double x = sigma / (c * n13 * n13 * n13 * n13 * n13 * n13 * n13 * n13);  // s^2
+  const double den = 1.0 + mu * x / kappa;
+  const double F = 1.0 + kappa - kappa / den;
+  const double dF = mu / (den * den);
+  e = ax * n13 * F;
+  vn = (4.0 / 3.0) * ax * n13 * F + ax * n13 * n * dF * (-8.0 / 3.0) * x / n;
+  vs = ax * n13 * n * dF / (c * n13 * n13 * n13 * n13 * n13 * n13 * n13 * n13);
+}
+
+static void pbe_c_point(double n, double sigma, double &e, double &vn, double &vs) {
+  e = vn = vs = 0.0;
+  if (n < 1e-15) return;
+  const double beta = 0.06672455060314922, gamma = (1.0 - std::log(2.0)) / (M_PI * M_PI);
+  const double rs = std::cbrt(3.0 / (4.0 * M_PI * n));
+  double ec, decdrs;
+  pw92_point(rs, ec, decdrs);
+  const double dec = decdrs * (-rs / (3.0 * n));  // d ec / dn
+  const double kf = std::cbrt(3.0 * M_PI * M_PI * n);
+  const double y = sigma * M_PI / (16.0 * kf * n * n);  // t^2
+  const double ex = std::exp(-ec / gamma);
+  const double A = (beta / gamma) / (ex - 1.0);
+  const double dAde = (beta / (gamma * gamma)) * ex / ((ex - 1.0) * (ex - 1.0));
+  const double P = 1.0 + A * y, D = 1.0 + A * y + A * A * y * y;
+  const double N = (beta / gamma) * y * P / D;
+  const double H = gamma * std::log(1.0 + N);
+  const double dNdy = (beta / gamma) * ((P + A * y) * D - y * P * (A + 2.0 * A * A * y)) / (D * D);
+  const double dNdA = (beta / gamma) * y * (y * D - P * (y + 2.0 * A * y * y)) / (D * D);
+  const double dHdy = gamma / (1.0 + N) * dNdy, dHdA = gamma / (1.0 + N) * dNdA;
+  e = ec + H;
+  vn = e + n * (dec + dHdA * dAde * dec + dHdy * (-7.0 / 3.0) * y / n);
+  vs = n * dHdy * M_PI / (16.0 * kf * n * n);
+}
+
+/* hamiltonian::xc_term::evaluate_functional for a GGA, one spin component
+ * (src/hamiltonian/xc_term.hpp:267-300): grad n = Re to_real(i G to_fourier
+ * n) (operations::gradient, src/operations/gradient.hpp:22-45), sigma =
+ * |grad n|^2, the point forms, then v = vn - 2 div(vs grad n) with the same
+ * spectral divergence; e_xc = dV sum e n.  which: 1 = exchange, 2 =
+ * correlation, 3 = both. */
+static void gga_xc(const int n[3], const cell &c, const double *rho, int which, double *vxc,
+                   double *exc) {
+  const long N = (long)n[0] * n[1] * n[2];
+  std::vector<cplx> f((size_t)N), g((size_t)N), h((size_t)N);
+  for (long p = 0; p < N; p++) f[p] = rho[p];
+  fft3d(n, 1, 1, f.data(), g.data(), -1, false);
+  auto gcart = [&](int ix, int iy, int iz) {
+    vec3 gv{{2.0 * M_PI * symmetric(ix, n[0]), 2.0 * M_PI * symmetric(iy, n[1]),
+             2.0 * M_PI * symmetric(iz, n[2])}};
+    return c.cov_to_cart(gv);
+  };
+  std::vector<double> grad[3];
+  for (int d = 0; d < 3; d++) {
+    grad[d].resize((size_t)N);
+    for (int ix = 0; ix < n[0]; ix++)
+      for (int iy = 0; iy < n[1]; iy++)
+        for (int iz = 0; iz < n[2]; iz++) {
+          const long p = ((long)ix * n[1] + iy) * n[2] + iz;
+          h[p] = cplx(0.0, gcart(ix, iy, iz)[d]) * g[p] / (double)N;
+        }
+    fft3d(n, 1, 1, h.data(), f.data(), +1, false);
+    for (long p = 0; p < N; p++) grad[d][p] = std::real(f[p]);
+  }
+  std::vector<double> vs((size_t)N);
+  double es = 0.0;
+  for (long p = 0; p < N; p++) {
+    const double nn = rho[p] < 0.0 ? 0.0 : rho[p];
+    const double sigma = grad[0][p] * grad[0][p] + grad[1][p] * grad[1][p] + grad[2][p] * grad[2][p];
+    double e = 0.0, vn = 0.0, vsig = 0.0;
+    if (which & 1) {
+      double e1, v1, s1;
+      pbe_x_point(nn, sigma, e1, v1, s1);
+      e += e1, vn += v1, vsig += s1;
+    }
+    if (which & 2) {
+      double e1, v1, s1;
+      pbe_c_point(nn, sigma, e1, v1, s1);
+      e += e1, vn += v1, vsig += s1;
+    }
+    vxc[p] = vn;
+    vs[p] = vsig;
+    es += e * nn;
+  }
+  // div(-2 vs grad n)
+  std::vector<cplx> acc((size_t)N, cplx(0.0, 0.0));
+  for (int d = 0; d < 3; d++) {
+    for (long p = 0; p < N; p++) f[p] = -2.0 * vs[p] * grad[d][p];
+    fft3d(n, 1, 1, f.data(), g.data(), -1, false);
+    for (int ix = 0; ix < n[0]; ix++)
+      for (int iy = 0; iy < n[1]; iy++)
+        for (int iz = 0; iz < n[2]; iz++) {
+          const long p = ((long)ix * n[1] + iy) * n[2] + iz;
+          acc[p] += cplx(0.0, gcart(ix, iy, iz)[d]) * g[p] / (double)N;
+        }
+  }
+  fft3d(n, 1, 1, acc.data(), f.data(), +1, false);
+  for (long p = 0; p < N; p++) vxc[p] += std::real(f[p]);
+  *exc = c.volume / (double)N * es;
 }
 
 /* hamiltonian::xc_term::operator() (src/hamiltonian/xc_term.hpp:120-142) for
@@ -1175,6 +1307,29 @@ int orc_poisson(const orc_system *sys, const double *rho, double *vh) {
 int orc_lda_xc(long npoint, double dV, const double *rho, double *vxc, double *exc, double *nvxc) {
   if (npoint <= 0) return 1;
   lda_xc(npoint, dV, rho, vxc, exc, nvxc);
+  return 0;
+}
+
+int orc_xc(const orc_system *sys, const double *rho, int kind, double *vxc, double *exc) {
+  if (!sys || kind < 1 || kind > 5) return 1;
+  cell c(sys->lattice);
+  const long N = (long)sys->n[0] * sys->n[1] * sys->n[2];
+  if (kind >= 3) {
+    gga_xc(sys->n, c, rho, kind - 2, vxc, exc);
+    return 0;
+  }
+  double es = 0.0;
+  for (long p = 0; p < N; p++) {
+    const double nn = rho[p] < 0.0 ? 0.0 : rho[p];
+    double e, v;
+    if (kind == 1)
+      lda_x_point(nn, e, v);
+    else
+      lda_point(nn, e, v);
+    vxc[p] = v;
+    es += e * nn;
+  }
+  *exc = c.volume / (double)N * es;
   return 0;
 }
 

@@ -51,6 +51,11 @@
  *                       values and continuity at rs = 1, v = d(n e)/dn by
  *                       finite differences (libxc's own rounding: unpinned)
  *   orc_dipole          reference known answers of dipole.hpp:60-115
+ *   orc_xc              reference known answers of xc_term.hpp:340-430 (LDA_X and
+ *                       GGA_C_PBE energies and potentials on the reference's density:
+ *                       PW92 gas, FFT gradient and divergence); GGA_X_PBE by its
+ *                       closed-form limits; every kind's potential as the functional
+ *                       derivative of its energy (central differences)
  *   orc_ks_potential    sum of the pinned parts; E_H = 1/2 int rho V_H
  *   orc_etrs_scf        frozen limit = orc_etrs_step; time reversal; energy
  *                       conservation of the interacting step
@@ -196,6 +201,10 @@ int orc_poisson(const orc_system *sys, const double *rho, double *vh);
 int orc_lda_xc(long npoint, double dV, const double *rho, double *vxc, double *exc, double *nvxc);
 /* observables::dipole of a density (dipole.hpp:24-40): dV sum r_cart rho */
 int orc_dipole(const orc_system *sys, const double *rho, double *out);
+/* Exchange-correlation of one spin component: kind 1 = LDA_X, 2 = LDA_X +
+ * LDA_C_PZ, 3 = GGA_X_PBE, 4 = GGA_C_PBE, 5 = both PBE parts; vxc[N] and
+ * exc = dV sum e n (xc_term.hpp:254-300) */
+int orc_xc(const orc_system *sys, const double *rho, int kind, double *vxc, double *exc);
 /* self_consistency::update_hamiltonian (self_consistency.hpp:88-162):
  * vks = vion + V_H (hartree) + V_xc (xc); energies = {external, hartree, exc, nvxc} */
 int orc_ks_potential(const orc_system *sys, const double *vion, const double *rho, int hartree,

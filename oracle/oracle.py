@@ -99,6 +99,8 @@ def lib():
         L.orc_lda_xc.argtypes = [ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.orc_dipole.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_xc.argtypes = [sysp, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                             ctypes.POINTER(ctypes.c_double)]
         L.orc_ks_potential.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_etrs_scf.argtypes = [sysp, ctypes.c_void_p, ctypes.c_void_p, _c_dbl_p, _c_dbl_p,
@@ -314,6 +316,18 @@ class System:
         out = np.zeros(3)
         _check(lib().orc_dipole(ctypes.byref(self.s), _ptr(rho), _ptr(out)), "dipole")
         return out
+
+    XC_KINDS = {"lda_x": 1, "lda": 2, "pbe_x": 3, "pbe_c": 4, "pbe": 5}
+
+    def xc(self, rho, kind):
+        """One exchange-correlation functional of a density (xc_term.hpp:254-300):
+        (vxc, exc) for kind in lda_x, lda (LDA_X + LDA_C_PZ), pbe_x, pbe_c, pbe."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        vxc = np.zeros_like(rho)
+        exc = ctypes.c_double(0.0)
+        _check(lib().orc_xc(ctypes.byref(self.s), _ptr(rho), self.XC_KINDS[kind], _ptr(vxc),
+                            ctypes.byref(exc)), "xc")
+        return vxc, exc.value
 
     def ks_potential(self, vion, rho, hartree=True, xc=True):
         """self_consistency::update_hamiltonian: (vks, {external, hartree, exc, nvxc})."""

@@ -186,3 +186,68 @@ def test_dipole_reference_known_answers():
     d = s.dipole((np.sin(y) + x * z) * e)
     assert d[0] == pytest.approx(k["xz"][0], abs=1e-10)
     assert d[1] == pytest.approx(k["xz"][1], rel=1e-9) and abs(d[2]) < 1e-14
+
+
+def _xc_term_density():
+    """The density of the reference's xc_term test (xc_term.hpp:340-372):
+    max(0, cos(k.r) + 1), k = 2 pi (1/lx, 1/ly, 1/lz), normalised to 42."""
+    k = KNOWN["xc_term"]
+    lat = np.diag(k["cell"])
+    n = tuple(int(round(L / k["spacing"])) for L in k["cell"])  # real_space.hpp:183-194
+    r = grid_positions(lat, n)
+    kv = 2.0 * np.pi / np.array(k["cell"])
+    f = np.maximum(0.0, np.cos(r @ kv) + 1.0)
+    dV = np.prod(k["cell"]) / np.prod(n)
+    return n, lat, f * (k["electrons"] / (f.sum() * dV))
+
+
+def test_xc_reference_known_answers():
+    """LDA exchange and PBE correlation (PW92 uniform gas, FFT gradient and
+    divergence) reproduce the reference's xc_term known answers."""
+    k = KNOWN["xc_term"]
+    n, lat, rho = _xc_term_density()
+    assert n == (25, 34, 11)
+    assert rho.reshape(-1)[5439] == pytest.approx(k["density_5439"], abs=1e-10)
+    s = O.System(n, lat, np.zeros(n))
+    for kind in ("lda_x", "pbe_c"):
+        v, e = s.xc(rho, kind)
+        v = v.reshape(-1)
+        assert e == pytest.approx(k[kind]["exc"], abs=1e-9), kind
+        assert v[5439] == pytest.approx(k[kind]["v5439"], abs=1e-10), kind
+        assert v[4444] == pytest.approx(k[kind]["v4444"], abs=1e-10), kind
+
+
+def test_pbe_limits_and_functional_derivative():
+    """PBE closed forms: no gradient -> LDA exchange (F_x(0) = 1) and PW92
+    correlation with H = 0 (against PZ81 at high density within 2 %); a
+    steep gradient saturates F_x at 1 + kappa; the potential is the
+    functional derivative of the energy (central differences along a smooth
+    perturbation), which pins vn, vsigma and the divergence term."""
+    cell = np.diag([7.0, 6.0, 8.0])
+    n = (16, 15, 18)
+    s = O.System(n, cell, np.zeros(n))
+    const = np.full(n, 0.03)
+    vx, ex = s.xc(const, "pbe_x")
+    vl, el = s.xc(const, "lda_x")
+    assert np.allclose(vx, vl, rtol=1e-13) and ex == pytest.approx(el, rel=1e-13)
+    vc, ec = s.xc(const, "pbe_c")
+    vpz, epz = s.xc(const, "lda")
+    assert ec == pytest.approx(epz - el, rel=2e-2)  # PW92 vs PZ81 uniform gas
+    # F_x saturation: a density with a steep step has s >> 1 where n is small
+    r = grid_positions(cell, n)
+    x = r[..., 0]
+    step = 1e-6 + np.exp(-((x / 0.6) ** 2) * 4)
+    _, e_step = s.xc(step, "pbe_x")
+    _, e_lda = s.xc(step, "lda_x")
+    assert 1.0 < e_step / e_lda < 1.0 + 0.804
+    # functional derivative
+    rho = 0.02 + 0.015 * np.cos(2 * np.pi * x / 7.0) * np.sin(2 * np.pi * r[..., 1] / 6.0) ** 2
+    dlt = np.cos(2 * np.pi * r[..., 2] / 8.0) * np.exp(-(r * r).sum(-1) / 4.0)
+    dV = np.linalg.det(cell) / np.prod(n)
+    h = 1e-5
+    for kind in ("pbe_x", "pbe_c", "pbe", "lda"):
+        v, _ = s.xc(rho, kind)
+        ep = s.xc(rho + h * dlt, kind)[1]
+        em = s.xc(rho - h * dlt, kind)[1]
+        fd = (ep - em) / (2 * h)
+        assert fd == pytest.approx((v * dlt).sum() * dV, rel=1e-7), kind
