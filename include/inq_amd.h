@@ -198,9 +198,18 @@ int inqamd_poisson(inqamd_hamiltonian_t h, const double *rho, double *vh);
  * vxc[N], exc = int e(n) n, nvxc = int rho vxc (host scalars).  Synchronises. */
 int inqamd_lda_xc(inqamd_hamiltonian_t h, const double *rho, double *vxc, double *exc,
                   double *nvxc);
+/* hamiltonian::xc_term::evaluate_functional (xc_term.hpp:254-300) for one
+ * functional of one spin component: kind 1 = LDA_X, 2 = LDA_X + LDA_C_PZ,
+ * 3 = GGA_X_PBE, 4 = GGA_C_PBE, 5 = both PBE parts (the default theory,
+ * theory.hpp:40-45).  GGAs take the density gradient and the divergence of
+ * -2 vsigma grad rho spectrally (gradient.hpp:22-45).  vxc (device, N);
+ * exc = int e rho, nvxc = int rho vxc (host).  Synchronises. */
+int inqamd_xc(inqamd_hamiltonian_t h, const double *rho, int kind, double *vxc, double *exc,
+              double *nvxc);
 /* self_consistency::update_hamiltonian (self_consistency.hpp:88-162) for
  * static ions: the handle's scalar potential <- vion + V_H[rho] (hartree) +
- * V_xc[rho] (xc); energies[4] (host) = {external, hartree, exc, nvxc}.
+ * V_xc[rho] (xc: 0 none, 1 LDA, 2 PBE); energies[4] (host) = {external,
+ * hartree, exc, nvxc}.
  * Synchronises. */
 int inqamd_ks_potential(inqamd_hamiltonian_t h, const double *vion, const double *rho,
                         int hartree, int xc, double energies[4]);
@@ -208,8 +217,9 @@ int inqamd_ks_potential(inqamd_hamiltonian_t h, const double *vion, const double
  * 2-for-1 Taylor step with H(t), density, H(t+dt) from it, then up to 5
  * half steps from the saved half step until int |drho| < 5e-5.  The
  * handle's scalar potential must be V_KS of H(t) on entry; it holds V_KS of
- * H(t+dt) on return.  occ is a HOST array [nst]; *nscf receives the SCF
- * passes.  Allocates an [N][max_bands] half-step array on first use.
+ * H(t+dt) on return.  occ is a HOST array [nst]; xc as for
+ * inqamd_ks_potential; *nscf receives the SCF passes.  Allocates an
+ * [N][max_bands] half-step array on first use.
  * Synchronises. */
 int inqamd_etrs_scf_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov[3],
                          const double A_tdt_cov[3], int order, void *psi, int nst, int ld,

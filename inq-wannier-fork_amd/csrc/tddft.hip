@@ -33,6 +33,7 @@ struct td_state {
   double *red = nullptr;       // [TD_RED_BLOCKS][4] partial sums + [4] totals
   cplx *cf = nullptr;          // [N] complex Poisson work
   cplx *save = nullptr;        // [N][max_bands] the ETRS half step
+  cplx *g3 = nullptr;          // [N][3] density gradient / GGA flux (spectral)
 };
 
 td_state &state(handle &h) {
@@ -58,6 +59,11 @@ void ensure_fields(handle &h) {
   t.occ = (double *)dalloc((size_t)h.max_bands * sizeof(double));
   t.red = (double *)dalloc((size_t)(TD_RED_BLOCKS + 1) * 4 * sizeof(double));
   t.cf = (cplx *)dalloc((size_t)h.npoint * sizeof(cplx));
+}
+
+void ensure_gga(handle &h) {
+  td_state &t = state(h);
+  if (!t.g3) t.g3 = (cplx *)dalloc((size_t)h.npoint * 3 * sizeof(cplx));
 }
 
 void ensure_save(handle &h) {
@@ -157,6 +163,97 @@ __device__ void lda_point(double n, double &e, double &v) {
   v = vx + vc;
 }
 
+// PBE (Perdew, Burke, Ernzerhof 1996), one spin component: energy per
+// particle e, vn = d(n e)/dn and vs = d(n e)/dsigma (sigma = |grad n|^2);
+// the uniform-gas correlation is Perdew-Wang 1992 with libxc's LDA_C_PW_MOD
+// parameters, as GGA_C_PBE uses it.
+__device__ void pbe_exchange(double n, double sigma, double &e, double &vn, double &vs) {
+  const double kappa = 0.8040, mu = 0.2195149727645171;
+  const double ax = -0.75 * cbrt(3.0 / M_PI);
+  const double n13 = cbrt(n);
+  const double n83 = n13 * n13 * n * n;                                // n^(8/3)
+  const double c = 4.0 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+  const double x = sigma / (c * n83);                                  // s^2
+  const double den = 1.0 + mu * x / kappa;
+  const double F = 1.0 + kappa - kappa / den, dF = mu / (den * den);
+  e = ax * n13 * F;
+  vn = ax * n13 * ((4.0 / 3.0) * F - (8.0 / 3.0) * x * dF);
+  vs = ax * n13 * n * dF / (c * n83);
+}
+
+__device__ void pbe_correlation(double n, double sigma, double &e, double &vn, double &vs) {
+  const double beta = 0.06672455060314922, gamma = (1.0 - M_LN2) / (M_PI * M_PI);
+  const double A0 = 0.0310907, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382,
+               b4 = 0.49294;
+  const double rs = cbrt(3.0 / (4.0 * M_PI * n)), sq = sqrt(rs);
+  const double q0 = -2.0 * A0 * (1.0 + a1 * rs);
+  const double q1 = 2.0 * A0 * (b1 * sq + b2 * rs + b3 * rs * sq + b4 * rs * rs);
+  const double q1p = A0 * (b1 / sq + 2.0 * b2 + 3.0 * b3 * sq + 4.0 * b4 * rs);
+  const double lg = log(1.0 + 1.0 / q1);
+  const double ec = q0 * lg;
+  const double dec = (-2.0 * A0 * a1 * lg - q0 * q1p / (q1 * q1 + q1)) * (-rs / (3.0 * n));
+  const double kf = cbrt(3.0 * M_PI * M_PI * n);
+  const double ty = M_PI / (16.0 * kf * n * n);  // t^2 / sigma
+  const double y = sigma * ty;
+  const double ex = exp(-ec / gamma);
+  const double A = (beta / gamma) / (ex - 1.0);
+  const double dA = (beta / (gamma * gamma)) * ex / ((ex - 1.0) * (ex - 1.0));
+  const double P = 1.0 + A * y, D = 1.0 + A * y + A * A * y * y;
+  const double N = (beta / gamma) * y * P / D;
+  const double H = gamma * log(1.0 + N);
+  const double dNy = (beta / gamma) * ((P + A * y) * D - y * P * (A + 2.0 * A * A * y)) / (D * D);
+  const double dNA = (beta / gamma) * y * (y * D - P * (y + 2.0 * A * y * y)) / (D * D);
+  const double g = gamma / (1.0 + N);
+  e = ec + H;
+  vn = e + n * (dec + g * dNA * dA * dec - g * dNy * (7.0 / 3.0) * y / n);
+  vs = n * g * dNy * ty;
+}
+
+// i G_d f(G) / N into g3[p][d] (operations::gradient in Fourier space)
+__global__ __launch_bounds__(256) void grad_spec_kernel(const cplx *__restrict__ f, int n0, int n1,
+                                                        int n2, double b0, double b1, double b2,
+                                                        double b3, double b4, double b5,
+                                                        double b6, double b7, double b8,
+                                                        double inv_n, cplx *__restrict__ g3) {
+  const long N = (long)n0 * n1 * n2;
+  for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += (long)gridDim.x * 256) {
+    const int iz = (int)(p % n2);
+    const long t = p / n2;
+    const int iy = (int)(t % n1), ix = (int)(t / n1);
+    const int m0 = ix >= (n0 + 1) / 2 ? ix - n0 : ix, m1 = iy >= (n1 + 1) / 2 ? iy - n1 : iy,
+              m2 = iz >= (n2 + 1) / 2 ? iz - n2 : iz;
+    const double g[3] = {m0 * b0 + m1 * b3 + m2 * b6, m0 * b1 + m1 * b4 + m2 * b7,
+                         m0 * b2 + m1 * b5 + m2 * b8};
+    const cplx v = f[p];
+    for (int d = 0; d < 3; d++) g3[p * 3 + d] = mk(-g[d] * v.y * inv_n, g[d] * v.x * inv_n);
+  }
+}
+
+// sum_d i G_d g3[p][d] / N into f (the divergence in Fourier space)
+__global__ __launch_bounds__(256) void div_spec_kernel(const cplx *__restrict__ g3, int n0, int n1,
+                                                       int n2, double b0, double b1, double b2,
+                                                       double b3, double b4, double b5, double b6,
+                                                       double b7, double b8, double inv_n,
+                                                       cplx *__restrict__ f) {
+  const long N = (long)n0 * n1 * n2;
+  for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += (long)gridDim.x * 256) {
+    const int iz = (int)(p % n2);
+    const long t = p / n2;
+    const int iy = (int)(t % n1), ix = (int)(t / n1);
+    const int m0 = ix >= (n0 + 1) / 2 ? ix - n0 : ix, m1 = iy >= (n1 + 1) / 2 ? iy - n1 : iy,
+              m2 = iz >= (n2 + 1) / 2 ? iz - n2 : iz;
+    const double g[3] = {m0 * b0 + m1 * b3 + m2 * b6, m0 * b1 + m1 * b4 + m2 * b7,
+                         m0 * b2 + m1 * b5 + m2 * b8};
+    double re = 0.0, im = 0.0;
+    for (int d = 0; d < 3; d++) {
+      const cplx v = g3[p * 3 + d];
+      re -= g[d] * v.y;
+      im += g[d] * v.x;
+    }
+    f[p] = mk(re * inv_n, im * inv_n);
+  }
+}
+
 // Block partial sums of up to 4 values per point into red[block][4].
 template <class F>
 __device__ __forceinline__ void block_sums4(long n, double *red, F term) {
@@ -194,6 +291,64 @@ __global__ __launch_bounds__(256) void lda_kernel(const double *__restrict__ rho
     const double n = r < 0.0 ? 0.0 : r;
     double e, v;
     lda_point(n, e, v);
+    vxc[p] = v;
+    a[0] += e * n;
+    a[1] += r * v;
+  });
+}
+
+// xc_term.hpp:267-292 for a GGA: n = max(rho, 0), sigma from the real part
+// of the gradient; vxc = vn, the flux -2 vs grad n back into g3; sums of
+// e n.  which: 1 = exchange, 2 = correlation, 3 = both.
+__global__ __launch_bounds__(256) void gga_point_kernel(const double *__restrict__ rho,
+                                                        long npoint, int which,
+                                                        double *__restrict__ vxc,
+                                                        cplx *__restrict__ g3, double *red) {
+  block_sums4(npoint, red, [&](long p, double *a) {
+    const double r = rho[p];
+    const double n = r < 0.0 ? 0.0 : r;
+    const double gx = g3[p * 3].x, gy = g3[p * 3 + 1].x, gz = g3[p * 3 + 2].x;
+    const double sigma = gx * gx + gy * gy + gz * gz;
+    double e = 0.0, vn = 0.0, vs = 0.0;
+    if (n >= 1e-15) {
+      double e1, v1, s1;
+      if (which & 1) {
+        pbe_exchange(n, sigma, e1, v1, s1);
+        e += e1, vn += v1, vs += s1;
+      }
+      if (which & 2) {
+        pbe_correlation(n, sigma, e1, v1, s1);
+        e += e1, vn += v1, vs += s1;
+      }
+    }
+    vxc[p] = vn;
+    g3[p * 3] = mk(-2.0 * vs * gx, 0.0);
+    g3[p * 3 + 1] = mk(-2.0 * vs * gy, 0.0);
+    g3[p * 3 + 2] = mk(-2.0 * vs * gz, 0.0);
+    a[0] += e * n;
+  });
+}
+
+// vxc += Re f (the divergence term); sums of rho vxc
+__global__ __launch_bounds__(256) void gga_finish_kernel(const cplx *__restrict__ f,
+                                                         const double *__restrict__ rho,
+                                                         long npoint, double *__restrict__ vxc,
+                                                         double *red) {
+  block_sums4(npoint, red, [&](long p, double *a) {
+    const double v = vxc[p] + f[p].x;
+    vxc[p] = v;
+    a[0] += rho[p] * v;
+  });
+}
+
+// LDA exchange alone (LDA_X): sums of e n and rho v
+__global__ __launch_bounds__(256) void lda_x_kernel(const double *__restrict__ rho, long npoint,
+                                                    double *__restrict__ vxc, double *red) {
+  block_sums4(npoint, red, [&](long p, double *a) {
+    const double r = rho[p];
+    const double n = r < 0.0 ? 0.0 : r;
+    const double n13 = cbrt(n), cx = cbrt(3.0 / M_PI);
+    const double e = n < 1e-15 ? 0.0 : -0.75 * cx * n13, v = n < 1e-15 ? 0.0 : -cx * n13;
     vxc[p] = v;
     a[0] += e * n;
     a[1] += r * v;
@@ -310,16 +465,66 @@ void lda_xc(handle &h, const double *rho, double *vxc, double *exc, double *nvxc
   *nvxc = dV * s[1];
 }
 
+// hamiltonian::xc_term::evaluate_functional for a GGA (xc_term.hpp:267-300):
+// grad rho = Re to_real(i G to_fourier rho), the point forms, then
+// vxc = vn + div(-2 vs grad rho) by the same spectral derivative
+void gga_xc(handle &h, const double *rho, int which, double *vxc, double *exc, double *nvxc) {
+  ensure_fields(h);
+  ensure_gga(h);
+  td_state &t = state(h);
+  const double dV = h.volume / (double)h.npoint, inv_n = 1.0 / (double)h.npoint;
+  const double *b = h.rlat;
+  hipLaunchKernelGGL(real_to_complex, dim3(grid1(h.npoint)), dim3(256), 0, h.stream, rho,
+                     h.npoint, t.cf);
+  check_launch();
+  fft3d(h, t.cf, 1, t.cf, 1, 1, -1, 1.0);
+  hipLaunchKernelGGL(grad_spec_kernel, dim3(grid1(h.npoint)), dim3(256), 0, h.stream,
+                     (const cplx *)t.cf, h.n[0], h.n[1], h.n[2], b[0], b[1], b[2], b[3], b[4],
+                     b[5], b[6], b[7], b[8], inv_n, t.g3);
+  check_launch();
+  fft3d(h, t.g3, 3, t.g3, 3, 3, +1, 1.0);
+  double s[4];
+  reduce4(h, h.npoint, s, gga_point_kernel, rho, h.npoint, which, vxc, t.g3);
+  *exc = dV * s[0];
+  fft3d(h, t.g3, 3, t.g3, 3, 3, -1, 1.0);
+  hipLaunchKernelGGL(div_spec_kernel, dim3(grid1(h.npoint)), dim3(256), 0, h.stream,
+                     (const cplx *)t.g3, h.n[0], h.n[1], h.n[2], b[0], b[1], b[2], b[3], b[4],
+                     b[5], b[6], b[7], b[8], inv_n, t.cf);
+  check_launch();
+  fft3d(h, t.cf, 1, t.cf, 1, 1, +1, 1.0);
+  reduce4(h, h.npoint, s, gga_finish_kernel, (const cplx *)t.cf, rho, h.npoint, vxc);
+  *nvxc = dV * s[0];
+}
+
+// One functional: 1 = LDA_X, 2 = LDA_X + LDA_C_PZ, 3 = GGA_X_PBE,
+// 4 = GGA_C_PBE, 5 = both PBE parts
+void xc_functional(handle &h, const double *rho, int kind, double *vxc, double *exc,
+                   double *nvxc) {
+  if (kind == 2) {
+    lda_xc(h, rho, vxc, exc, nvxc);
+  } else if (kind == 1) {
+    ensure_fields(h);
+    double s[4];
+    reduce4(h, h.npoint, s, lda_x_kernel, rho, h.npoint, vxc);
+    const double dV = h.volume / (double)h.npoint;
+    *exc = dV * s[0];
+    *nvxc = dV * s[1];
+  } else {
+    gga_xc(h, rho, kind - 2, vxc, exc, nvxc);
+  }
+}
+
 // update_hamiltonian: the handle's V <- vion + V_H[rho] + V_xc[rho];
-// energies {external, hartree, exc, nvxc}
-void ks_potential(handle &h, const double *vion, const double *rho, bool hartree, bool xc,
+// energies {external, hartree, exc, nvxc}; xc: 0 none, 1 LDA
+// (options::theory::lda), 2 PBE (the theory's default)
+void ks_potential(handle &h, const double *vion, const double *rho, bool hartree, int xc,
                   double energies[4]) {
   ensure_fields(h);
   td_state &t = state(h);
   const double dV = h.volume / (double)h.npoint;
   energies[2] = energies[3] = 0.0;
   if (hartree) poisson(h, rho, t.vh);
-  if (xc) lda_xc(h, rho, t.vxc, &energies[2], &energies[3]);
+  if (xc) xc_functional(h, rho, xc == 2 ? 5 : 2, t.vxc, &energies[2], &energies[3]);
   double s[4];
   reduce4(h, h.npoint, s, ks_kernel, vion, rho, hartree ? (const double *)t.vh : nullptr,
           xc ? (const double *)t.vxc : nullptr, h.npoint, h.V);
@@ -337,7 +542,7 @@ void upload_occ(handle &h, const double *occ_host, int nst) {
 // static ions, no induced vector potential.  The handle's V is V_KS of
 // H(t) on entry and of H(t+dt) on return.
 int etrs_scf(handle &h, double dt, const double A_t[3], const double A_tdt[3], int order,
-             cplx *psi, long ld, int nst, const double *vion, bool hartree, bool xc) {
+             cplx *psi, long ld, int nst, const double *vion, bool hartree, int xc) {
   ensure_fields(h);
   ensure_save(h);
   td_state &t = state(h);
@@ -381,7 +586,8 @@ void td_release(handle &h) {
   if (!h.tddft) return;
   td_state *t = static_cast<td_state *>(h.tddft);
   for (void *p : {(void *)t->rho, (void *)t->rho_old, (void *)t->vh, (void *)t->vxc,
-                  (void *)t->occ, (void *)t->red, (void *)t->cf, (void *)t->save})
+                  (void *)t->occ, (void *)t->red, (void *)t->cf, (void *)t->save,
+                  (void *)t->g3})
     if (p) (void)hipFree(p);
   delete t;
   h.tddft = nullptr;
@@ -430,12 +636,24 @@ int inqamd_lda_xc(inqamd_hamiltonian_t h, const double *rho, double *vxc, double
   });
 }
 
+int inqamd_xc(inqamd_hamiltonian_t h, const double *rho, int kind, double *vxc, double *exc,
+              double *nvxc) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    INQAMD_REQUIRE(rho && vxc && exc && nvxc, "rho, vxc, exc and nvxc are required");
+    INQAMD_REQUIRE(kind >= 1 && kind <= 5, "kind must be 1 (LDA_X), 2 (LDA), 3 (PBE_X), 4 "
+                                           "(PBE_C) or 5 (PBE)");
+    xc_functional(*h, rho, kind, vxc, exc, nvxc);
+  });
+}
+
 int inqamd_ks_potential(inqamd_hamiltonian_t h, const double *vion, const double *rho,
                         int hartree, int xc, double energies[4]) {
   if (!h) return INQAMD_ERR_ARGUMENT;
   return guarded(h, [&] {
     INQAMD_REQUIRE(vion && rho && energies, "vion, rho and energies are required");
-    ks_potential(*h, vion, rho, hartree != 0, xc != 0, energies);
+    INQAMD_REQUIRE(xc >= 0 && xc <= 2, "xc must be 0 (none), 1 (LDA) or 2 (PBE)");
+    ks_potential(*h, vion, rho, hartree != 0, xc, energies);
   });
 }
 
@@ -450,8 +668,9 @@ int inqamd_etrs_scf_step(inqamd_hamiltonian_t h, double dt, const double A_t_cov
     INQAMD_REQUIRE(vion && occ, "vion and occ are required");
     INQAMD_REQUIRE(order >= 1, "order must be >= 1");
     upload_occ(*h, occ, nst);
+    INQAMD_REQUIRE(xc >= 0 && xc <= 2, "xc must be 0 (none), 1 (LDA) or 2 (PBE)");
     const int n = etrs_scf(*h, dt, A_t_cov, A_tdt_cov, order, (cplx *)psi, ld, nst, vion,
-                           hartree != 0, xc != 0);
+                           hartree != 0, xc);
     if (nscf) *nscf = n;
   });
 }

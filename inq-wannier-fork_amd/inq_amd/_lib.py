@@ -107,6 +107,7 @@ SIGNATURES = [
     ("inqamd_poisson", _i, [_vp, _vp, _vp]),
     ("inqamd_dipole", _i, [_vp, _vp, _c_dbl_p]),
     ("inqamd_lda_xc", _i, [_vp, _vp, _vp, _c_dbl_p, _c_dbl_p]),
+    ("inqamd_xc", _i, [_vp, _vp, ctypes.c_int, _vp, _c_dbl_p, _c_dbl_p]),
     ("inqamd_ks_potential", _i, [_vp, _vp, _vp, _i, _i, _c_dbl_p]),
     ("inqamd_etrs_scf_step", _i, [_vp, _d, _c_dbl_p, _c_dbl_p, _i, _vp, _i, _i, _vp, _c_dbl_p,
                                   _i, _i, ctypes.POINTER(ctypes.c_int)]),

@@ -62,12 +62,37 @@ def lda_xc(ham, rho):
     return vxc, exc.value, nvxc.value
 
 
+XC_KINDS = {"lda_x": 1, "lda": 2, "pbe_x": 3, "pbe_c": 4, "pbe": 5}
+
+
+def xc(ham, rho, kind="pbe"):
+    """One exchange-correlation functional (lda_x, lda, pbe_x, pbe_c, pbe):
+    (vxc, exc, nvxc)."""
+    vxc = torch.empty_like(rho)
+    exc, nvxc = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    ham.handle.call("inqamd_xc", _field(rho, ham), XC_KINDS[kind], _field(vxc, ham),
+                    ctypes.byref(exc), ctypes.byref(nvxc))
+    return vxc, exc.value, nvxc.value
+
+
+def _xc_code(xc):
+    """ks_potential's theory: False/None -> 0, True/'lda' -> 1, 'pbe' -> 2."""
+    if xc is None or xc is False:
+        return 0
+    if xc is True or xc == "lda":
+        return 1
+    if xc == "pbe":
+        return 2
+    raise ValueError("xc must be False, True/'lda' or 'pbe'")
+
+
 def ks_potential(ham, vion, rho, hartree=True, xc=True):
     """update_hamiltonian: the Hamiltonian's scalar potential becomes
-    vion + V_H[rho] + V_xc[rho]; returns {external, hartree, exc, nvxc}."""
+    vion + V_H[rho] + V_xc[rho] (xc: False, True/'lda' or 'pbe'); returns
+    {external, hartree, exc, nvxc}."""
     en = (ctypes.c_double * 4)()
     ham.handle.call("inqamd_ks_potential", _field(vion, ham), _field(rho, ham), int(hartree),
-                    int(xc), en)
+                    _xc_code(xc), en)
     return dict(zip(("external", "hartree", "exc", "nvxc"), list(en)))
 
 
@@ -86,6 +111,6 @@ def etrs(ham, psi, vion, time, dt, occ=2.0, pert=None, order=4, hartree=True, xc
     o, op = _occ(occ, nst)
     nscf = ctypes.c_int(0)
     ham.handle.call("inqamd_etrs_scf_step", ctypes.c_double(dt), _lib.dvec(A_t), _lib.dvec(A_tdt),
-                    int(order), p, nst, ld, _field(vion, ham), op, int(hartree), int(xc),
+                    int(order), p, nst, ld, _field(vion, ham), op, int(hartree), _xc_code(xc),
                     ctypes.byref(nscf))
     return nscf.value

@@ -899,8 +899,16 @@ static void ks_potential(const int n[3], const cell &c, const double *vion, cons
   }
   energies[2] = energies[3] = 0.0;
   if (xc) {
+    // xc 1: LDA (options::theory::lda); 2: PBE (the default theory)
     std::vector<double> vx((size_t)N);
-    lda_xc(N, dV, rho, vx.data(), &energies[2], &energies[3]);
+    if (xc == 2) {
+      gga_xc(n, c, rho, 3, vx.data(), &energies[2]);
+      double s = 0.0;
+      for (long p = 0; p < N; p++) s += rho[p] * vx[p];
+      energies[3] = dV * s;
+    } else {
+      lda_xc(N, dV, rho, vx.data(), &energies[2], &energies[3]);
+    }
     for (long p = 0; p < N; p++) vks[p] += vx[p];
   }
 }

@@ -336,7 +336,7 @@ class System:
         vks = np.zeros_like(rho)
         en = np.zeros(4)
         _check(lib().orc_ks_potential(ctypes.byref(self.s), _ptr(vion), _ptr(rho), int(hartree),
-                                      int(xc), _ptr(vks), _ptr(en)), "ks_potential")
+                                      _xc_code(xc), _ptr(vks), _ptr(en)), "ks_potential")
         return vks, dict(zip(("external", "hartree", "exc", "nvxc"), en))
 
     def etrs_scf(self, psi, vion, occ, dt, kA_t=(0.0, 0.0, 0.0), kA_tdt=(0.0, 0.0, 0.0), order=4,
@@ -350,7 +350,7 @@ class System:
         vks = np.zeros(self.n)
         nscf = ctypes.c_int(0)
         _check(lib().orc_etrs_scf(ctypes.byref(self.s), _ptr(vion), _ptr(occ), _dvec(kA_t),
-                                  _dvec(kA_tdt), float(dt), int(order), int(hartree), int(xc), nst,
+                                  _dvec(kA_tdt), float(dt), int(order), int(hartree), _xc_code(xc), nst,
                                   nst, _ptr(out), _ptr(vks), ctypes.byref(nscf)), "etrs_scf")
         return out, vks, nscf.value
 
@@ -467,6 +467,17 @@ def steepest_descent_dense(matrix, phi, dV, niter):
     _check(lib().orc_steepest_descent_dense(npoint, nvec, _ptr(M), float(dV), int(niter),
                                             _ptr(out), _ptr(ev)), "steepest_descent_dense")
     return out, ev
+
+
+def _xc_code(xc):
+    """The theory of ks_potential/etrs_scf: False -> 0, True/'lda' -> 1, 'pbe' -> 2."""
+    if xc is None or xc is False:
+        return 0
+    if xc is True or xc == "lda":
+        return 1
+    if xc == "pbe":
+        return 2
+    raise ValueError("xc must be False, True/'lda' or 'pbe'")
 
 
 def density(psi, occ):

@@ -159,3 +159,68 @@ def test_dipole_reference_known_answers_gpu(device):
     assert np.allclose(tddft.dipole(ham, rho), O.System(n, lat, np.zeros(n)).dipole(
         (y + 2.0 * z) * e), rtol=1e-13, atol=1e-14)
     ham.close()
+
+
+def test_xc_reference_known_answers_gpu(device):
+    """LDA_X and GGA_C_PBE on the reference's xc_term density (a 25 x 34 x 11
+    orthorhombic grid: spectral gradient and divergence through the 17- and
+    11-point axes) reproduce its known answers; every kind matches the
+    oracle."""
+    k = KNOWN["xc_term"]
+    lat = np.diag(k["cell"])
+    n = tuple(int(round(L / k["spacing"])) for L in k["cell"])
+    from helpers import grid_positions
+    r = grid_positions(lat, n)
+    f = np.maximum(0.0, np.cos(r @ (2.0 * np.pi / np.array(k["cell"]))) + 1.0)
+    rho = f * (k["electrons"] / (f.sum() * np.prod(k["cell"]) / np.prod(n)))
+    say("xc known answers", n)
+    s = systems.make_system(n, lat, np.zeros((0, 3)), seed=0, system_id=93, with_projectors=False)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=1)
+    osys = O.System(n, lat, np.zeros(n))
+    d = to_dev(rho, device, np.float64)
+    for kind in ("lda_x", "lda", "pbe_x", "pbe_c", "pbe"):
+        v, exc, nvxc = tddft.xc(ham, d, kind)
+        v = v.cpu().numpy()
+        vr, er = osys.xc(rho, kind)
+        say(kind, "exc %.12f oracle %.12f" % (exc, er))
+        assert exc == pytest.approx(er, rel=TOL), kind
+        assert np.abs(v - vr).max() < TOL * np.abs(vr).max(), kind
+        assert nvxc == pytest.approx((rho * vr).sum() * np.prod(k["cell"]) / np.prod(n), rel=TOL)
+        if kind in k:
+            vf = v.reshape(-1)
+            assert exc == pytest.approx(k[kind]["exc"], abs=1e-9), kind
+            assert vf[5439] == pytest.approx(k[kind]["v5439"], abs=1e-10), kind
+            assert vf[4444] == pytest.approx(k[kind]["v4444"], abs=1e-10), kind
+    ham.close()
+
+
+def test_pbe_ks_potential_and_etrs_match_oracle(device):
+    """The PBE theory through the Kohn-Sham update and two self-consistent
+    ETRS steps (three-sweep grid) against the oracle."""
+    s = systems.diamond_system(1)
+    nst = 8
+    psi = systems.random_orbitals(s, nst, ecut=6.0)
+    occ = np.full(nst, 2.0)
+    say("pbe ks potential + etrs", s.n)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    d = to_dev(psi, device)
+    vion = to_dev(s.vloc, device, np.float64)
+    en = tddft.ks_potential(ham, vion, tddft.density(ham, d, occ), xc="pbe")
+    v, ren = O.System(s.n, s.lattice, s.vloc, s.tables).ks_potential(s.vloc, O.density(psi, occ),
+                                                                     xc="pbe")
+    for key in ("external", "hartree", "exc", "nvxc"):
+        assert en[key] == pytest.approx(ren[key], rel=TOL), key
+    coef = systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=8)
+    pert = inq_amd.real_time.fourier_series_vector_potential(coef, 0.1)
+    dt, x, lat = 0.02, psi, s.lattice
+    for it in range(2):
+        t = it * dt
+        n_gpu = tddft.etrs(ham, d, vion, t, dt, occ, pert, xc="pbe")
+        A0 = lat @ pert.uniform_vector_potential(t)
+        A1 = lat @ pert.uniform_vector_potential(t + dt)
+        x, v, n_ref = O.System(s.n, s.lattice, v, s.tables).etrs_scf(x, s.vloc, occ, dt, A0, A1,
+                                                                     xc="pbe")
+        say("step", it, "scf passes gpu", n_gpu, "oracle", n_ref)
+        assert n_gpu == n_ref
+    assert relerr(d.cpu().numpy(), x) < 1e-11
+    ham.close()
