@@ -466,9 +466,10 @@ void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *ps
 }
 
 // out = T(z, H) in (exponential_in_place on a copy of in, without the
-// copy: the first H application writes in + z H in to out)
-void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, cplx *out,
-                      long ld, int nst, int slot) {
+// copy: the first H application writes in + z H in to out); in and out have
+// their own leading dimensions
+void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, long ild,
+                      cplx *out, long old, int nst, int slot) {
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
   std::complex<double> coeff = 1.0;
@@ -476,12 +477,12 @@ void exponential_from(handle &h, std::complex<double> z, int order, const cplx *
     coeff *= z / double(iter);
     taylor_acc acc;
     acc.p1 = out;
-    acc.ld1 = ld;
+    acc.ld1 = old;
     acc.c1re = coeff.real();
     acc.c1im = coeff.imag();
     acc.init1 = iter == 1;
     if (iter == 1) {
-      hamiltonian_apply(h, in, ld, hn, tl, nst, slot, &acc);
+      hamiltonian_apply(h, in, ild, hn, tl, nst, slot, &acc);
     } else {
       hamiltonian_apply(h, hn, tl, hn2, tl, nst, slot, &acc);
       std::swap(hn, hn2);
@@ -489,10 +490,11 @@ void exponential_from(handle &h, std::complex<double> z, int order, const cplx *
   }
 }
 
-// exponential_2_for_1 (exponential.hpp:50-69); expphi starts as phi + z1 H phi
-// from the first H application (no copy of phi)
+// exponential_2_for_1 (exponential.hpp:50-69); expphi (leading dimension
+// eld) starts as phi + z1 H phi from the first H application (no copy of phi)
 void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2,
-                                int order, cplx *phi, cplx *expphi, long ld, int nst, int slot) {
+                                int order, cplx *phi, cplx *expphi, long ld, long eld, int nst,
+                                int slot) {
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
   std::complex<double> c1 = 1.0, c2 = 1.0;
@@ -501,7 +503,7 @@ void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double
     c2 *= z2 / double(iter);
     taylor_acc acc;
     acc.p1 = expphi;
-    acc.ld1 = ld;
+    acc.ld1 = eld;
     acc.c1re = c1.real();
     acc.c1im = c1.imag();
     acc.init1 = iter == 1;
@@ -928,7 +930,7 @@ int inqamd_exponential_2_for_1(inqamd_hamiltonian_t h, double z1re, double z1im,
     INQAMD_REQUIRE(order >= 1, "order must be >= 1");
     INQAMD_REQUIRE(phi != expphi, "phi and expphi must not alias");
     exponential_2_for_1(*h, std::complex<double>(z1re, z1im), std::complex<double>(z2re, z2im),
-                        order, (cplx *)phi, (cplx *)expphi, ld, nst, KA_CURRENT);
+                        order, (cplx *)phi, (cplx *)expphi, ld, ld, nst, KA_CURRENT);
   });
 }
 

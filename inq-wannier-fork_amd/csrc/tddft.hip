@@ -19,9 +19,9 @@
 namespace inqamd {
 
 void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2, int order,
-                         cplx *phi, cplx *expphi, long ld, int nst, int slot);
-void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, cplx *out,
-                      long ld, int nst, int slot);
+                         cplx *phi, cplx *expphi, long ld, long eld, int nst, int slot);
+void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, long ild,
+                      cplx *out, long old, int nst, int slot);
 
 namespace {
 
@@ -557,8 +557,9 @@ int etrs_scf(handle &h, double dt, const double A_t[3], const double A_tdt[3], i
   set_slot(h, KA_T, kt);
   set_slot(h, KA_TDT, ktdt);
   // :33-38 save = T(i dt/2, H(t)) psi, psi <- T(i dt, H(t)) psi
+  // (the saved half step is compact: leading dimension nst)
   exponential_2_for_1(h, std::complex<double>(0.0, dt / 2.0), std::complex<double>(0.0, dt), order,
-                      psi, t.save, ld, nst, KA_T);
+                      psi, t.save, ld, nst, nst, KA_T);
   // :44 the density of the full step, :53 H(t + dt)
   density(h, psi, ld, nst, t.occ, t.rho);
   double en[4];
@@ -569,7 +570,8 @@ int etrs_scf(handle &h, double dt, const double A_t[3], const double A_tdt[3], i
   // (each pass psi = T(i dt/2, H(t+dt)) save, written from save directly)
   int iscf = 0;
   for (; iscf < nscf; iscf++) {
-    exponential_from(h, std::complex<double>(0.0, dt / 2.0), order, t.save, psi, ld, nst, KA_TDT);
+    exponential_from(h, std::complex<double>(0.0, dt / 2.0), order, t.save, nst, psi, ld, nst,
+                     KA_TDT);
     std::swap(t.rho, t.rho_old);
     density(h, psi, ld, nst, t.occ, t.rho);
     double s[4];

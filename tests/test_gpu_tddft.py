@@ -224,3 +224,38 @@ def test_pbe_ks_potential_and_etrs_match_oracle(device):
         assert n_gpu == n_ref
     assert relerr(d.cpu().numpy(), x) < 1e-11
     ham.close()
+
+
+def test_etrs_scf_leading_dimension_beyond_max_bands(device):
+    """psi as the first nst bands of a wider buffer (ld > max_bands): the
+    saved half step is compact, so the step matches the contiguous one and
+    leaves the other bands alone."""
+    import ctypes
+    from inq_amd import _lib
+    lat = np.diag([6.5, 6.5, 6.5])
+    s = systems.make_system((12, 14, 12), lat, np.array([[0.3, 0.2, 0.1]]), seed=3, system_id=56)
+    nst, ld = 4, 8
+    psi = systems.random_orbitals(s, nst, ecut=6.0)
+    occ = np.full(nst, 2.0)
+    vion = to_dev(s.vloc, device, np.float64)
+    say("etrs scf ld", ld, "max_bands", nst)
+    ha = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    a = to_dev(psi, device)
+    tddft.ks_potential(ha, vion, tddft.density(ha, a, occ))
+    tddft.etrs(ha, a, vion, 0.0, 0.02, occ)
+    hb = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    wide = np.zeros(psi.shape[:-1] + (ld,), np.complex128)
+    wide[..., :nst] = psi
+    w = to_dev(wide, device)
+    tddft.ks_potential(hb, vion, tddft.density(hb, to_dev(psi, device), occ))
+    nscf = ctypes.c_int(0)
+    hb.handle.call("inqamd_etrs_scf_step", ctypes.c_double(0.02), _lib.dvec(np.zeros(3)),
+                   _lib.dvec(np.zeros(3)), 4, ctypes.c_void_p(w.data_ptr()), nst, ld,
+                   ctypes.c_void_p(vion.data_ptr()),
+                   np.ascontiguousarray(occ).ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, 1,
+                   ctypes.byref(nscf))
+    got = w.cpu().numpy()
+    assert relerr(got[..., :nst], a.cpu().numpy()) < 1e-14
+    assert np.abs(got[..., nst:]).max() == 0.0
+    ha.close()
+    hb.close()
