@@ -406,8 +406,9 @@ def _tddft_bytes(N, nst, sum_k, nscf):
 
 
 def run_tddft_64(args):
-    """One interacting ETRS step (Hartree + LDA, self-consistent second half)
-    at 64^3 x 256 bands, from the KS potential of the initial density."""
+    """One interacting ETRS step (Hartree + PBE, INQ's default theory, or
+    LDA with --xc lda; self-consistent second half) at 64^3 x 256 bands,
+    from the KS potential of the initial density."""
     import numpy as np
     import torch
 
@@ -423,11 +424,11 @@ def run_tddft_64(args):
     coef = inq_amd.systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=32)
     pert = real_time.fourier_series_vector_potential(coef, 0.1)
     dt = 0.015
-    tddft.ks_potential(ham, vion, tddft.density(ham, d_psi, 2.0))
+    tddft.ks_potential(ham, vion, tddft.density(ham, d_psi, 2.0), xc=args.xc)
     state = {"t": 0.0, "nscf": []}
 
     def step():
-        state["nscf"].append(tddft.etrs(ham, d_psi, vion, state["t"], dt, 2.0, pert))
+        state["nscf"].append(tddft.etrs(ham, d_psi, vion, state["t"], dt, 2.0, pert, xc=args.xc))
         state["t"] += dt
 
     ms = timed(step, args.steps, args.warmup)
@@ -444,9 +445,10 @@ def run_tddft_64(args):
         nb = 8
         sample = np.ascontiguousarray(psi[..., :nb])
         osys = O.System(s.n, s.lattice, s.vloc, s.tables)
-        v0, _ = osys.ks_potential(s.vloc, O.density(sample, 2.0))
+        v0, _ = osys.ks_potential(s.vloc, O.density(sample, 2.0), xc=args.xc)
         t0 = time.perf_counter()
-        _, _, n_ref = O.System(s.n, s.lattice, v0, s.tables).etrs_scf(sample, s.vloc, 2.0, dt)
+        _, _, n_ref = O.System(s.n, s.lattice, v0, s.tables).etrs_scf(sample, s.vloc, 2.0, dt,
+                                                                     xc=args.xc)
         dtc = time.perf_counter() - t0
         cpu = {"value": nb / dtc, "unit": "band-steps/s", "cores": O.get_threads(), "kind": "port",
                "sample": f"{nb} of {nst} bands of the same 64^3 system (their own density), one "
@@ -456,8 +458,8 @@ def run_tddft_64(args):
     ham.close()
     cfg = {"workload": "tddft_64: 64^3 grid, 256 bands complex128 (occupation 2), 2x2x2 diamond "
                        "supercell a=20.52 bohr, 64 atoms x 4 projectors; interacting ETRS step "
-                       "(Taylor-4, Hartree + LDA, self-consistent second half) under a 32-term "
-                       "A(t), dt=0.015",
+                       f"(Taylor-4, Hartree + {args.xc.upper()}, self-consistent second half) "
+                       "under a 32-term A(t), dt=0.015",
            "grid": list(s.n), "bands": nst, "atoms": s.natoms, "parallelism": "single GPU",
            "scf_passes_per_step": nscf}
     out = common_fields(args, nst * 1e3 / ms, "band-steps/s", ms, True, cfg)
@@ -479,6 +481,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--xc", choices=["pbe", "lda"], default="pbe",
+                    help="exchange-correlation of the tddft_64 workload (PBE is INQ's default)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run 3 plain device copies of this size (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against a kernel of known traffic)")
