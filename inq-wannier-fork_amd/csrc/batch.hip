@@ -71,6 +71,17 @@ struct batch_piece {
 
 constexpr int XY_MAX_NT = 384;  // plane_threads(72)
 
+// Dynamic LDS of an x-y plane of grid length L in these kernels
+static size_t batch_xy_lds(int L) {
+  switch (L) {
+#define INQAMD_BATCH_XY_LDS(LL, R0, R1, R2) \
+  case LL: return xy_kin_lds<R0, R1, R2>();
+    INQAMD_BATCH_TABLE(INQAMD_BATCH_XY_LDS)
+#undef INQAMD_BATCH_XY_LDS
+    default: return (size_t)L * (L + 1) * sizeof(cplx);
+  }
+}
+
 // Out-of-line tile bodies for the mixed-grid kernels: each (grid length,
 // tile shape) variant is register-allocated on its own instead of as one
 // merged switch.
@@ -232,7 +243,9 @@ constexpr int group_xy_threads(int L) {
 }
 
 template <int R0, int R1, int R2>
-__global__ __launch_bounds__(group_xy_threads(R0 * R1 * R2)) void group_xy_kernel(
+__global__ __launch_bounds__(group_xy_threads(R0 * R1 * R2))
+__attribute__((amdgpu_waves_per_eu(xy_kin_waves<R0, R1, R2, group_xy_threads(R0 * R1 * R2)>())))
+void group_xy_kernel(
     const batch_piece *__restrict__ pc, int np, unsigned long long *, int) {
   extern __shared__ cplx lds[];
   const int item = blockIdx.x;
@@ -521,7 +534,7 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       g.zitems += (int)zi;
       g.xyitems += (int)xi;
       g.zlds = std::max(g.zlds, (size_t)q.L * Z_COLS * sizeof(cplx));
-      g.xylds = std::max(g.xylds, (size_t)q.L * (q.L + 1) * sizeof(cplx));
+      g.xylds = std::max(g.xylds, batch_xy_lds(q.L));
       g.xynt = std::max(g.xynt, plane_threads(q.L));
       g.np++;
       pc.push_back(q);

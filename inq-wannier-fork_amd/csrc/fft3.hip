@@ -38,7 +38,8 @@ constexpr int wide_threads(int L) {
 }
 
 template <int R0, int R1, int R2, int NT>
-__global__ __launch_bounds__(NT) void xy_kin_kernel(const fft3_args a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(xy_kin_waves<R0, R1, R2, NT>())))
+void xy_kin_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
   xy_kin_plane<R0, R1, R2, NT>(a, blockIdx.x, lds);
 }
@@ -104,8 +105,8 @@ static void prepare3() {
   const size_t zl = (size_t)L * Z_COLS * sizeof(cplx), pl = (size_t)L * (L + 1) * sizeof(cplx);
   ensure_lds(z_fwd_kernel<R0, R1, R2, 8>, zl);
   ensure_lds(z_fwd_kernel<R0, R1, R2, 16>, zl);
-  ensure_lds(xy_kin_kernel<R0, R1, R2, plane_threads(L)>, pl);
-  ensure_lds(xy_kin_kernel<R0, R1, R2, wide_threads(L)>, pl);
+  ensure_lds(xy_kin_kernel<R0, R1, R2, plane_threads(L)>, xy_kin_lds<R0, R1, R2>());
+  ensure_lds(xy_kin_kernel<R0, R1, R2, wide_threads(L)>, xy_kin_lds<R0, R1, R2>());
   ensure_lds(z_pot_kernel<R0, R1, R2, 8>, zl);
   ensure_lds(z_kin_kernel<R0, R1, R2, 8>, zl);
   ensure_lds(xy_fwd_kernel<R0, R1, R2, wide_threads(L)>, pl);
@@ -120,7 +121,7 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
   constexpr int L = R0 * R1 * R2;
   const long ztiles = z_tiles<TY>(L, a.nb);
   const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
-  const size_t plds = (size_t)L * (L + 1) * sizeof(cplx);
+  const size_t plds = xy_kin_lds<R0, R1, R2>();
   if (sweep < 0 || sweep == 0) {
     hipLaunchKernelGGL((z_fwd_kernel<R0, R1, R2, TY>), dim3((unsigned)ztiles), dim3(z_threads(L)), zlds,
                        stream, a);

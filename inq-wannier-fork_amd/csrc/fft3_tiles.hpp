@@ -106,6 +106,32 @@ struct xy_plan {
   static_assert(P::LEN == R0 * R1 * R2, "x-y radices must multiply to the grid length");
 };
 
+// Two-stage x-y planes whose complex LDS array (L x (L+1) x 16 B) leaves room
+// for one workgroup per CU exchange real and imaginary parts in turn through
+// half of it (xchg_split), so that two share a CU: at 72 (80 registers at
+// 768 threads).  At 96 two workgroups' registers would spill (64 or 96 per
+// thread against 110-128), so it keeps the complex exchange.
+template <int R0, int R1, int R2>
+constexpr bool xy_split() {
+  constexpr int L = R0 * R1 * R2;
+  return xy_plan<R0, R1, R2>::P::NSTAGE == 2 && 2 * L * (L + 1) * 16 > 160 * 1024 && L < 96;
+}
+
+// Dynamic LDS of an x-y kinetic plane
+template <int R0, int R1, int R2>
+constexpr size_t xy_kin_lds() {
+  constexpr size_t L = R0 * R1 * R2;
+  return L * (L + 1) * (xy_split<R0, R1, R2>() ? sizeof(double) : sizeof(cplx));
+}
+
+// Occupancy floor (amdgpu_waves_per_eu) of an NT-thread x-y kinetic kernel:
+// planes with split exchanges ask for the registers of two workgroups per
+// CU, the others for one (the floor the launch bound implies by itself)
+template <int R0, int R1, int R2, int NT>
+constexpr int xy_kin_waves() {
+  return ((xy_split<R0, R1, R2>() ? 2 : 1) * (NT / 64) + 3) / 4;
+}
+
 // Registers of the plane's first stage (the x-forward inputs)
 template <int R0, int R1, int R2, int NT>
 using xy_regs = stage_regs<xy_plan<R0, R1, R2>::X0, R0 * R1 * R2, R0 * R1 * R2, NT>;
@@ -124,6 +150,36 @@ __device__ __forceinline__ void xy_kc(const fft3_args &a, double kc[3]) {
 #pragma unroll
   for (int d = 0; d < 3; d++)
     kc[d] = (a.rlat[0 * 3 + d] * k0 + a.rlat[1 * 3 + d] * k1 + a.rlat[2 * 3 + d] * k2) * inv2pi;
+}
+
+// Kinetic factor 1/2 |G+k+A|^2 on the registers of the y-forward last
+// stage (radix X1, span X0), where butterfly j < X0 of column c holds
+// outputs (y = j + r X0, x = c), r < X1; they are exactly the inputs of the
+// first stage of the inverse run with the radices in reverse order (X1, X0),
+// which therefore starts from the same registers (one LDS round trip and a
+// barrier fewer).  G (optional) receives the transform before the factor.
+template <int X0, int X1, int L, int NT>
+__device__ __forceinline__ void xy_kin_regs(stage_regs<X1, L, L, NT> &r, cplx *G, const double b0[3],
+                                            const double b1[3], const double gz[3]) {
+  constexpr int NB = L / X1;  // = X0
+  constexpr int NBF = stage_regs<X1, L, L, NT>::NBF;
+#pragma unroll
+  for (int q = 0; q < NBF; q++) {
+    const int w = threadIdx.x + q * NT;
+    if ((int)threadIdx.x < NT && (NBF * NT == NB * L || w < NB * L)) {
+      const int j = w / L, c = w % L;
+#pragma unroll
+      for (int rr = 0; rr < X1; rr++) {
+        const int i = j + rr * X0;
+        if (G) G[c * L + i] = r.v[q][rr];
+        const int my = symmetric_index(i, L), mx = symmetric_index(c, L);
+        const double q0 = mx * b0[0] + my * b1[0] + gz[0];
+        const double q1 = mx * b0[1] + my * b1[1] + gz[1];
+        const double q2 = mx * b0[2] + my * b1[2] + gz[2];
+        r.v[q][rr] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), r.v[q][rr]);
+      }
+    }
+  }
 }
 
 // The plane transform from its first-stage inputs already in registers
@@ -168,20 +224,62 @@ __device__ __forceinline__ void xy_kin_plane_from(const fft3_args &a, long plane
     lds[c * PAD + i] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), v);
   };
 
+  if constexpr (xy_split<R0, R1, R2>()) {
+    // every stage from registers; the exchanges between them go through
+    // the double array one component at a time.  Slots of element (x, y):
+    // x * PAD + y, as (i = x, c = y) on the x axis, (i = y, c = x) on y.
+    double *ld = reinterpret_cast<double *>(lds);
+    auto xs = [&](int i, int c) { return i * PAD + c; };
+    auto ys = [&](int i, int c) { return c * PAD + i; };
+    pcompute<X0, 1, L, L, NT>(in, tw, -1);  // x forward
+    stage_regs<X1, L, L, NT> r1;
+    xchg_split<X0, 1, X1, L, L, NT>(in, r1, ld, xs, xs);
+    pcompute<X1, X0, L, L, NT>(r1, tw, -1);
+    stage_regs<X0, L, L, NT> r2;  // y forward
+    xchg_split<X1, X0, X0, L, L, NT>(r1, r2, ld, xs, ys);
+    pcompute<X0, 1, L, L, NT>(r2, tw, -1);
+    stage_regs<X1, L, L, NT> r3;
+    xchg_split<X0, 1, X1, L, L, NT>(r2, r3, ld, ys, ys);
+    pcompute<X1, X0, L, L, NT>(r3, tw, -1);
+    xy_kin_regs<X0, X1, L, NT>(r3, G, b0, b1, gz);
+    pcompute<X1, 1, L, L, NT>(r3, tw, +1);  // y backward, plan (X1, X0)
+    stage_regs<X0, L, L, NT> r4;
+    xchg_split<X1, 1, X0, L, L, NT>(r3, r4, ld, ys, ys);
+    pcompute<X0, X1, L, L, NT>(r4, tw, +1);
+    stage_regs<X0, L, L, NT> r5;  // x backward, plan (X0, X1)
+    xchg_split<X0, X1, X0, L, L, NT>(r4, r5, ld, ys, xs);
+    pcompute<X0, 1, L, L, NT>(r5, tw, +1);
+    stage_regs<X1, L, L, NT> r6;
+    xchg_split<X0, 1, X1, L, L, NT>(r5, r6, ld, xs, xs);
+    pcompute<X1, X0, L, L, NT>(r6, tw, +1);
+    // last stage (radix X1, span X0): butterfly j < X0 of column y holds
+    // x = j + r X0
+    constexpr int NB = L / X1;
+    constexpr int NBF = stage_regs<X1, L, L, NT>::NBF;
+#pragma unroll
+    for (int q = 0; q < NBF; q++) {
+      const int w = threadIdx.x + q * NT;
+      if ((int)threadIdx.x < NT && (NBF * NT == NB * L || w < NB * L)) {
+        const int j = w / L, c = w % L;
+#pragma unroll
+        for (int rr = 0; rr < X1; rr++) W[(j + rr * X0) * L + c] = r6.v[q][rr];
+      }
+    }
+    return;
+  }
+
   run_plan_from<P, X0, X1, X2, L, NT>(in, tw, -1, false, lx, sx, sx, true);  // x forward
   if constexpr (P::NSTAGE == 2) {
-    // y forward: its last stage stays in registers, where butterfly j < X0
-    // holds outputs j + r X0 (r < X1); after the kinetic factor they are
-    // exactly the inputs of the first stage of the inverse run with the
-    // radices in reverse order (X1, X0), which therefore starts from the
-    // same registers (one LDS round trip and a barrier fewer)
-    constexpr int NB = L / X1;  // = X0
-    constexpr int NBF = stage_regs<X1, L, L, NT>::NBF;
-    pstage<X0, 1, L, L, NT>(tw, -1, ly, sy, true);
+    pstage<X0, 1, L, L, NT>(tw, -1, ly, sy, true);  // y forward
     __syncthreads();
     stage_regs<X1, L, L, NT> r;
     pload<X1, L, L, NT>(r, ly);
     pcompute<X1, X0, L, L, NT>(r, tw, -1);
+    // the factor of xy_kin_regs, written out: through the call the compiler
+    // schedules these planes' code differently (fewer FMAs, 2 % more
+    // instructions at 64)
+    constexpr int NB = L / X1;  // = X0
+    constexpr int NBF = stage_regs<X1, L, L, NT>::NBF;
 #pragma unroll
     for (int q = 0; q < NBF; q++) {
       const int w = threadIdx.x + q * NT;

@@ -102,6 +102,50 @@ __device__ __forceinline__ void pstage(const cplx *__restrict__ tw, int sign, LD
   pfinish<R, NS, LEN, B, NT>(in, tw, sign, st, sync_mid);
 }
 
+// Hands a stage's outputs (registers `a`, radix RA, span NSA) to the next
+// stage's inputs (registers `b`, radix RB) through an LDS array of doubles
+// that holds one component at a time: real parts, then imaginary parts.
+// Half the LDS of a complex exchange for twice the barriers, so that twice
+// as many workgroups share a CU.  `sa(i, c)` is the slot of element (i, c)
+// as stored, `sb(i, c)` as loaded.  Starts with a barrier (earlier readers
+// of the array are done); ends after the loads, without one.
+template <int RA, int NSA, int RB, int LEN, int B, int NT, class SA, class SB>
+__device__ __forceinline__ void xchg_split(stage_regs<RA, LEN, B, NT> &a, stage_regs<RB, LEN, B, NT> &b,
+                                           double *lds, SA sa, SB sb) {
+  constexpr int NBA = LEN / RA, NFA = stage_regs<RA, LEN, B, NT>::NBF;
+  constexpr int NBB = LEN / RB, NFB = stage_regs<RB, LEN, B, NT>::NBF;
+#pragma unroll
+  for (int part = 0; part < 2; part++) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NFA; q++) {
+      const int w = threadIdx.x + q * NT;
+      if ((int)threadIdx.x < NT && (NFA * NT == NBA * B || w < NBA * B)) {
+        const int j = w / B, bb = w % B;
+        const int base = (j / NSA) * NSA * RA + j % NSA;
+#pragma unroll
+        for (int r = 0; r < RA; r++) lds[sa(base + r * NSA, bb)] = part ? a.v[q][r].y : a.v[q][r].x;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NFB; q++) {
+      const int w = threadIdx.x + q * NT;
+      if ((int)threadIdx.x < NT && (NFB * NT == NBB * B || w < NBB * B)) {
+        const int j = w / B, bb = w % B;
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          const double v = lds[sb(j + r * NBB, bb)];
+          if (part)
+            b.v[q][r].y = v;
+          else
+            b.v[q][r].x = v;
+        }
+      }
+    }
+  }
+}
+
 // Runs the stages of plan P (R0, R1, R2; radix 1 = absent) from the first
 // stage's inputs already in registers (pload with the plan's R0); the last
 // stage writes with `last_st`, stages in between go through LDS in place.
