@@ -229,13 +229,14 @@ __global__ __launch_bounds__(Z_NT) void batch_z_bwd_kernel(const batch_piece *__
 // shape, so their items cost the same; one workgroup per item (the item is
 // blockIdx, uniform by construction, so the piece table is read through
 // scalar loads).
+// (z tiles of z_threads(L) threads, as in the one-system kernels)
 template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(Z_NT) void group_z_fwd_kernel(const batch_piece *__restrict__ pc,
-                                                           int np, unsigned long long *, int) {
+__global__ __launch_bounds__(z_threads(R0 * R1 * R2)) void group_z_fwd_kernel(
+    const batch_piece *__restrict__ pc, int np, unsigned long long *, int) {
   extern __shared__ cplx lds[];
   const int item = blockIdx.x;
   const int i = find_segment(np, item, [&](int k) { return pc[k].z0; });
-  z_fwd_tile<R0, R1, R2, TY>(pc[i].a, item - pc[i].z0, lds);
+  z_fwd_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2)>(pc[i].a, item - pc[i].z0, lds);
 }
 
 constexpr int group_xy_threads(int L) {
@@ -254,13 +255,13 @@ void group_xy_kernel(
 }
 
 template <int R0, int R1, int R2, int TY>
-__global__ __launch_bounds__(Z_NT)
-__attribute__((amdgpu_waves_per_eu(z_bwd_min_waves(R0 * R1 * R2))))
+__global__ __launch_bounds__(z_threads(R0 * R1 * R2))
+__attribute__((amdgpu_waves_per_eu(z_bwd_waves(R0 * R1 * R2))))
 void group_z_bwd_kernel(const batch_piece *__restrict__ pc, int np, unsigned long long *, int) {
   extern __shared__ cplx lds[];
   const int item = blockIdx.x;
   const int i = find_segment(np, item, [&](int k) { return pc[k].z0; });
-  z_bwd_epi_tile<R0, R1, R2, TY>(pc[i].a, item - pc[i].z0, lds);
+  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2)>(pc[i].a, item - pc[i].z0, lds);
 }
 
 // kernels of one launch group
@@ -347,7 +348,7 @@ struct inqamd_batch {
     int L, ty;
     int zitems, xyitems;
     size_t zlds, xylds;
-    int xynt;
+    int znt, xynt;
     int zgrid_f, zgrid_b, xygrid;
     group_kernels k;
   };
@@ -564,7 +565,9 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       g.zgrid_f = g.zgrid_b = g.zitems;
       g.xygrid = g.xyitems;
       g.xynt = group_xy_threads(g.L);
+      g.znt = z_threads(g.L);
     } else {
+      g.znt = Z_NT;
       g.zgrid_f = resident_grid(g.k.zf, Z_NT, g.zlds, b->cus, g.zitems);
       g.zgrid_b = resident_grid(g.k.zb, Z_NT, g.zlds, b->cus, g.zitems);
       g.xygrid = resident_grid(g.k.xy, g.xynt, g.xylds, b->cus, g.xyitems);
@@ -639,7 +642,7 @@ void batch_run(inqamd_batch *b) {
         unsigned long long *cnt = b->counters + 1 + 3 * gi + sweep;
         const void *k = sweep == 0 ? g.k.zf : (sweep == 1 ? g.k.xy : g.k.zb);
         const int grid = sweep == 0 ? g.zgrid_f : (sweep == 1 ? g.xygrid : g.zgrid_b);
-        const int nt = sweep == 1 ? g.xynt : Z_NT;
+        const int nt = sweep == 1 ? g.xynt : g.znt;
         const size_t lds = sweep == 1 ? g.xylds : g.zlds;
         int np = g.np, total = sweep == 1 ? g.xyitems : g.zitems;
         void *args[] = {(void *)&pc, (void *)&np, (void *)&cnt, (void *)&total};

@@ -18,14 +18,6 @@ namespace inqamd {
 // planes with plane_threads(L) or twice as many threads.  The defaults (8,
 // 2x: measured fastest at 64^3 x 256) can be overridden through INQAMD_Z_TY
 // (8 or 16) and INQAMD_XY_WIDE (0 or 1).
-// Threads of a z tile: twice Z_NT on the long grids, whose 96 KB+ of LDS
-// leaves room for one workgroup per CU; 768 at 72, where the 512-thread
-// z_bwd_epi tile took 146 registers (two radix-8 butterflies and nine
-// outputs per thread) and so held a CU alone with 8 waves, while at 768 it
-// takes 124 and 12 waves share the CU (z_bwd_epi 0.58 vs 0.66 ms at
-// 72^3 x 128).
-constexpr int z_threads(int L) { return L >= 80 ? 1024 : (L == 72 ? 768 : Z_NT); }
-constexpr int z_bwd_waves(int L) { return L == 72 ? 4 : z_bwd_min_waves(L); }
 
 template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(z_threads(R0 * R1 * R2)) void z_fwd_kernel(const fft3_args a) {

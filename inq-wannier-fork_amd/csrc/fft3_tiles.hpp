@@ -513,6 +513,15 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
 // and keeps 3.
 constexpr int z_bwd_min_waves(int L) { return L == 72 ? 1 : 4; }
 
+// Threads of a z tile: twice Z_NT on the long grids, whose 96 KB+ of LDS
+// leaves room for one workgroup per CU; 768 at 72, where the 512-thread
+// z_bwd_epi tile took 146 registers (two radix-8 butterflies and nine
+// outputs per thread) and so held a CU alone with 8 waves, while at 768 it
+// takes 124 and 12 waves share the CU (z_bwd_epi 0.58 vs 0.66 ms at
+// 72^3 x 128).
+constexpr int z_threads(int L) { return L >= 80 ? 1024 : (L == 72 ? 768 : Z_NT); }
+constexpr int z_bwd_waves(int L) { return L == 72 ? 4 : z_bwd_min_waves(L); }
+
 // Grid lengths with three-sweep kernels: (L, radices of the plan)
 #define INQAMD_FFT3_TABLE(X) \
   X(24, 8, 3, 1)             \
