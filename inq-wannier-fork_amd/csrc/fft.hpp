@@ -89,6 +89,9 @@ struct fft3_args {
   long hld;
   cplx *W;           // band-major intermediate [nb][L][L][L]
   int nb;
+  // psi, hpsi, Qe and the accumulators hold chunk band cb at offset
+  // cb + (cb / 8) * gskip (0: the chunk's bands are contiguous)
+  int gskip;
   const double *V;   // [N]
   const double *kA;  // device covariant k + A
   double rlat[9];

@@ -53,6 +53,7 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
   const int yt = (int)(t % nyt);
   const int x = (int)(t / nyt);
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
+  const int gb0 = cb0 + (cb0 >> 3) * a.gskip;  // the tile's first band in psi
 
   // The first stage reads psi straight into registers: a wave takes one
   // butterfly input of all 64 columns (TY y rows x TB bands), i.e. TY runs
@@ -61,7 +62,7 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
     const int b = c / Z_TY, y = c % Z_TY;
     if (!(y0 + y < L && cb0 + b < a.nb)) return mk(0.0, 0.0);
     const long p = ((long)x * L + y0 + y) * L + i;
-    cplx v = a.psi[p * a.ld + cb0 + b];
+    cplx v = a.psi[p * a.ld + gb0 + b];
     if constexpr (POT) {
       const double vp = a.V[p];
       v = mk(vp * v.x, vp * v.y);
@@ -69,7 +70,7 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
       const int w = a.own ? a.own[p] : -1;
       if (w >= 0) {
         const int e0 = w >> OWN_COUNT_BITS, cnt = w & OWN_MAX_COUNT;
-        for (int j = 0; j < cnt; j++) v = cadd(v, a.Qe[(long)(e0 + j) * a.ldQ + cb0 + b]);
+        for (int j = 0; j < cnt; j++) v = cadd(v, a.Qe[(long)(e0 + j) * a.ldQ + gb0 + b]);
       }
     }
     return v;
@@ -359,6 +360,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   const int yt = (int)(t % nyt);
   const int x = (int)(t / nyt);
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
+  const int gb0 = cb0 + (cb0 >> 3) * a.gskip;  // the tile's first band in psi, hpsi
 
   // prefetch V psi (and the ownership entries) of this thread's outputs,
   // or 1/2 |G+k+A|^2 kin_in
@@ -385,9 +387,9 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
         const double q0 = gx[0] + my * a.rlat[3] + mz * a.rlat[6];
         const double q1 = gx[1] + my * a.rlat[4] + mz * a.rlat[7];
         const double q2 = gx[2] + my * a.rlat[5] + mz * a.rlat[8];
-        pre[k] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), a.kin_in[p * a.kin_ld + cb0 + b]);
+        pre[k] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), a.kin_in[p * a.kin_ld + gb0 + b]);
       } else {
-        const cplx ps = a.psi[p * a.ld + cb0 + b];
+        const cplx ps = a.psi[p * a.ld + gb0 + b];
         const double vp = a.V[p];
         pre[k] = mk(vp * ps.x, vp * ps.y);
         if (a.own) own[k] = a.own[p];
@@ -423,14 +425,14 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   for (int k = 0; k < NE; k++) {
     q[k] = mk(0.0, 0.0);
     if (own[k] >= 0 && (own[k] & OWN_MAX_COUNT) > 0) {
-      const int cb = cb0 + (int)(threadIdx.x + k * NT) % Z_TB;
+      const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
       q[k] = a.Qe[(long)(own[k] >> OWN_COUNT_BITS) * a.ldQ + cb];
     }
   }
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     if (own[k] >= 0) {
-      const int cb = cb0 + (int)(threadIdx.x + k * NT) % Z_TB;
+      const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
       const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
       if (cnt > 0) pre[k] = cadd(pre[k], q[k]);
       for (int j = 1; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
@@ -458,7 +460,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
       const int e = threadIdx.x + k * NT;
       const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
       if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb)
-        store(((long)x * L + y0 + y) * L + z, cb0 + b, result(k, b, y, z));
+        store(((long)x * L + y0 + y) * L + z, gb0 + b, result(k, b, y, z));
     }
   } else {
     // Outputs in groups of G: the group's Taylor accumulators are read
@@ -476,8 +478,8 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
         if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
           const long p = ((long)x * L + y0 + y) * L + z;
           if (a.acc1)
-            o1[g] = a.acc1_init ? a.psi[p * a.ld + cb0 + b] : a.acc1[p * a.acc1_ld + cb0 + b];
-          if (a.acc2) o2[g] = a.acc2[p * a.acc2_ld + cb0 + b];
+            o1[g] = a.acc1_init ? a.psi[p * a.ld + gb0 + b] : a.acc1[p * a.acc1_ld + gb0 + b];
+          if (a.acc2) o2[g] = a.acc2[p * a.acc2_ld + gb0 + b];
         }
       }
 #pragma unroll
@@ -486,7 +488,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
         const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
         if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
           const long p = ((long)x * L + y0 + y) * L + z;
-          const int cb = cb0 + b;
+          const int cb = gb0 + b;
           const cplx v = result(k0 + g, b, y, z);
           store(p, cb, v);
           if (a.acc1) {

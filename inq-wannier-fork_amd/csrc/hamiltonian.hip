@@ -243,9 +243,10 @@ void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int ns
 // path leaves both to the caller.
 static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
                          int slot, int b0, long ldP, const taylor_acc *acc, int sweep = -1,
-                         hipStream_t stream = nullptr, cplx *W = nullptr) {
+                         hipStream_t stream = nullptr, cplx *W = nullptr, int gskip = 0) {
   if (h.three_sweep) {
     fft3_args a{};
+    a.gskip = gskip;
     a.psi = psi;
     a.ld = ld;
     a.hpsi = hpsi;
@@ -276,7 +277,10 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       a.c2re = acc->c2re;
       a.c2im = acc->c2im;
     }
-    if (acc && acc->gdump) a.gdump = acc->gdump + (size_t)b0 * h.npoint;
+    if (acc && acc->gdump) {
+      if (gskip) throw error{ERR_UNSUPPORTED, "band-group chunks with a transform dump"};
+      a.gdump = acc->gdump + (size_t)b0 * h.npoint;
+    }
     static const int nt_store = [] {
       const char *e = getenv("INQAMD_NT_STORE");
       return e && atoi(e) == 1 ? 1 : 0;
@@ -316,6 +320,38 @@ static int band_chunk(const handle &h, int nst) {
   return c < nst ? (int)c : nst;
 }
 
+// Band chunks of the three-sweep Hpsi.  Where rows are a multiple of 4 KB
+// (ld % 256 == 0) and the bands come in whole groups of 8, chunk c of n
+// takes the groups c, c + n, c + 2n, ... rather than a contiguous range: a
+// contiguous chunk's reads and writes all fall in one 768-byte window of
+// every row and crowd a few HBM channels (64^3 x 256: z_fwd 4.4 TB/s,
+// against 5.0 at ld = 240 or 288), while groups spread over the row do not.
+// Other strides keep contiguous chunks (groups were 3 % slower at ld = 128).
+// INQAMD_BAND_GROUPS=0 keeps them always.
+struct band_chunks {
+  int nst = 0, chunk = 0, n = 1, gskip = 0;
+  int b0(int c) const { return gskip ? 8 * c : c * chunk; }
+  int nb(int c) const {
+    if (gskip) return 8 * ((nst / 8 - c + n - 1) / n);
+    return nst - c * chunk < chunk ? nst - c * chunk : chunk;
+  }
+};
+
+static band_chunks chunk_plan(const handle &h, int nst, long ld, bool groups) {
+  static const bool enabled = [] {
+    const char *e = getenv("INQAMD_BAND_GROUPS");
+    return !(e && atoi(e) == 0);
+  }();
+  band_chunks p;
+  p.nst = nst;
+  p.chunk = band_chunk(h, nst);
+  p.n = (nst + p.chunk - 1) / p.chunk;
+  if (enabled && groups && h.three_sweep && ld % 256 == 0 && p.n > 1 && nst % 8 == 0 &&
+      p.chunk % 8 == 0)
+    p.gskip = 8 * (p.n - 1);
+  return p;
+}
+
 void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
            int nst);
 
@@ -329,7 +365,8 @@ void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, 
 // exponential.hpp:32, :64-65) in the same write.
 void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nst,
                        int slot, const taylor_acc *acc) {
-  const int chunk = band_chunk(h, nst);
+  const band_chunks plan = chunk_plan(h, nst, ld, !(acc && acc->gdump));
+  const int chunk = plan.chunk;
   // three-sweep chunks alternate between two streams (and two halves of the
   // intermediate) so one chunk's launches fill the tails of the other's.
   // Called eagerly, the first chunk (on the second stream) also runs its two
@@ -359,22 +396,23 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
     projector_project_expand(h, psi, ld, nst, nullptr, 0, true);
   if (overlap) INQAMD_HIP_CHECK(hipEventRecord(h.ev_proj, h.stream));
   if (two && !overlap) fork();
-  int c = 0;
-  for (int b0 = 0; b0 < nst; b0 += chunk, c++) {
-    const int nb = nst - b0 < chunk ? nst - b0 : chunk;
+  const int gs = plan.gskip;  // 0 unless three-sweep
+  for (int c = 0; c < plan.n; c++) {
+    const int b0 = plan.b0(c), nb = plan.nb(c);
     if (two) {
       hipStream_t s = ((c + (overlap ? 1 : 0)) & 1) ? h.aux : h.stream;
       cplx *W = h.work + (size_t)(c & 1) * chunk * h.npoint;
       if (c == 0 && overlap) {
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 0, s, W);
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 1, s, W);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 0, s, W, gs);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 1, s, W, gs);
         INQAMD_HIP_CHECK(hipStreamWaitEvent(s, h.ev_proj, 0));
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 2, s, W);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 2, s, W, gs);
       } else {
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, -1, s, W);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, -1, s, W, gs);
       }
     } else {
-      local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc);
+      local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, -1, nullptr, nullptr,
+                   gs);
     }
   }
   if (two) {
@@ -944,8 +982,8 @@ int inqamd_hamiltonian_profile(inqamd_hamiltonian_t h, const void *psi, void *hp
     INQAMD_REQUIRE(ms && launches, "outputs are null");
     if (!h->three_sweep)
       throw error{ERR_UNSUPPORTED, "per-sweep timing needs the three-sweep Hpsi (cubic grid)"};
-    const int chunk = band_chunk(*h, nst);
-    const int nchunk = (nst + chunk - 1) / chunk;
+    const band_chunks plan = chunk_plan(*h, nst, ld, true);
+    const int nchunk = plan.n;
     std::vector<hipEvent_t> ev(2 + 3 * nchunk, nullptr);
     auto cleanup = [&] {
       for (hipEvent_t e : ev)
@@ -958,11 +996,11 @@ int inqamd_hamiltonian_profile(inqamd_hamiltonian_t h, const void *psi, void *hp
       projector_project_expand(*h, (const cplx *)psi, ld, nst, h->vnl, nst, false);
       INQAMD_HIP_CHECK(hipEventRecord(ev[1], h->stream));
       int k = 1;
-      for (int b0 = 0; b0 < nst; b0 += chunk) {
-        const int nb = nst - b0 < chunk ? nst - b0 : chunk;
+      for (int c = 0; c < nchunk; c++) {
+        const int b0 = plan.b0(c), nb = plan.nb(c);
         for (int sweep = 0; sweep < 3; sweep++) {
           local_passes(*h, (const cplx *)psi + b0, ld, (cplx *)hpsi + b0, ld, nb, KA_CURRENT, b0,
-                       nst, nullptr, sweep);
+                       nst, nullptr, sweep, nullptr, nullptr, plan.gskip);
           INQAMD_HIP_CHECK(hipEventRecord(ev[++k], h->stream));
         }
       }

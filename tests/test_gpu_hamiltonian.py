@@ -383,6 +383,34 @@ def test_hpsi_60_72_match_oracle(device, L):
     ham.close()
 
 
+def test_band_group_chunks_match_contiguous_bitwise(device, cfg1):
+    """64^3 x 256 at ld = 256 runs its chunks as interleaved 8-band groups;
+    the same bands at ld = 264 run contiguous chunks.  Hpsi and one ETRS step
+    (Taylor accumulators) must agree bitwise: every band goes through the
+    same arithmetic whichever chunk holds it."""
+    s, psi = cfg1
+    say("band-group chunks vs contiguous, 64^3 x 256")
+    ham = inq_amd.ks_hamiltonian(s, max_bands=256)
+    nst, ldp = 256, 264
+    d = to_dev(psi, device)
+    pad = torch.zeros(psi.shape[:3] + (ldp,), dtype=torch.complex128, device=device)
+    pad[..., :nst] = d
+    hp = torch.zeros_like(pad)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    h = ham(d)
+    ham.handle.call("inqamd_hamiltonian_apply", ptr(pad), ptr(hp), nst, ldp)
+    torch.cuda.synchronize()
+    assert torch.equal(h, hp[..., :nst])
+    assert torch.count_nonzero(hp[..., nst:]).item() == 0
+    A0 = (ctypes.c_double * 3)(0.01, -0.02, 0.03)
+    A1 = (ctypes.c_double * 3)(0.012, -0.018, 0.031)
+    for t, ld in ((d, nst), (pad, ldp)):
+        ham.handle.call("inqamd_etrs_step", ctypes.c_double(0.02), A0, A1, 4, ptr(t), nst, ld)
+    torch.cuda.synchronize()
+    assert torch.equal(d, pad[..., :nst])
+    ham.close()
+
+
 def test_pipelined_chunks_graph_and_oracle(device):
     """A 64^3 system whose Hpsi runs two band chunks on two streams (max_bands
     64, 40 bands, 32-band chunks): Hpsi against the oracle, and a captured
