@@ -408,6 +408,15 @@ def test_band_group_chunks_match_contiguous_bitwise(device, cfg1):
         ham.handle.call("inqamd_etrs_step", ctypes.c_double(0.02), A0, A1, 4, ptr(t), nst, ld)
     torch.cuda.synchronize()
     assert torch.equal(d, pad[..., :nst])
+    # a captured step (grouped chunks, A = 0) against the eager contiguous one
+    zero = (ctypes.c_double * 3)(0.0, 0.0, 0.0)
+    g = inq_amd.real_time.etrs_graph(ham, d, 0.02, 4, pert=None)
+    for _ in range(2):
+        g.launch(0.0)
+        ham.handle.call("inqamd_etrs_step", ctypes.c_double(0.02), zero, zero, 4, ptr(pad), nst, ldp)
+        torch.cuda.synchronize()
+        assert torch.equal(d, pad[..., :nst])
+    g.close()
     ham.close()
 
 
