@@ -90,8 +90,9 @@ struct fft3_args {
   cplx *W;           // band-major intermediate [nb][L][L][L]
   int nb;
   // psi, hpsi, Qe and the accumulators hold chunk band cb at offset
-  // cb + (cb / 8) * gskip (0: the chunk's bands are contiguous)
-  int gskip;
+  // cb + (cb >> (3 + glog)) * gskip: groups of 8 << glog bands
+  // (gskip 0: the chunk's bands are contiguous)
+  int gskip, glog;
   const double *V;   // [N]
   const double *kA;  // device covariant k + A
   double rlat[9];

@@ -53,7 +53,7 @@ __device__ __forceinline__ void z_fwd_tile(const fft3_args &a, long t, cplx *lds
   const int yt = (int)(t % nyt);
   const int x = (int)(t / nyt);
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
-  const int gb0 = cb0 + (cb0 >> 3) * a.gskip;  // the tile's first band in psi
+  const int gb0 = cb0 + (cb0 >> (3 + a.glog)) * a.gskip;  // the tile's first band in psi
 
   // The first stage reads psi straight into registers: a wave takes one
   // butterfly input of all 64 columns (TY y rows x TB bands), i.e. TY runs
@@ -360,7 +360,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   const int yt = (int)(t % nyt);
   const int x = (int)(t / nyt);
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
-  const int gb0 = cb0 + (cb0 >> 3) * a.gskip;  // the tile's first band in psi, hpsi
+  const int gb0 = cb0 + (cb0 >> (3 + a.glog)) * a.gskip;  // the tile's first band in psi, hpsi
 
   // prefetch V psi (and the ownership entries) of this thread's outputs,
   // or 1/2 |G+k+A|^2 kin_in

@@ -243,10 +243,12 @@ void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int ns
 // path leaves both to the caller.
 static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
                          int slot, int b0, long ldP, const taylor_acc *acc, int sweep = -1,
-                         hipStream_t stream = nullptr, cplx *W = nullptr, int gskip = 0) {
+                         hipStream_t stream = nullptr, cplx *W = nullptr, int gskip = 0,
+                         int glog = 0) {
   if (h.three_sweep) {
     fft3_args a{};
     a.gskip = gskip;
+    a.glog = glog;
     a.psi = psi;
     a.ld = ld;
     a.hpsi = hpsi;
@@ -329,10 +331,11 @@ static int band_chunk(const handle &h, int nst) {
 // Other strides keep contiguous chunks (groups were 3 % slower at ld = 128).
 // INQAMD_BAND_GROUPS=0 keeps them always.
 struct band_chunks {
-  int nst = 0, chunk = 0, n = 1, gskip = 0;
-  int b0(int c) const { return gskip ? 8 * c : c * chunk; }
+  int nst = 0, chunk = 0, n = 1, gskip = 0, glog = 0;
+  int gsize() const { return 8 << glog; }
+  int b0(int c) const { return gskip ? gsize() * c : c * chunk; }
   int nb(int c) const {
-    if (gskip) return 8 * ((nst / 8 - c + n - 1) / n);
+    if (gskip) return gsize() * ((nst / gsize() - c + n - 1) / n);
     return nst - c * chunk < chunk ? nst - c * chunk : chunk;
   }
 };
@@ -342,13 +345,21 @@ static band_chunks chunk_plan(const handle &h, int nst, long ld, bool groups) {
     const char *e = getenv("INQAMD_BAND_GROUPS");
     return !(e && atoi(e) == 0);
   }();
+  static const int glog = [] {
+    const char *e = getenv("INQAMD_BAND_GROUP_SIZE");
+    const int g = e ? atoi(e) : 8;
+    return g == 32 ? 2 : (g == 16 ? 1 : 0);
+  }();
   band_chunks p;
   p.nst = nst;
   p.chunk = band_chunk(h, nst);
   p.n = (nst + p.chunk - 1) / p.chunk;
-  if (enabled && groups && h.three_sweep && ld % 256 == 0 && p.n > 1 && nst % 8 == 0 &&
-      p.chunk % 8 == 0)
-    p.gskip = 8 * (p.n - 1);
+  const int gs = 8 << glog;
+  if (enabled && groups && h.three_sweep && ld % 256 == 0 && p.n > 1 && nst % gs == 0 &&
+      p.chunk % gs == 0 && p.n <= nst / gs) {
+    p.glog = glog;
+    p.gskip = gs * (p.n - 1);
+  }
   return p;
 }
 
@@ -403,16 +414,16 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
       hipStream_t s = ((c + (overlap ? 1 : 0)) & 1) ? h.aux : h.stream;
       cplx *W = h.work + (size_t)(c & 1) * chunk * h.npoint;
       if (c == 0 && overlap) {
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 0, s, W, gs);
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 1, s, W, gs);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 0, s, W, gs, plan.glog);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 1, s, W, gs, plan.glog);
         INQAMD_HIP_CHECK(hipStreamWaitEvent(s, h.ev_proj, 0));
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 2, s, W, gs);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 2, s, W, gs, plan.glog);
       } else {
-        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, -1, s, W, gs);
+        local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, -1, s, W, gs, plan.glog);
       }
     } else {
       local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, -1, nullptr, nullptr,
-                   gs);
+                   gs, plan.glog);
     }
   }
   if (two) {
@@ -1000,7 +1011,7 @@ int inqamd_hamiltonian_profile(inqamd_hamiltonian_t h, const void *psi, void *hp
         const int b0 = plan.b0(c), nb = plan.nb(c);
         for (int sweep = 0; sweep < 3; sweep++) {
           local_passes(*h, (const cplx *)psi + b0, ld, (cplx *)hpsi + b0, ld, nb, KA_CURRENT, b0,
-                       nst, nullptr, sweep, nullptr, nullptr, plan.gskip);
+                       nst, nullptr, sweep, nullptr, nullptr, plan.gskip, plan.glog);
           INQAMD_HIP_CHECK(hipEventRecord(ev[++k], h->stream));
         }
       }
