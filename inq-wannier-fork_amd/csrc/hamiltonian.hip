@@ -449,10 +449,12 @@ void hamiltonian_fourier_sweeps(handle &h, const cplx *phi, long ld, cplx *hphi,
   fft3d(h, phi, ld, h.hn, nst, nst, +1, 1.0 / (double)h.npoint);
   projector_phases(h, KA_CURRENT);
   projector_project_expand(h, h.hn, nst, nst, h.vnl, nst, false);
-  const int chunk = band_chunk(h, nst);
-  for (int b0 = 0; b0 < nst; b0 += chunk) {
-    const int nb = nst - b0 < chunk ? nst - b0 : chunk;
+  const band_chunks plan = chunk_plan(h, nst, ld, true);
+  for (int c = 0; c < plan.n; c++) {
+    const int b0 = plan.b0(c), nb = plan.nb(c);
     fft3_args a{};
+    a.gskip = plan.gskip;
+    a.glog = plan.glog;
     a.psi = h.hn + b0;
     a.ld = nst;
     a.hpsi = hphi + b0;

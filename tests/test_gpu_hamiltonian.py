@@ -385,9 +385,10 @@ def test_hpsi_60_72_match_oracle(device, L):
 
 def test_band_group_chunks_match_contiguous_bitwise(device, cfg1):
     """64^3 x 256 at ld = 256 runs its chunks as interleaved 8-band groups;
-    the same bands at ld = 264 run contiguous chunks.  Hpsi and one ETRS step
-    (Taylor accumulators) must agree bitwise: every band goes through the
-    same arithmetic whichever chunk holds it."""
+    the same bands at ld = 264 run contiguous chunks.  Hpsi, G-space Hpsi,
+    an ETRS step (Taylor accumulators) and a captured ETRS step must agree
+    bitwise: every band goes through the same arithmetic whichever chunk
+    holds it."""
     s, psi = cfg1
     say("band-group chunks vs contiguous, 64^3 x 256")
     ham = inq_amd.ks_hamiltonian(s, max_bands=256)
@@ -408,6 +409,11 @@ def test_band_group_chunks_match_contiguous_bitwise(device, cfg1):
         ham.handle.call("inqamd_etrs_step", ctypes.c_double(0.02), A0, A1, 4, ptr(t), nst, ld)
     torch.cuda.synchronize()
     assert torch.equal(d, pad[..., :nst])
+    # G-space Hpsi (its sweeps take the same chunks)
+    ham.handle.call("inqamd_hamiltonian_apply_fourier", ptr(d), ptr(h), nst, nst)
+    ham.handle.call("inqamd_hamiltonian_apply_fourier", ptr(pad), ptr(hp), nst, ldp)
+    torch.cuda.synchronize()
+    assert torch.equal(h, hp[..., :nst])
     # a captured step (grouped chunks, A = 0) against the eager contiguous one
     zero = (ctypes.c_double * 3)(0.0, 0.0, 0.0)
     g = inq_amd.real_time.etrs_graph(ham, d, 0.02, 4, pert=None)
