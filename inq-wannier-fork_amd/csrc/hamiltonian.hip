@@ -395,9 +395,17 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
     INQAMD_HIP_CHECK(hipStreamIsCapturing(h.stream, &cs));
     overlap = cs == hipStreamCaptureStatusNone;
   }
+  // INQAMD_STREAMS=3 deals the chunks over three streams (and three slots
+  // of the intermediate) where three chunks fit
+  static const int nstreams = [] {
+    const char *e = getenv("INQAMD_STREAMS");
+    return e && atoi(e) == 3 ? 3 : 2;
+  }();
+  const int ns = two && nstreams == 3 && h.aux2 && plan.n >= 3 && 3 * chunk <= h.max_bands ? 3 : 2;
   auto fork = [&] {
     INQAMD_HIP_CHECK(hipEventRecord(h.ev_fork, h.stream));
     INQAMD_HIP_CHECK(hipStreamWaitEvent(h.aux, h.ev_fork, 0));
+    if (ns == 3) INQAMD_HIP_CHECK(hipStreamWaitEvent(h.aux2, h.ev_fork, 0));
   };
   if (overlap) fork();
   projector_phases(h, slot);
@@ -408,12 +416,15 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
   if (overlap) INQAMD_HIP_CHECK(hipEventRecord(h.ev_proj, h.stream));
   if (two && !overlap) fork();
   const int gs = plan.gskip;  // 0 unless three-sweep
+  bool waited[3] = {true, false, false};  // the main stream runs after the projector
   for (int c = 0; c < plan.n; c++) {
     const int b0 = plan.b0(c), nb = plan.nb(c);
     if (two) {
-      hipStream_t s = ((c + (overlap ? 1 : 0)) & 1) ? h.aux : h.stream;
-      cplx *W = h.work + (size_t)(c & 1) * chunk * h.npoint;
-      if (c == 0 && overlap) {
+      const int si = (c + (overlap ? 1 : 0)) % ns;
+      hipStream_t s = si == 0 ? h.stream : (si == 1 ? h.aux : h.aux2);
+      cplx *W = h.work + (size_t)(c % ns) * chunk * h.npoint;
+      if (overlap && !waited[si]) {
+        waited[si] = true;
         local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 0, s, W, gs, plan.glog);
         local_passes(h, psi + b0, ld, hpsi + b0, hld, nb, slot, b0, nst, acc, 1, s, W, gs, plan.glog);
         INQAMD_HIP_CHECK(hipStreamWaitEvent(s, h.ev_proj, 0));
@@ -429,6 +440,10 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
   if (two) {
     INQAMD_HIP_CHECK(hipEventRecord(h.ev_join, h.aux));
     INQAMD_HIP_CHECK(hipStreamWaitEvent(h.stream, h.ev_join, 0));
+    if (ns == 3) {
+      INQAMD_HIP_CHECK(hipEventRecord(h.ev_join2, h.aux2));
+      INQAMD_HIP_CHECK(hipStreamWaitEvent(h.stream, h.ev_join2, 0));
+    }
   }
   if (!h.three_sweep) {
     projector_apply_coefficients(h, hpsi, hld, nst, slot);
@@ -615,10 +630,12 @@ static void release(handle &h) {
   h.proj_part = nullptr;
   if (h.ev_fork) (void)hipEventDestroy(h.ev_fork);
   if (h.ev_join) (void)hipEventDestroy(h.ev_join);
+  if (h.ev_join2) (void)hipEventDestroy(h.ev_join2);
+  if (h.aux2) (void)hipStreamDestroy(h.aux2);
   if (h.ev_proj) (void)hipEventDestroy(h.ev_proj);
   if (h.aux) (void)hipStreamDestroy(h.aux);
-  h.ev_fork = h.ev_join = h.ev_proj = nullptr;
-  h.aux = nullptr;
+  h.ev_fork = h.ev_join = h.ev_join2 = h.ev_proj = nullptr;
+  h.aux = h.aux2 = nullptr;
   projector_free(h);
 }
 
@@ -686,6 +703,8 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
       INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&h.aux, hipStreamNonBlocking));
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_fork, hipEventDisableTiming));
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_join, hipEventDisableTiming));
+      INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&h.aux2, hipStreamNonBlocking));
+      INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_join2, hipEventDisableTiming));
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_proj, hipEventDisableTiming));
     }
   }

@@ -81,7 +81,8 @@ struct inqamd_hamiltonian {
   double volume = 0.0;
   hipStream_t stream = nullptr;
   hipStream_t aux = nullptr;        // second stream: odd band chunks of Hpsi
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_proj = nullptr;
+  hipStream_t aux2 = nullptr;       // third stream (INQAMD_STREAMS=3)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr, ev_proj = nullptr;
   int max_bands = 0;
 
   inqamd::fft_axis_plan axis[3];
