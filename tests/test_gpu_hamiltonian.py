@@ -426,6 +426,32 @@ def test_band_group_chunks_match_contiguous_bitwise(device, cfg1):
     ham.close()
 
 
+def test_band_groups_uneven_chunks_bitwise(device, cfg1):
+    """232 bands in rows of 256 (ld = 256): 29 groups of 8 over five chunks
+    (6, 6, 6, 6, 5 groups), against the same bands at ld = 264 (contiguous
+    chunks): Hpsi bitwise, and the padding columns untouched."""
+    s, psi = cfg1
+    say("band-group chunks, 232 of 256 columns")
+    nst = 232
+    ham = inq_amd.ks_hamiltonian(s, max_bands=256)
+    src = to_dev(psi[..., :nst], device)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    outs = []
+    for ld in (256, 264):
+        a = torch.zeros(psi.shape[:3] + (ld,), dtype=torch.complex128, device=device)
+        a[..., :nst] = src
+        o = torch.zeros_like(a)
+        ham.handle.call("inqamd_hamiltonian_apply", ptr(a), ptr(o), nst, ld)
+        torch.cuda.synchronize()
+        assert torch.count_nonzero(o[..., nst:]).item() == 0
+        outs.append(o[..., :nst])
+    assert torch.equal(outs[0], outs[1])
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(np.ascontiguousarray(psi[..., [0, 231]]))
+    got = to_host(outs[0])[..., [0, 231]]
+    assert relerr(got, ref) < TOL
+    ham.close()
+
+
 def test_pipelined_chunks_graph_and_oracle(device):
     """A 64^3 system whose Hpsi runs two band chunks on two streams (max_bands
     64, 40 bands, 32-band chunks): Hpsi against the oracle, and a captured
