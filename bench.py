@@ -471,11 +471,64 @@ def run_tddft_64(args):
     return out
 
 
+def run_stream(args):
+    """The streaming ceiling: the library's own 16-byte vector read / write /
+    copy kernel (inqamd_stream_probe) over 1 GiB contiguous, and over a
+    48-band chunk of 64^3 x 256 rows (ld = 256), as a contiguous window and as
+    interleaved 8-band groups, at 1, 2 and 4 workgroups per CU."""
+    import ctypes
+    import torch
+    from inq_amd import _lib
+
+    L = _lib.lib()
+    dev = torch.device("cuda:0")
+    N, ld, nb, nch = 64 ** 3, 256, 48, 6
+    total = N * ld                      # complex128 elements (1 GiB)
+    a = torch.ones(total, dtype=torch.complex128, device=dev)
+    b = torch.zeros_like(a)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ms = ctypes.c_float(0.0)
+    cases = [("contiguous", total, 1, 1, 0, 1)]
+    cases += [("rows ld=256, 48-band window", N, ld, nb, 0, 1),
+              ("rows ld=256, 48 bands in 8-band groups", N, ld, nb, 1, nch)]
+    rows_out = []
+    best = {}
+    for name, rows, rld, rnb, grouped, nchunks in cases:
+        for kind, kname in ((0, "read"), (1, "write"), (2, "copy")):
+            for bpc in (1, 2, 4, 8):
+                times = []
+                for _ in range(args.warmup + args.steps):
+                    _lib.check(L.inqamd_stream_probe(kind, ctypes.c_void_p(a.data_ptr()),
+                                                     ctypes.c_void_p(b.data_ptr()), rows, rld,
+                                                     rnb, grouped, nchunks, bpc, stream,
+                                                     ctypes.byref(ms)))
+                    times.append(ms.value)
+                t = sorted(times[args.warmup:])[len(times[args.warmup:]) // 2]
+                nbytes = rows * rnb * 16.0 * (2 if kind == 2 else 1)
+                gbs = nbytes / (t / 1e3) / 1e9
+                rows_out.append({"pattern": name, "kind": kname, "wg_per_cu": bpc,
+                                 "bytes": nbytes, "median_ms": t, "GBs": gbs})
+                key = (name, kname)
+                best[key] = max(best.get(key, 0.0), gbs)
+                log(f"{name:40s} {kname:5s} {bpc} WG/CU: {t:.3f} ms {gbs:7.0f} GB/s")
+    cfg = {"workload": "stream: 16-byte vector read/write/copy, 1 GiB contiguous and a 48-band "
+                       "chunk of 64^3 x 256 complex128 rows", "parallelism": "single GPU"}
+    top = best[("contiguous", "copy")]
+    out = common_fields(args, top, "GB/s", None, True, cfg)
+    out["best"] = {f"{k[0]} {k[1]}": v for k, v in best.items()}
+    out["cases"] = rows_out
+    out["roofline"] = {"bound": "hbm", "achieved": top, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": top / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "probe_kernel<2> (copy), 1 GiB contiguous"}
+    out["cpu_baseline"] = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="hpsi_64",
                     choices=["hpsi_64", "uneven_batch", "adjoint_48", "etrs_96", "adjoint_96",
-                             "gs_64", "tddft_64"])
+                             "gs_64", "tddft_64", "stream"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -494,6 +547,9 @@ def main():
 
     if args.workload == "uneven_batch":
         print(json.dumps(run_uneven_batch(args)), flush=True)
+        return
+    if args.workload == "stream":
+        print(json.dumps(run_stream(args)), flush=True)
         return
     if args.workload == "adjoint_48":
         out = run_adjoint(args, 1, 48, 64, 400, 20, 0.015,

@@ -83,6 +83,17 @@ const char *inqamd_version(void);
  * B[k][j] = 2 + 3j - 5k into the DEVICE array out[256].  No reference
  * counterpart; a diagnostic for the projector contraction. */
 int inqamd_selftest_mfma(double *out_device, void *stream);
+/* Streaming-bandwidth probe (diagnostic, no reference counterpart): one
+ * launch of blocks_per_cu x CUs workgroups moving 16-byte complex128 elements
+ * with plain vector loads/stores -- kind 0 read, 1 write, 2 copy (src -> dst,
+ * same positions) -- over rows x nb elements of rows of ld elements: the
+ * first nb of each row, or with grouped != 0 the 8-element groups 0,
+ * nchunks, 2 nchunks, ... (chunk 0 of nchunks interleaved band groups).
+ * ld == nb is a flat contiguous range.  DEVICE src/dst of rows x ld
+ * elements (the read probe also needs dst).  Writes the launch's duration
+ * (HIP events on `stream`) to *ms; synchronises. */
+int inqamd_stream_probe(int kind, const void *src, void *dst, long rows, int ld, int nb,
+                        int grouped, int nchunks, int blocks_per_cu, void *stream, float *ms);
 
 /* orbital_set::kpoint() (orbital_set.hpp) -- covariant k. */
 int inqamd_set_kpoint(inqamd_hamiltonian_t h, const double k_cov[3]);
@@ -285,12 +296,19 @@ void inqamd_graph_destroy(inqamd_graph_t g);
  * observables::current (current.hpp:25-53) for bands of equal occupation occ:
  *   J = occ sum_b Im <phi_b| g |phi_b>,  g = -i(grad + i(k+A)) + [V_nl, r]
  * (gradient.hpp:43-48 with factor -1; projector_all::position_commutator,
- * projector_all.hpp:319-403, with each KB coefficient applied once -- the
- * reference multiplies the plain projection by it twice and never scales the
- * r-weighted one, :362-366; applied once, J = -dE/dA exactly).  J_cov is the
- * covariant vector current() returns, J_cart its Cartesian form
+ * projector_all.hpp:319-403).  The [V_nl, r] term has two forms:
+ *   INQAMD_COMMUTATOR_INQ (the default): as the reference writes it, the
+ *     plain projection multiplied by the KB coefficient D twice and the
+ *     r-weighted one never (:362-366) -- identical results to INQ;
+ *   INQAMD_COMMUTATOR_CORRECTED: each projection scaled by D once, which
+ *     makes J = -dE/dA exactly (the INQ form equals it with every D replaced
+ *     by (1 + D^2)/2).
+ * inqamd_set_position_commutator picks the form for the handle.  J_cov is
+ * the covariant vector current() returns, J_cart its Cartesian form
  * (viewables.hpp:100-102).  nst <= max_bands.  HOST outputs: synchronises
  * the handle's stream. */
+enum { INQAMD_COMMUTATOR_INQ = 0, INQAMD_COMMUTATOR_CORRECTED = 1 };
+int inqamd_set_position_commutator(inqamd_hamiltonian_t h, int commutator);
 int inqamd_current(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, double occ,
                    double J_cov[3], double J_cart[3]);
 
@@ -322,10 +340,17 @@ typedef struct {
   int checkpoint_every;     /* steps per checkpoint segment (>= 1) */
   int band_slice;           /* bands per backward slice (0: all) */
   int store_powers;         /* 1: keep H^j phi of a segment's half steps; 0: recompute */
+  int commutator;           /* the loss's [V_nl, r] form: INQAMD_COMMUTATOR_INQ (0) or
+                               INQAMD_COMMUTATOR_CORRECTED (see inqamd_current) */
 } inqamd_td_desc;
 
 /* Allocates checkpoints and adjoint workspaces (device memory). */
 int inqamd_td_create(inqamd_hamiltonian_t h, const inqamd_td_desc *desc, inqamd_td_t *out);
+/* Forwards run on this object so far (-1 for NULL).  The backward always
+ * differentiates the LAST forward; a caller that keeps several losses alive
+ * (e.g. autograd) records this number at its forward and compares it before
+ * its backward.  No reference counterpart. */
+long inqamd_td_forward_count(inqamd_td_t td);
 /* psi0: DEVICE [N][ld]; coeff: HOST [ncoef]; psi_T: DEVICE [N][ld] or NULL;
  * loss: HOST.  Synchronises the handle's stream. */
 int inqamd_td_forward(inqamd_td_t td, const double *coeff, const void *psi0, int ld,
@@ -333,7 +358,9 @@ int inqamd_td_forward(inqamd_td_t td, const double *coeff, const void *psi0, int
 /* Gradients of dloss * L at the last forward's inputs: grad_coeff HOST
  * [ncoef], grad_vloc DEVICE [N] (overwritten; may be NULL), lambda0 DEVICE
  * [N][nst] or NULL: the adjoint state at t = 0 (dL = Re sum conj(lambda0)
- * dpsi0 over grid points and bands).  Synchronises the handle's stream. */
+ * dpsi0 over grid points and bands).  Fails with INQAMD_ERR_ARGUMENT if the
+ * handle's scalar potential or k-point was set after the last forward.
+ * Synchronises the handle's stream. */
 int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double *grad_vloc,
                        void *lambda0);
 /* Device bytes held by the propagation object. */

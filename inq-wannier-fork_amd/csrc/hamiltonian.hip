@@ -171,6 +171,17 @@ __global__ void energies_final_kernel(const double *partial, int nchunk, int nst
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
+// Streams the three-sweep band chunks are dealt over: 2, or 3 with
+// INQAMD_STREAMS=3 (read once per process; the third stream is created only
+// then).
+static int pipeline_streams() {
+  static const int n = [] {
+    const char *e = getenv("INQAMD_STREAMS");
+    return e && atoi(e) == 3 ? 3 : 2;
+  }();
+  return n;
+}
+
 static unsigned grid_for(long total) {
   long blocks = (total + 255) / 256;
   if (blocks > 65536L * 8) blocks = 65536L * 8;
@@ -397,11 +408,8 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
   }
   // INQAMD_STREAMS=3 deals the chunks over three streams (and three slots
   // of the intermediate) where three chunks fit
-  static const int nstreams = [] {
-    const char *e = getenv("INQAMD_STREAMS");
-    return e && atoi(e) == 3 ? 3 : 2;
-  }();
-  const int ns = two && nstreams == 3 && h.aux2 && plan.n >= 3 && 3 * chunk <= h.max_bands ? 3 : 2;
+  const int ns =
+      two && pipeline_streams() == 3 && h.aux2 && plan.n >= 3 && 3 * chunk <= h.max_bands ? 3 : 2;
   auto fork = [&] {
     INQAMD_HIP_CHECK(hipEventRecord(h.ev_fork, h.stream));
     INQAMD_HIP_CHECK(hipStreamWaitEvent(h.aux, h.ev_fork, 0));
@@ -703,8 +711,10 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
       INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&h.aux, hipStreamNonBlocking));
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_fork, hipEventDisableTiming));
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_join, hipEventDisableTiming));
-      INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&h.aux2, hipStreamNonBlocking));
-      INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_join2, hipEventDisableTiming));
+      if (pipeline_streams() == 3) {
+        INQAMD_HIP_CHECK(hipStreamCreateWithFlags(&h.aux2, hipStreamNonBlocking));
+        INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_join2, hipEventDisableTiming));
+      }
       INQAMD_HIP_CHECK(hipEventCreateWithFlags(&h.ev_proj, hipEventDisableTiming));
     }
   }
@@ -814,6 +824,7 @@ int inqamd_set_kpoint(inqamd_hamiltonian_t h, const double k_cov[3]) {
   if (!h || !k_cov) return INQAMD_ERR_ARGUMENT;
   return guarded(h, [&] {
     for (int d = 0; d < 3; d++) h->k_cov[d] = k_cov[d];
+    h->generation++;
     update_current_slot(*h);
   });
 }
@@ -832,6 +843,7 @@ int inqamd_set_scalar_potential(inqamd_hamiltonian_t h, const double *vloc_devic
     INQAMD_REQUIRE(vloc_device != nullptr, "vloc is null");
     INQAMD_HIP_CHECK(hipMemcpyAsync(h->V, vloc_device, h->npoint * sizeof(double),
                                     hipMemcpyDeviceToDevice, h->stream));
+    h->generation++;
   });
 }
 

@@ -25,6 +25,10 @@ struct projector_tables {
   float *pos = nullptr;    // [nproj][max_sphere][3] contravariant, FP32
   double *mat = nullptr;   // [nproj][max_nlm][max_sphere], zero padded
   double *coeff = nullptr; // [nproj][max_nlm], zero where lm >= nlm
+  // (1 + D^2)/2 per coefficient: with it the A-derivative form of [V_nl, r]
+  // gives the current of position_commutator as the reference writes it
+  // (projector_all.hpp:362-366)
+  double *coeff_inq = nullptr;
   int *size = nullptr;     // [nproj] points (0 for locally empty spheres)
   // ownership map of the scatter: unique grid points and their contributors
   int nunique = 0;
@@ -109,6 +113,8 @@ struct inqamd_hamiltonian {
   long mom_partial_elems = 0;
 
   inqamd::projector_tables pt;
+  int commutator = 0;           // INQAMD_COMMUTATOR_* of inqamd_current
+  long generation = 0;          // bumped whenever V or k changes (stale-state checks)
 
   void *gs = nullptr;           // steepest-descent workspace and BLAS handle (gs.hip)
   void *tddft = nullptr;        // density / potential fields and the ETRS half step (tddft.hip)

@@ -230,6 +230,14 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
     }
     for (int lm = 0; lm < L && !empty; lm++) co[(long)p * max_nlm + lm] = coeff[(long)p * max_nlm + lm];
   }
+  // Im(conj p r) - D^2 Im(conj r p) = 2 ((1 + D^2)/2) Im(conj p r): every live
+  // projector has one, D = 0 included (the unscaled r-weighted projection)
+  std::vector<double> co_inq(co.size(), 0.0);
+  for (int p = 0; p < nproj; p++)
+    for (int lm = 0; lm < nlm[p] && size[p] > 0; lm++) {
+      const double d = co[(size_t)p * max_nlm + lm];
+      co_inq[(size_t)p * max_nlm + lm] = 0.5 * (1.0 + d * d);
+    }
   std::sort(owners.begin(), owners.end());
   std::vector<int> upoint, uoff, uent;
   for (size_t e = 0; e < owners.size(); e++) {
@@ -264,6 +272,7 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
   t.pos = upload(pos);
   t.mat = upload(mat);
   t.coeff = upload(co);
+  t.coeff_inq = upload(co_inq);
   t.size = upload(size);
   t.nunique = (int)upoint.size();
   t.upoint = upload(upoint);
@@ -273,7 +282,8 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
 
 void projector_free(handle &h) {
   projector_tables &t = h.pt;
-  for (void *p : {(void *)t.point, (void *)t.pos, (void *)t.mat, (void *)t.coeff, (void *)t.size,
+  for (void *p : {(void *)t.point, (void *)t.pos, (void *)t.mat, (void *)t.coeff,
+                  (void *)t.coeff_inq, (void *)t.size,
                   (void *)t.upoint, (void *)t.uoff, (void *)t.uent, (void *)t.owner,
                   (void *)t.phase_sph, (void *)t.phase_ent, (void *)t.own, (void *)t.ent})
     if (p) (void)hipFree(p);

@@ -686,14 +686,16 @@ static void gspace_record(handle &h, const cplx *Xh, const cplx *Yh, long ld, in
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 
-// record[8..25] = sphere pair sums of (mX, mY)
-static void nl_record(handle &h, const cplx *mX, const cplx *mY, int nb, double *rec) {
+// record[8..25] = sphere pair sums of (mX, mY), weighted by coeff
+// [nproj][max_nlm] (the KB coefficients, or commutator_coeff's)
+static void nl_record(handle &h, const cplx *mX, const cplx *mY, int nb, double *rec,
+                      const double *coeff) {
   const projector_tables &t = h.pt;
   double *part = obs_partial(h);
   const int nrows = t.nproj * t.max_nlm;
   if (nrows > 0) {
     hipLaunchKernelGGL(nl_pair_kernel, dim3(nrows), dim3(64), 0, h.stream, mX, mY,
-                       (const double *)t.coeff, t.max_nlm, nb, part);
+                       coeff, t.max_nlm, nb, part);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(SUM_ROWS_NT), 0, h.stream,
@@ -739,6 +741,14 @@ static void lattice_dot(const handle &h, const double d[3], double out[3]) {
     out[e] = h.lattice[3 * e + 0] * d[0] + h.lattice[3 * e + 1] * d[1] + h.lattice[3 * e + 2] * d[2];
 }
 
+// Coefficients of the current's [V_nl, r] term in the A-derivative form:
+// D itself (INQAMD_COMMUTATOR_CORRECTED) or (1 + D^2)/2, which gives the
+// current of position_commutator as the reference writes it
+// (INQAMD_COMMUTATOR_INQ, projector_all.hpp:362-366).
+static const double *commutator_coeff(const handle &h, int commutator) {
+  return commutator == INQAMD_COMMUTATOR_CORRECTED ? h.pt.coeff : h.pt.coeff_inq;
+}
+
 static pair_weights make_weights(const handle &h, const double d[3], const double kA_cov[3]) {
   pair_weights w;
   w.inv_n = 1.0 / (double)h.npoint;
@@ -764,6 +774,8 @@ struct inqamd_td {
   int nseg = 0, nb = 0;
   std::vector<double> coef;   // coefficients of the last forward
   bool have_forward = false;
+  long generation = 0;        // the handle's V/k generation at the last forward
+  long forwards = 0;          // forwards run so far
   double dir_cov[3] = {0, 0, 0};
   long bytes = 0;
   cplx *ckpt = nullptr;       // [nseg + 1][N][nst]; the last one holds psi(T)
@@ -1047,7 +1059,7 @@ void adjoint_half(inqamd_td *td, int nb, cplx *const *Y, int slot, double t_h, c
     gspace_record(h, td->gA, td->gB, nb, nb, rec);
     sphere_moments(h, td->X[j], nb, nb, L, false, h.mom_a);
     sphere_moments(h, Yp[j], nb, nb, L, false, h.mom_b);
-    nl_record(h, h.mom_a, h.mom_b, nb, rec);
+    nl_record(h, h.mom_a, h.mom_b, nb, rec, h.pt.coeff);
   }
   grad_step_args ga;
   ga.nrec = M;
@@ -1079,7 +1091,8 @@ void adjoint_start(inqamd_td *td, const cplx *psi, int nb, int slot, const doubl
   fft3d(h, psi, nb, td->gA, nb, nb, -1, 1.0);
   gspace_record(h, td->gA, td->gA, nb, nb, rec);
   sphere_moments(h, psi, nb, nb, L, true, h.mom_a);
-  nl_record(h, h.mom_a, h.mom_a, nb, rec);
+  const double *lcoeff = commutator_coeff(h, td->d.commutator);
+  nl_record(h, h.mom_a, h.mom_a, nb, rec, lcoeff);
   final_term_args fa;
   fa.occ = occ;
   fa.dV = dV;
@@ -1109,7 +1122,7 @@ void adjoint_start(inqamd_td *td, const cplx *psi, int nb, int slot, const doubl
     a.max_sphere = tb.max_sphere;
     a.max_nlm = tb.max_nlm;
     a.mat = tb.mat;
-    a.coeff = tb.coeff;
+    a.coeff = lcoeff;
     a.pos = tb.pos;
     a.size = tb.size;
     for (int e = 0; e < 3; e++) a.L0[e] = L[e];
@@ -1125,6 +1138,15 @@ void adjoint_start(inqamd_td *td, const cplx *psi, int nb, int slot, const doubl
 }  // namespace
 
 extern "C" {
+
+int inqamd_set_position_commutator(inqamd_hamiltonian_t h, int commutator) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    INQAMD_REQUIRE(commutator == INQAMD_COMMUTATOR_INQ || commutator == INQAMD_COMMUTATOR_CORRECTED,
+                   "commutator must be INQAMD_COMMUTATOR_INQ or INQAMD_COMMUTATOR_CORRECTED");
+    h->commutator = commutator;
+  });
+}
 
 int inqamd_current(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, double occ,
                    double J_cov[3], double J_cart[3]) {
@@ -1144,7 +1166,7 @@ int inqamd_current(inqamd_hamiltonian_t h, const void *psi, int nst, int ld, dou
       for (int e = 0; e < 3; e++) L[c * 3 + e] = h->lattice[e * 3 + c];
     projector_phases(*h, KA_CURRENT);
     sphere_moments(*h, f, ld, nst, L, false, h->mom_a);
-    nl_record(*h, h->mom_a, h->mom_a, nst, rec);
+    nl_record(*h, h->mom_a, h->mom_a, nst, rec, commutator_coeff(*h, h->commutator));
     double r[OBS_RECORD];
     INQAMD_HIP_CHECK(hipMemcpyAsync(r, rec, sizeof(r), hipMemcpyDeviceToHost, h->stream));
     INQAMD_HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -1179,6 +1201,9 @@ int inqamd_td_create(inqamd_hamiltonian_t h, const inqamd_td_desc *desc, inqamd_
     INQAMD_REQUIRE(d.ncoef >= 0 && d.ncoef <= 64, "ncoef must be in [0, 64]");
     INQAMD_REQUIRE(d.checkpoint_every >= 1, "checkpoint_every must be >= 1");
     INQAMD_REQUIRE(d.band_slice >= 0, "band_slice must be >= 0");
+    INQAMD_REQUIRE(d.commutator == INQAMD_COMMUTATOR_INQ ||
+                       d.commutator == INQAMD_COMMUTATOR_CORRECTED,
+                   "commutator must be INQAMD_COMMUTATOR_INQ or INQAMD_COMMUTATOR_CORRECTED");
     td->d = d;
     td->nb = d.band_slice == 0 || d.band_slice > d.nst ? d.nst : d.band_slice;
     td->nseg = (d.nsteps + d.checkpoint_every - 1) / d.checkpoint_every;
@@ -1267,7 +1292,7 @@ int inqamd_td_forward(inqamd_td_t td, const double *coeff, const void *psi0, int
     fft3d(*h, state, nst, h->work, nst, nst, -1, 1.0);
     gspace_record(*h, h->work, h->work, nst, nst, rec);
     sphere_moments(*h, state, nst, nst, L, false, h->mom_a);
-    nl_record(*h, h->mom_a, h->mom_a, nst, rec);
+    nl_record(*h, h->mom_a, h->mom_a, nst, rec, commutator_coeff(*h, d.commutator));
     pair_weights w = make_weights(*h, d.direction, kT);
     hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(64), 0, h->stream, (const double *)rec, w, d.occ,
                        td->loss);
@@ -1280,6 +1305,8 @@ int inqamd_td_forward(inqamd_td_t td, const double *coeff, const void *psi0, int
                                     h->stream));
     INQAMD_HIP_CHECK(hipStreamSynchronize(h->stream));
     td->have_forward = true;
+    td->generation = h->generation;
+    td->forwards++;
   });
 }
 
@@ -1289,6 +1316,9 @@ int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double 
   inqamd_hamiltonian *h = td->h;
   return guarded(h, [&] {
     INQAMD_REQUIRE(td->have_forward, "backward needs a forward first");
+    INQAMD_REQUIRE(td->generation == h->generation,
+                   "the Hamiltonian's scalar potential or k-point changed since the forward; "
+                   "run the forward again before the backward");
     INQAMD_REQUIRE(grad_coeff || td->d.ncoef == 0, "grad_coeff is null");
     if (lambda0) check_orbital(*h, lambda0, "lambda0");
     const inqamd_td_desc &d = td->d;
@@ -1358,6 +1388,8 @@ int inqamd_td_backward(inqamd_td_t td, double dloss, double *grad_coeff, double 
 }
 
 long inqamd_td_memory(inqamd_td_t td) { return td ? td->bytes : 0; }
+
+long inqamd_td_forward_count(inqamd_td_t td) { return td ? td->forwards : -1; }
 
 void inqamd_td_destroy(inqamd_td_t td) {
   if (!td) return;

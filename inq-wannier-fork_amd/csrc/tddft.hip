@@ -528,6 +528,7 @@ void ks_potential(handle &h, const double *vion, const double *rho, bool hartree
   double s[4];
   reduce4(h, h.npoint, s, ks_kernel, vion, rho, hartree ? (const double *)t.vh : nullptr,
           xc ? (const double *)t.vxc : nullptr, h.npoint, h.V);
+  h.generation++;
   energies[0] = dV * s[0];
   energies[1] = hartree ? 0.5 * dV * s[1] : 0.0;
 }

@@ -25,6 +25,15 @@ ERR_HIP = 2
 ERR_MEMORY = 3
 ERR_UNSUPPORTED = 4
 
+# INQAMD_COMMUTATOR_*: the [V_nl, r] form of the current (inq_amd.h)
+COMMUTATORS = {"inq": 0, "corrected": 1}
+
+
+def commutator_code(name):
+    if name not in COMMUTATORS:
+        raise ValueError(f"commutator must be one of {sorted(COMMUTATORS)}, not {name!r}")
+    return COMMUTATORS[name]
+
 
 class SystemDesc(ctypes.Structure):
     """inqamd_system_desc."""
@@ -72,6 +81,7 @@ class TDDesc(ctypes.Structure):
         ("checkpoint_every", ctypes.c_int),
         ("band_slice", ctypes.c_int),
         ("store_powers", ctypes.c_int),
+        ("commutator", ctypes.c_int),
     ]
 
 
@@ -83,6 +93,8 @@ SIGNATURES = [
     ("inqamd_last_error", ctypes.c_char_p, [_vp]),
     ("inqamd_version", ctypes.c_char_p, []),
     ("inqamd_selftest_mfma", _i, [_vp, _vp]),
+    ("inqamd_stream_probe", _i, [_i, _vp, _vp, ctypes.c_long, _i, _i, _i, _i, _i, _vp,
+                                 ctypes.POINTER(ctypes.c_float)]),
     ("inqamd_set_kpoint", _i, [_vp, _c_dbl_p]),
     ("inqamd_set_uniform_vector_potential", _i, [_vp, _c_dbl_p]),
     ("inqamd_set_scalar_potential", _i, [_vp, _vp]),
@@ -120,11 +132,13 @@ SIGNATURES = [
     ("inqamd_etrs_graph_launch", _i, [_vp, _c_dbl_p, _c_dbl_p]),
     ("inqamd_etrs_graph_set_step", _i, [_vp, ctypes.c_long]),
     ("inqamd_graph_destroy", None, [_vp]),
+    ("inqamd_set_position_commutator", _i, [_vp, _i]),
     ("inqamd_current", _i, [_vp, _vp, _i, _i, _d, _c_dbl_p, _c_dbl_p]),
     ("inqamd_td_create", _i, [_vp, ctypes.POINTER(TDDesc), ctypes.POINTER(_vp)]),
     ("inqamd_td_forward", _i, [_vp, _c_dbl_p, _vp, _i, _vp, _c_dbl_p]),
     ("inqamd_td_backward", _i, [_vp, _d, _c_dbl_p, _vp, _vp]),
     ("inqamd_td_memory", ctypes.c_long, [_vp]),
+    ("inqamd_td_forward_count", ctypes.c_long, [_vp]),
     ("inqamd_td_destroy", None, [_vp]),
     ("inqamd_batch_create", _i, [_vp, _i, _vp, ctypes.POINTER(_vp)]),
     ("inqamd_batch_apply", _i, [_vp, _vp, _vp, _vp, _vp]),

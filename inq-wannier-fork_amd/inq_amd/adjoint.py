@@ -25,8 +25,13 @@ class propagation_adjoint:
     """Owns an inqamd_td_t bound to a ks_hamiltonian."""
 
     def __init__(self, ham, nst, nsteps, dt, ncoef, omega, direction=(1.0, 0.0, 0.0), order=4,
-                 occ=2.0, checkpoint_every=20, band_slice=0, store_powers=True):
+                 occ=2.0, checkpoint_every=20, band_slice=0, store_powers=True, commutator=None):
+        """commutator: the loss's [V_nl, r] form, "inq" or "corrected"
+        (default: the Hamiltonian's position_commutator)."""
         self.ham = ham
+        if commutator is None:
+            commutator = ham.position_commutator
+        self.commutator = commutator
         d = _lib.TDDesc()
         d.nst = int(nst)
         d.nsteps = int(nsteps)
@@ -39,6 +44,7 @@ class propagation_adjoint:
         d.checkpoint_every = int(checkpoint_every)
         d.band_slice = int(band_slice)
         d.store_powers = 1 if store_powers else 0
+        d.commutator = _lib.commutator_code(commutator)
         self.desc = d
         t = ctypes.c_void_p()
         ham.handle.call("inqamd_td_create", ctypes.byref(d), ctypes.byref(t))
@@ -49,6 +55,11 @@ class propagation_adjoint:
     @property
     def memory_bytes(self):
         return int(_lib.lib().inqamd_td_memory(self._t))
+
+    @property
+    def forward_count(self):
+        """Forwards run so far; the backward differentiates the last one."""
+        return int(_lib.lib().inqamd_td_forward_count(self._t))
 
     def _set_stream(self):
         h = self.ham.handle
@@ -109,12 +120,19 @@ class TDCurrentLoss(torch.autograd.Function):
         prop.ham.set_scalar_potential(vloc.reshape(-1))
         loss, _ = prop.forward(coef.detach().cpu().numpy(), psi0)
         ctx.prop = prop
+        ctx.forward_id = prop.forward_count
         ctx.cinfo = (coef.dtype, coef.device)
         ctx.vshape = vloc.shape
         return torch.tensor(loss, dtype=torch.float64, device=coef.device)
 
     @staticmethod
     def backward(ctx, g):
+        # the propagation object keeps only its last forward's checkpoints
+        if ctx.prop.forward_count != ctx.forward_id:
+            raise RuntimeError(
+                "td_current_loss: another forward ran on this propagation object after this "
+                "loss's forward, so its checkpoints are gone; give each live loss its own "
+                "propagation_adjoint or call backward before the next forward")
         gc, gV, _ = ctx.prop.backward(float(g))
         dt, dev = ctx.cinfo
         return (torch.from_numpy(np.array(gc)).to(device=dev, dtype=dt), gV.reshape(ctx.vshape),

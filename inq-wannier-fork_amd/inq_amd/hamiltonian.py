@@ -124,6 +124,7 @@ class ks_hamiltonian:
         self._A = np.zeros(3)
         self._k = np.zeros(3)
         self.kpoint = kpoint
+        self._commutator = "inq"
 
     @property
     def handle(self):
@@ -151,10 +152,34 @@ class ks_hamiltonian:
         self._k = np.asarray(k_cov, float).reshape(3).copy()
         self._h.call("inqamd_set_kpoint", _lib.dvec(self._k))
 
+    @property
+    def position_commutator(self):
+        """The current's [V_nl, r] form: "inq" (default; projector_all.hpp:
+        319-403 as written, the plain projection scaled by the KB coefficient
+        twice) or "corrected" (each projection scaled once: J = -dE/dA)."""
+        return self._commutator
+
+    @position_commutator.setter
+    def position_commutator(self, name):
+        self._h.call("inqamd_set_position_commutator", _lib.commutator_code(name))
+        self._commutator = name
+
     def set_scalar_potential(self, vloc):
         v = vloc.to(dtype=torch.float64).contiguous()
         self._h.call("inqamd_set_scalar_potential", ctypes.c_void_p(v.data_ptr()))
         self._vkeep = v
+
+    def scalar_potential(self, device=None):
+        """The scalar potential last given to the Hamiltonian from the host side
+        (the system's V, or the last set_scalar_potential), as a float64 device
+        tensor [N] (ks_hamiltonian.hpp:200-202).  A potential the library
+        updates itself (tddft.ks_potential) is not reflected here."""
+        v = getattr(self, "_vkeep", None)
+        if v is None:
+            dev = device or torch.device("cuda", torch.cuda.current_device())
+            v = torch.from_numpy(np.ascontiguousarray(np.ravel(self.system.vloc), np.float64)).to(dev)
+            self._vkeep = v
+        return v
 
     def __call__(self, phi, out=None, fourier=False):
         """H phi for real-space orbitals (ks_hamiltonian.hpp:132-152) or, with

@@ -1,14 +1,16 @@
-"""real_time:: mirror (src/real_time/): the ETRS step and a frozen-potential
-propagation loop.
+"""real_time:: mirror (src/real_time/): the ETRS step, the propagation loop
+with its per-step energy and observables, and hamiltonian::energy.
 
 Perturbations follow INQ's duck-typed interface (perturbations/none.hpp:20-55):
 an object with has_uniform_vector_potential() and uniform_vector_potential(t)
 returning the Cartesian A(t); self_consistency converts it to covariant
-components (self_consistency.hpp:152-156).
+components (self_consistency.hpp:152-156).  An optional zero_step(phi) runs
+before a propagation that starts at step 0 (propagate.hpp:63-65).
 """
 import ctypes
 
 import numpy as np
+import torch
 
 from . import _lib
 from .hamiltonian import _orbital_args, _stream_ptr
@@ -36,14 +38,171 @@ def etrs(ham, phi, time, dt, pert=None, order=4):
     return phi
 
 
-def propagate(ham, phi, dt, num_steps, pert=None, start_step=0, order=4, callback=None):
-    """real_time::propagate's step loop (propagate.hpp:93-133) without the
-    energy/observable bookkeeping; `callback(step, time, phi)` after each step."""
+class energy:
+    """hamiltonian::energy (energy.hpp:19-200): the terms the propagation
+    tracks; kinetic() and total() combine them as the reference does."""
+
+    def __init__(self):
+        self.ion = 0.0
+        self.ion_kinetic = 0.0
+        self.eigenvalues = 0.0
+        self.external = 0.0
+        self.non_local = 0.0
+        self.hartree = 0.0
+        self.xc = 0.0
+        self.nvxc = 0.0
+        self.exact_exchange = 0.0
+
+    def calculate(self, ham, phi, occ):
+        """energy::calculate (energy.hpp:59-107): one H application for the
+        eigenvalues and residuals, the non-local expectation values, and the
+        occupation-weighted sums.  Returns the per-band residual norms."""
+        nst = phi.shape[-1]
+        w = torch.as_tensor(np.broadcast_to(np.asarray(occ, np.float64), (nst,)).copy(),
+                            dtype=torch.float64, device=phi.device)
+        eps, normres, nl = ham.energy(phi)
+        self.eigenvalues = float((w * eps).sum())
+        self.non_local = float((w * nl).sum())
+        self.exact_exchange = 0.0
+        self.band_eigenvalues = eps
+        return normres
+
+    def kinetic(self):
+        return (self.eigenvalues - 2.0 * self.hartree - self.nvxc - 2.0 * self.exact_exchange
+                - self.external - self.non_local)
+
+    def total(self):
+        return (self.kinetic() + self.hartree + self.external + self.non_local + self.xc
+                + self.exact_exchange + self.ion + self.ion_kinetic)
+
+
+class viewables:
+    """What a propagation callback sees (viewables.hpp:27-130)."""
+
+    def __init__(self, last_iter, it, time, ham, phi, occ, en, pert):
+        self._last = bool(last_iter)
+        self._iter = int(it)
+        self._time = float(time)
+        self._ham = ham
+        self._phi = phi
+        self._occ = occ
+        self._energy = en
+        self._pert = pert
+
+    def iter(self):
+        return self._iter
+
+    def last_iter(self):
+        return self._last
+
+    def every(self, every_iter):
+        if self._iter == 0:
+            return False
+        return self._iter % every_iter == 0 or self._last
+
+    def time(self):
+        return self._time
+
+    def energy(self):
+        return self._energy
+
+    def orbitals(self):
+        return self._phi
+
+    def uniform_vector_potential(self):
+        """Cartesian A of the Hamiltonian (viewables.hpp:94-96)."""
+        lat = np.asarray(self._ham.system.lattice, float)
+        return np.linalg.solve(lat, self._ham.uniform_vector_potential)
+
+    def current(self):
+        """Cartesian total current (viewables.hpp:100-102) with the
+        Hamiltonian's position_commutator form; bands of equal occupation."""
+        occ = np.asarray(self._occ, float)
+        if occ.ndim and not np.all(occ == occ.flat[0]):
+            raise ValueError("current() needs equally occupied bands")
+        return self._ham.current(self._phi, float(occ.flat[0]) if occ.ndim else float(occ))[1]
+
+    def density(self):
+        from . import tddft
+        return tddft.density(self._ham, self._phi, self._occ)
+
+    def dipole(self):
+        """Electronic dipole of the density (observables::dipole, dipole.hpp:24-40)."""
+        from . import tddft
+        return tddft.dipole(self._ham, self.density())
+
+    def num_electrons(self):
+        rho = self.density()
+        return float(rho.sum()) * self._ham.system.volume / rho.numel()
+
+
+def _set_vector_potential(ham, pert, time):
+    # self_consistency::update_hamiltonian's vector-potential part (:152-156)
+    if pert is not None and pert.has_uniform_vector_potential():
+        ham.uniform_vector_potential = _to_covariant(ham.system.lattice,
+                                                     pert.uniform_vector_potential(time))
+    else:
+        ham.uniform_vector_potential = np.zeros(3)
+
+
+def propagate(ham, phi, dt, num_steps, pert=None, start_step=0, order=4, callback=None, occ=2.0,
+              theory=None, vion=None, ion_energy=0.0):
+    """real_time::propagate (propagate.hpp:33-133) with ETRS for static ions.
+
+    Before the loop the Hamiltonian is brought to t = start_step dt (A(t)
+    and, with a theory, V_KS of phi's density) and the energy is calculated;
+    a propagation from step 0 runs pert.zero_step(phi) first and reports
+    step 0 to the callback.  Each step then runs etrs (the self-consistent
+    one when `theory` is given), energy.calculate (one more H application)
+    and callback(viewables).  `theory` is None (frozen potential, the
+    Hamiltonian's V as the ionic one) or a dict(hartree=bool, xc=False |
+    'lda' | 'pbe') with `vion` the ionic potential (a float64 device field).
+    `occ` is one occupation or one per band.  Returns the final energy.
+    """
+    from . import tddft
+    if start_step < 0:
+        raise ValueError("start_step must be >= 0")
+    nst = phi.shape[-1]
+    occ_arr = np.broadcast_to(np.asarray(occ, np.float64), (nst,)).copy()
+    dV = ham.system.volume / float(np.prod(ham.n))
+    en = energy()
+    en.ion = float(ion_energy)
+    interacting = theory is not None
+    if interacting:
+        if vion is None:
+            raise ValueError("an interacting propagation needs the ionic potential vion")
+        hartree = bool(theory.get("hartree", True))
+        xc = theory.get("xc", "pbe")
+    if start_step == 0 and pert is not None and hasattr(pert, "zero_step"):
+        pert.zero_step(phi)
+
+    def update_hamiltonian(time):
+        _set_vector_potential(ham, pert, time)
+        rho = tddft.density(ham, phi, occ_arr)
+        if interacting:
+            e = tddft.ks_potential(ham, vion, rho, hartree, xc)
+            en.external, en.hartree, en.xc, en.nvxc = (e["external"], e["hartree"], e["exc"],
+                                                       e["nvxc"])
+        else:
+            v = ham.scalar_potential(phi.device)
+            en.external = dV * float((rho.reshape(-1) * v).sum())
+
+    update_hamiltonian(start_step * dt)
+    en.calculate(ham, phi, occ_arr)
+    if start_step == 0 and callback is not None:
+        callback(viewables(False, start_step, start_step * dt, ham, phi, occ_arr, en, pert))
     for istep in range(start_step, num_steps):
-        etrs(ham, phi, istep * dt, dt, pert, order)
+        if interacting:
+            tddft.etrs(ham, phi, vion, istep * dt, dt, occ_arr, pert, order, hartree, xc)
+        else:
+            etrs(ham, phi, istep * dt, dt, pert, order)
+        # the step leaves H at t + dt; its energy terms are those of the new density
+        update_hamiltonian((istep + 1) * dt)
+        en.calculate(ham, phi, occ_arr)
         if callback is not None:
-            callback(istep + 1, (istep + 1) * dt, phi)
-    return phi
+            callback(viewables(istep == num_steps - 1, istep + 1, (istep + 1.0) * dt, ham, phi,
+                               occ_arr, en, pert))
+    return en
 
 
 class etrs_graph:

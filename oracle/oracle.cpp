@@ -387,9 +387,13 @@ struct pcg32 {
 };
 
 /* gradient(phi, -1, k+A) then current density (current.hpp:34-42) plus the
- * [V_nl, r] term; returns J_cov. */
+ * [V_nl, r] term; returns J_cov.  commutator 0 is position_commutator as the
+ * reference writes it (projector_all.hpp:319-403: the plain projection scaled
+ * by the KB coefficient twice, :362-366, the r-weighted one not at all);
+ * commutator 1 scales both projections once, which makes the term exactly
+ * -dV_nl/dA. */
 static void current(const orc_system *s, const double kA_cov[3], double occ, int nst, int ld,
-                    const cplx *psi, double J[3]) {
+                    const cplx *psi, int commutator, double J[3]) {
   cell c(s->lattice);
   const long N = npoints(s);
   std::vector<cplx> fs((size_t)N * nst);
@@ -442,8 +446,14 @@ static void current(const orc_system *s, const double kA_cov[3], double occ, int
       for (int lm = 0; lm < s->max_nlm; lm++) {
         double co = s->coeff[(long)ip * s->max_nlm + lm];
         for (int b = 0; b < nst; b++) {
-          P[lm * nst + b] *= co;
-          for (int d = 0; d < 3; d++) R[(lm * 3 + d) * nst + b] *= co;
+          if (commutator == 0) {
+            // projector_all.hpp:362-366: proj *= coe; proj *= coe; rproj untouched
+            P[lm * nst + b] *= co;
+            P[lm * nst + b] *= co;
+          } else {
+            P[lm * nst + b] *= co;
+            for (int d = 0; d < 3; d++) R[(lm * 3 + d) * nst + b] *= co;
+          }
         }
       }
       for (long i = 0; i < maxK; i++) {
@@ -1188,9 +1198,9 @@ int orc_randomize(const int n[3], int nst, int index, double *psi_) {
 }
 
 int orc_current(const orc_system *sys, const double kA_cov[3], double occ, int nst, int ld,
-                const double *psi, double J_cov[3], double J_cart[3]) {
-  if (!sys || nst <= 0 || ld < nst) return 1;
-  current(sys, kA_cov, occ, nst, ld, reinterpret_cast<const cplx *>(psi), J_cov);
+                const double *psi, int commutator, double J_cov[3], double J_cart[3]) {
+  if (!sys || nst <= 0 || ld < nst || commutator < 0 || commutator > 1) return 1;
+  current(sys, kA_cov, occ, nst, ld, reinterpret_cast<const cplx *>(psi), commutator, J_cov);
   cell c(sys->lattice);
   vec3 jc{{J_cov[0], J_cov[1], J_cov[2]}};
   vec3 cart = c.cov_to_cart(jc);
@@ -1200,8 +1210,8 @@ int orc_current(const orc_system *sys, const double kA_cov[3], double occ, int n
 
 int orc_td_current(const orc_system *sys, int ncoef, const double *coef, double omega,
                    const double direction[3], double dt, int nsteps, int order, double occ,
-                   int nst, int ld, double *psi_, double *loss) {
-  if (!sys || nst <= 0 || ld < nst) return 1;
+                   int nst, int ld, double *psi_, int commutator, double *loss) {
+  if (!sys || nst <= 0 || ld < nst || commutator < 0 || commutator > 1) return 1;
   cell c(sys->lattice);
   cplx *psi = reinterpret_cast<cplx *>(psi_);
   auto A_cov = [&](double t, double out[3]) {
@@ -1221,7 +1231,7 @@ int orc_td_current(const orc_system *sys, int ncoef, const double *coef, double 
   }
   double AN[3], Jc[3], Jcart[3];
   A_cov(nsteps > 0 ? (nsteps - 1) * dt + dt : 0.0, AN);
-  orc_current(sys, AN, occ, nst, ld, psi_, Jc, Jcart);
+  orc_current(sys, AN, occ, nst, ld, psi_, commutator, Jc, Jcart);
   *loss = Jcart[0] * direction[0] + Jcart[1] * direction[1] + Jcart[2] * direction[2];
   return 0;
 }

@@ -159,19 +159,22 @@ int orc_randomize(const int n[3], int nst, int index, double *psi);
 
 /* observables::current (src/observables/current.hpp:25-53):
  * j = sum_b occ Im(conj(phi) g phi), g phi = -i (G + k + A) phi + [V_nl, r] phi
- * (gradient.hpp:43-48 with factor -1; projector_all.hpp:319-403 with the KB
- * coefficient applied once to each projection).  J_cov = dV sum_r j;
- * J_cart = to_cartesian(J_cov) (viewables.hpp:100-102). */
+ * (gradient.hpp:43-48 with factor -1; projector_all.hpp:319-403).
+ * commutator 0: position_commutator as written (the plain projection scaled by
+ * the KB coefficient twice, the r-weighted one unscaled, :362-366);
+ * commutator 1: each projection scaled once (the A-derivative of V_nl).
+ * J_cov = dV sum_r j; J_cart = to_cartesian(J_cov) (viewables.hpp:100-102). */
 int orc_current(const orc_system *sys, const double kA_cov[3], double occ, int nst, int ld,
-                const double *psi, double J_cov[3], double J_cart[3]);
+                const double *psi, int commutator, double J_cov[3], double J_cart[3]);
 
 /* Frozen-potential propagation under A(t) = direction * sum_{j=1..n} c_j
  * sin(j w t) (Cartesian): nsteps ETRS steps from psi (in place, A(t) evaluated
  * at t = step*dt and t + dt as etrs.hpp:54), then returns the final current
- * projected on `direction` (Cartesian), with occupation occ per band. */
+ * projected on `direction` (Cartesian), with occupation occ per band and the
+ * commutator form of orc_current. */
 int orc_td_current(const orc_system *sys, int ncoef, const double *coef, double omega,
                    const double direction[3], double dt, int nsteps, int order, double occ,
-                   int nst, int ld, double *psi, double *loss);
+                   int nst, int ld, double *psi, int commutator, double *loss);
 
 /* Set the number of OpenMP threads the oracle uses (<=0: library default). */
 /* operations::laplacian_add, G-space (laplacian.hpp:24-38) */

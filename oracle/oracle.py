@@ -78,10 +78,10 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.orc_randomize.argtypes = [_c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.orc_current.argtypes = [sysp, _c_dbl_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                  ctypes.c_void_p, _c_dbl_p, _c_dbl_p]
+                                  ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p]
         L.orc_td_current.argtypes = [sysp, ctypes.c_int, _c_dbl_p, ctypes.c_double, _c_dbl_p,
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]
         L.orc_laplacian_add.argtypes = [sysp, ctypes.c_double, _c_dbl_p, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
@@ -412,24 +412,37 @@ def randomize(n, nst, index=0):
     return out
 
 
-def _system_current(self, psi, kA=(0.0, 0.0, 0.0), occ=2.0):
+COMMUTATORS = {"inq": 0, "corrected": 1}
+
+
+def _commutator(name):
+    if name not in COMMUTATORS:
+        raise ValueError(f"commutator must be one of {sorted(COMMUTATORS)}, not {name!r}")
+    return COMMUTATORS[name]
+
+
+def _system_current(self, psi, kA=(0.0, 0.0, 0.0), occ=2.0, commutator="inq"):
+    """observables::current; commutator "inq" is position_commutator as the
+    reference writes it, "corrected" scales both projections once."""
     psi = np.ascontiguousarray(psi, np.complex128)
     nst = psi.shape[-1]
     jc = (ctypes.c_double * 3)()
     jx = (ctypes.c_double * 3)()
-    _check(lib().orc_current(ctypes.byref(self.s), _dvec(kA), float(occ), nst, nst, _ptr(psi), jc,
-                             jx), "current")
+    _check(lib().orc_current(ctypes.byref(self.s), _dvec(kA), float(occ), nst, nst, _ptr(psi),
+                             _commutator(commutator), jc, jx), "current")
     return np.array(list(jc)), np.array(list(jx))
 
 
-def _system_td_current(self, psi, coef, omega, direction, dt, nsteps, order=4, occ=2.0):
+def _system_td_current(self, psi, coef, omega, direction, dt, nsteps, order=4, occ=2.0,
+                       commutator="inq"):
     psi = np.array(psi, dtype=np.complex128, order="C", copy=True)
     nst = psi.shape[-1]
     c = np.ascontiguousarray(coef, np.float64)
     loss = ctypes.c_double(0.0)
     _check(lib().orc_td_current(ctypes.byref(self.s), c.size, c.ctypes.data_as(_c_dbl_p),
                                 float(omega), _dvec(direction), float(dt), int(nsteps), int(order),
-                                float(occ), nst, nst, _ptr(psi), ctypes.byref(loss)), "td_current")
+                                float(occ), nst, nst, _ptr(psi), _commutator(commutator),
+                                ctypes.byref(loss)), "td_current")
     return loss.value, psi
 
 

@@ -518,3 +518,52 @@ def test_aa_mfma_f64_lane_layout(device):
     A = 1.0 + i + 16.0 * k
     B = 2.0 + 3.0 * np.arange(16)[None, :] - 5.0 * np.arange(4)[:, None]
     assert np.array_equal(got, A @ B)
+
+
+@pytest.mark.parametrize("env", [{"INQAMD_STREAMS": "3"}, {"INQAMD_BAND_GROUP_SIZE": "16"},
+                                 {"INQAMD_BAND_GROUP_SIZE": "32", "INQAMD_BAND_CHUNK": "64"}],
+                         ids=["three-streams", "groups-16", "groups-32-chunk-64"])
+def test_pipeline_variants_in_fresh_process(device, env):
+    """The library reads these switches once per process, so each variant runs
+    in a child process: grouped chunks (ld = 256) against contiguous ones
+    (ld = 264) bitwise for Hpsi, an eager and a captured ETRS step, plus Hpsi
+    against the oracle (tests/env_variant_check.py)."""
+    import os
+    import subprocess
+    say("pipeline variant", env)
+    e = dict(os.environ)
+    e.update(env)
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "env_variant_check.py")
+    r = subprocess.run([sys.executable, "-u", script], env=e, capture_output=True, text=True,
+                       timeout=240)
+    say(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "ENV-VARIANT-OK" in r.stdout
+
+
+def test_band_groups_96_match_padded_rows(device):
+    """96^3 x 512 (config 4's shape, 216 atoms): the default chunks of 24
+    bands take interleaved 8-band groups at ld = 512 and contiguous ranges at
+    ld = 520; Hpsi must agree bitwise, and two bands against the oracle."""
+    s = systems.diamond_system(3)
+    nst, ldp = 512, 520
+    say("band groups 96^3 x 512 vs ld 520")
+    g = torch.Generator(device=device).manual_seed(96)
+    d = torch.randn(tuple(s.n) + (nst,), dtype=torch.complex128, device=device, generator=g)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
+    h = ham(d)
+    pad = torch.zeros(tuple(s.n) + (ldp,), dtype=torch.complex128, device=device)
+    pad[..., :nst] = d
+    del d
+    hp = torch.zeros_like(pad)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    ham.handle.call("inqamd_hamiltonian_apply", ptr(pad), ptr(hp), nst, ldp)
+    torch.cuda.synchronize()
+    assert torch.equal(h, hp[..., :nst])
+    assert torch.count_nonzero(hp[..., nst:]).item() == 0
+    two = h[..., [3, 509]].cpu().numpy()
+    x = np.ascontiguousarray(pad[..., [3, 509]].cpu().numpy())
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(x)
+    err = relerr(two, ref)
+    say("96^3 Hpsi vs oracle on 2 bands %.2e" % err)
+    assert err < TOL
+    ham.close()

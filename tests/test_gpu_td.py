@@ -1,7 +1,10 @@
 """GPU parity of the current observable and of the propagation adjoint.
 
-Current: the HIP library against the oracle (1e-12 relative to |J|) and
-against the reference's own known answers (current.hpp:80-177, 1e-9 abs).
+Current: the HIP library against the oracle (1e-12 relative to |J|) in both
+forms of the [V_nl, r] term -- "inq", position_commutator as the reference
+writes it (projector_all.hpp:362-366), and "corrected" -- on config 0 (8
+atoms with projectors, k + A != 0), and against the reference's own known
+answers (current.hpp:80-177, 1e-9 abs).
 Propagation: the forward loss against the oracle's td_current (1e-11
 relative); the adjoint gradients (coefficients, V at sampled points, and the
 adjoint state at t = 0 along a random direction) against central finite
@@ -40,17 +43,31 @@ def cfg0():
     return s, psi, O.System(s.n, s.lattice, s.vloc, s.tables)
 
 
+@pytest.mark.parametrize("mode", ["inq", "corrected"])
 @pytest.mark.parametrize("kA", [(0.0, 0.0, 0.0), (0.05, -0.02, 0.03)])
-def test_current_matches_oracle(device, cfg0, kA):
-    say("current config0", kA)
+def test_current_matches_oracle(device, cfg0, kA, mode):
+    say("current config0", kA, mode)
     s, psi, osys = cfg0
     ham = hamiltonian.ks_hamiltonian(s, 16)
     ham.uniform_vector_potential = kA
+    assert ham.position_commutator == "inq"
+    ham.position_commutator = mode
     jc, jx = ham.current(to_dev(psi, device), occ=2.0)
-    rc, rx = osys.current(psi, kA, 2.0)
+    rc, rx = osys.current(psi, kA, 2.0, commutator=mode)
+    say("current", mode, jx, "oracle", rx)
     assert np.abs(jx - rx).max() <= 1e-12 * np.abs(rx).max()
     assert np.abs(jc - rc).max() <= 1e-12 * np.abs(rc).max()
     ham.close()
+
+
+def test_current_forms_differ_on_config0(device, cfg0):
+    """The two commutator forms give different currents when projectors
+    overlap psi (so the parity test above tells them apart)."""
+    s, psi, osys = cfg0
+    kA = (0.05, -0.02, 0.03)
+    a = osys.current(psi, kA, 2.0, commutator="inq")[1]
+    b = osys.current(psi, kA, 2.0, commutator="corrected")[1]
+    assert np.abs(a - b).max() > 1e-8 * np.abs(a).max()
 
 
 @pytest.mark.parametrize("case", ["gamma", "gamma_A", "k_quarter", "k_quarter_A"])
@@ -94,16 +111,29 @@ PROP = dict(nsteps=5, dt=0.02, omega=0.9, direction=(0.6, 0.0, 0.8), order=4, oc
 COEF = np.array([0.04, -0.03, 0.025])
 
 
-@pytest.fixture(scope="module", params=["pencil", "cubic"])
+class Small:
+    def __init__(self, kind, mode):
+        self.s, self.psi = _small_case(kind)
+        self.mode = mode
+        self.osys = O.System(self.s.n, self.s.lattice, self.s.vloc, self.s.tables)
+
+    def __iter__(self):
+        return iter((self.s, self.psi, self.osys))
+
+
+# the loss in both commutator forms: "inq" on both Hpsi paths, "corrected" on one
+@pytest.fixture(scope="module", params=[("pencil", "inq"), ("cubic", "inq"),
+                                        ("cubic", "corrected")],
+                ids=["pencil-inq", "cubic-inq", "cubic-corrected"])
 def small(request):
-    s, psi = _small_case(request.param)
-    return s, psi, O.System(s.n, s.lattice, s.vloc, s.tables)
+    return Small(*request.param)
 
 
-def _oracle_loss(s, psi, coef, vloc=None):
+def _oracle_loss(s, psi, coef, vloc=None, mode="inq"):
     osys = O.System(s.n, s.lattice, s.vloc if vloc is None else vloc, s.tables)
     loss, _ = osys.td_current(psi, coef, PROP["omega"], PROP["direction"], PROP["dt"],
-                              PROP["nsteps"], PROP["order"], PROP["occ"])
+                              PROP["nsteps"], PROP["order"], PROP["occ"],
+                              commutator=mode)
     return loss
 
 
@@ -116,14 +146,15 @@ def _prop(ham, nst, **kw):
 
 
 def test_td_forward_loss_matches_oracle(device, small):
-    say("td forward small")
+    say("td forward small", small.mode)
     s, psi, osys = small
     ham = hamiltonian.ks_hamiltonian(s, 3)
-    prop = _prop(ham, 3)
+    prop = _prop(ham, 3, commutator=small.mode)
     psiT = torch.empty_like(to_dev(psi, device))
     loss, _ = prop.forward(COEF, to_dev(psi, device), psiT)
     ref, ref_psi = osys.td_current(psi, COEF, PROP["omega"], PROP["direction"], PROP["dt"],
-                                   PROP["nsteps"], PROP["order"], PROP["occ"])
+                                   PROP["nsteps"], PROP["order"], PROP["occ"],
+                                   commutator=small.mode)
     torch.cuda.synchronize()
     err = np.linalg.norm(psiT.cpu().numpy() - ref_psi) / np.linalg.norm(ref_psi)
     assert err < 1e-12
@@ -136,7 +167,9 @@ def test_td_forward_loss_matches_oracle(device, small):
 def small_grads(device, small):
     s, psi, _ = small
     ham = hamiltonian.ks_hamiltonian(s, 3)
+    ham.position_commutator = small.mode
     prop = _prop(ham, 3)
+    assert prop.commutator == small.mode
     prop.forward(COEF, to_dev(psi, device))
     gc, gV, lam = prop.backward(1.0, want_lambda0=True)
     torch.cuda.synchronize()
@@ -147,7 +180,7 @@ def small_grads(device, small):
 
 
 def test_adjoint_coefficients_match_oracle_fd(device, small, small_grads):
-    say("adjoint dL/dc vs oracle FD")
+    say("adjoint dL/dc vs oracle FD", small.mode)
     s, psi, _ = small
     gc = small_grads[0]
     h = 1e-5
@@ -155,7 +188,8 @@ def test_adjoint_coefficients_match_oracle_fd(device, small, small_grads):
     for j in range(COEF.size):
         e = np.zeros_like(COEF)
         e[j] = h
-        fd[j] = (_oracle_loss(s, psi, COEF + e) - _oracle_loss(s, psi, COEF - e)) / (2 * h)
+        fd[j] = (_oracle_loss(s, psi, COEF + e, mode=small.mode) -
+                 _oracle_loss(s, psi, COEF - e, mode=small.mode)) / (2 * h)
     scale = np.abs(fd).max()
     assert scale > 1e-6
     assert np.abs(gc - fd).max() <= 1e-6 * scale, (gc, fd)
@@ -176,7 +210,8 @@ def test_adjoint_potential_matches_oracle_fd(device, small, small_grads):
         vm = s.vloc.copy()
         vp[pt] += h
         vm[pt] -= h
-        fd = (_oracle_loss(s, psi, COEF, vp) - _oracle_loss(s, psi, COEF, vm)) / (2 * h)
+        fd = (_oracle_loss(s, psi, COEF, vp, mode=small.mode) -
+              _oracle_loss(s, psi, COEF, vm, mode=small.mode)) / (2 * h)
         assert abs(gV[pt] - fd) <= 1e-6 * scale, (pt, gV[pt], fd)
 
 
@@ -188,7 +223,8 @@ def test_adjoint_initial_state_matches_oracle_fd(device, small, small_grads):
     d = rng.standard_normal(psi.shape) + 1j * rng.standard_normal(psi.shape)
     d *= np.linalg.norm(psi) / np.linalg.norm(d)
     h = 1e-5
-    fd = (_oracle_loss(s, psi + h * d, COEF) - _oracle_loss(s, psi - h * d, COEF)) / (2 * h)
+    fd = (_oracle_loss(s, psi + h * d, COEF, mode=small.mode) -
+          _oracle_loss(s, psi - h * d, COEF, mode=small.mode)) / (2 * h)
     an = float(np.real(np.vdot(lam, d)))
     assert abs(an - fd) <= 1e-6 * max(abs(fd), 1e-3), (an, fd)
 
@@ -199,7 +235,7 @@ def test_adjoint_layouts_agree(device, small, small_grads, kw):
     say("adjoint layout", kw)
     s, psi, _ = small
     ham = hamiltonian.ks_hamiltonian(s, 3)
-    prop = _prop(ham, 3, **kw)
+    prop = _prop(ham, 3, commutator=small.mode, **kw)
     prop.forward(COEF, to_dev(psi, device))
     gc, gV, _ = prop.backward(1.0)
     torch.cuda.synchronize()
@@ -214,7 +250,7 @@ def test_autograd_function(device, small, small_grads):
     say("autograd function")
     s, psi, _ = small
     ham = hamiltonian.ks_hamiltonian(s, 3)
-    prop = _prop(ham, 3)
+    prop = _prop(ham, 3, commutator=small.mode)
     coef = torch.tensor(COEF, dtype=torch.float64, requires_grad=True)
     vloc = torch.from_numpy(s.vloc.copy()).to(device).requires_grad_(True)
     loss = adjoint.td_current_loss(coef, vloc, prop, to_dev(psi, device))
@@ -222,5 +258,28 @@ def test_autograd_function(device, small, small_grads):
     gc0, gV0, _ = small_grads
     assert np.abs(coef.grad.numpy() - 2 * gc0).max() <= 1e-12 * np.abs(gc0).max()
     assert np.abs(vloc.grad.cpu().numpy() - 2 * gV0).max() <= 1e-12 * np.abs(gV0).max()
+    prop.close()
+    ham.close()
+
+
+def test_autograd_second_forward_before_backward_raises(device, small):
+    """Two live losses on one propagation object: the first one's backward
+    would read the second forward's checkpoints, so it raises."""
+    if small.mode != "inq" or small.s.n[0] != 24:
+        pytest.skip("one case suffices")
+    s, psi, _ = small
+    ham = hamiltonian.ks_hamiltonian(s, 3)
+    prop = _prop(ham, 3)
+    coef = torch.tensor(COEF, dtype=torch.float64, requires_grad=True)
+    vloc = torch.from_numpy(s.vloc.copy()).to(device).requires_grad_(True)
+    l1 = adjoint.td_current_loss(coef, vloc, prop, to_dev(psi, device))
+    l2 = adjoint.td_current_loss(coef, vloc, prop, to_dev(psi, device))
+    with pytest.raises(RuntimeError, match="another forward"):
+        (l1 + l2).backward()
+    # the library itself refuses a backward after V changed
+    prop.forward(COEF, to_dev(psi, device))
+    ham.set_scalar_potential(vloc.detach() + 0.1)
+    with pytest.raises(Exception, match="changed since the forward"):
+        prop.backward(1.0)
     prop.close()
     ham.close()

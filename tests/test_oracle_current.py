@@ -81,7 +81,7 @@ def test_current_is_minus_energy_derivative_with_projectors():
         h = osys.hamiltonian(psi, kA)
         return occ * dV * np.real(np.sum(psi.conj() * h))
 
-    _, jx = osys.current(psi, lat @ A0, occ)
+    _, jx = osys.current(psi, lat @ A0, occ, commutator="corrected")
     eps = 1e-4
     for d in range(3):
         e = np.zeros(3)
@@ -92,6 +92,62 @@ def test_current_is_minus_energy_derivative_with_projectors():
     free = O.System(s.n, s.lattice, s.vloc)
     _, jfree = free.current(psi, lat @ A0, occ)
     assert np.abs(jx - jfree).max() > 1e-4 * np.abs(jx).max()
+
+
+def _scaled_tables(tables, f):
+    t = dict(tables)
+    t["coeff"] = f(np.asarray(tables["coeff"], float))
+    return t
+
+
+def test_inq_commutator_is_quadratic_in_the_kb_coefficient():
+    """position_commutator as the reference writes it (projector_all.hpp:
+    362-366) scales the plain projection by D twice and leaves the r-weighted
+    one unscaled, so for fixed psi J(t D) = J_0 + a + t^2 b: even in t, with a
+    D-independent part a that survives D = 0 (the unscaled R term)."""
+    lat = np.array([[7.0, 0.0, 0.0], [0.3, 6.5, 0.0], [0.0, -0.4, 7.2]])
+    n = (12, 12, 14)
+    atoms = np.array([[0.2, 0.1, -0.3], [2.9, -2.1, 1.6]])
+    s = systems.make_system(n, lat, atoms, seed=3, system_id=11)
+    psi = systems.random_orbitals(s, 3, ecut=6.0)
+    kA = lat @ np.array([0.05, -0.03, 0.02])
+
+    def J(t, mode="inq"):
+        osys = O.System(s.n, s.lattice, s.vloc, _scaled_tables(s.tables, lambda c: t * c))
+        return osys.current(psi, kA, 2.0, commutator=mode)[1]
+
+    j0, j1, jm1, j2 = J(0.0), J(1.0), J(-1.0), J(2.0)
+    scale = np.abs(j1).max()
+    assert np.abs(jm1 - j1).max() < 1e-12 * scale
+    assert np.abs((j2 - j0) - 4.0 * (j1 - j0)).max() < 1e-11 * scale
+    # the D-independent part is not the projector-free current ...
+    _, jfree = O.System(s.n, s.lattice, s.vloc).current(psi, kA, 2.0)
+    assert np.abs(j0 - jfree).max() > 1e-6 * scale
+    # ... while the corrected form vanishes with D
+    assert np.abs(J(0.0, "corrected") - jfree).max() < 1e-12 * scale
+
+
+def test_inq_commutator_equals_corrected_with_mean_coefficient():
+    """Im(conj(p) r) - D^2 Im(conj(r) p) = (1 + D^2) Im(conj(p) r): the
+    reference's current is the corrected one (pinned as -dE/dA above) with
+    every KB coefficient D replaced by (1 + D^2)/2; at D = 1 the two agree."""
+    lat = np.array([[7.0, 0.0, 0.0], [0.3, 6.5, 0.0], [0.0, -0.4, 7.2]])
+    n = (12, 12, 14)
+    atoms = np.array([[0.2, 0.1, -0.3], [2.9, -2.1, 1.6]])
+    s = systems.make_system(n, lat, atoms, seed=3, system_id=11)
+    psi = systems.random_orbitals(s, 3, ecut=6.0)
+    kA = lat @ np.array([0.05, -0.03, 0.02])
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    j_inq = osys.current(psi, kA, 2.0, commutator="inq")[1]
+    mean = O.System(s.n, s.lattice, s.vloc, _scaled_tables(s.tables, lambda c: (1.0 + c * c) / 2.0))
+    j_mean = mean.current(psi, kA, 2.0, commutator="corrected")[1]
+    assert np.abs(j_inq - j_mean).max() < 1e-12 * np.abs(j_inq).max()
+    j_cor = osys.current(psi, kA, 2.0, commutator="corrected")[1]
+    assert np.abs(j_inq - j_cor).max() > 1e-6 * np.abs(j_inq).max()
+    ones = O.System(s.n, s.lattice, s.vloc, _scaled_tables(s.tables, lambda c: np.where(c != 0, 1.0, 0.0)))
+    a = ones.current(psi, kA, 2.0, commutator="inq")[1]
+    b = ones.current(psi, kA, 2.0, commutator="corrected")[1]
+    assert np.abs(a - b).max() < 1e-12 * np.abs(a).max()
 
 
 def test_td_loss_zero_field_is_conserved_current():
