@@ -6,7 +6,12 @@ one MI355X.  Prints ONE JSON line on stdout; diagnostics go to stderr.
 
 hpsi_64 (default): band-Hpsi/s at the 64^3 grid x 256 bands FP64 workload
   (2x2x2 diamond supercell, a = 20.52 bohr, 64 atoms x 4 projectors, Gamma);
-  a step is one Hpsi of all 256 bands with psi resident in HBM.
+  a step is one Hpsi of all 256 bands with psi resident in HBM.  The same
+  line carries the rest of the metric: the uneven batch against one launch
+  per system (committed tuning, batched_speedup) and the adjoint/forward
+  cost of adjoint_48 (adjoint_over_forward), each with its parity figures,
+  and the oracle's rate at all host threads and at one (--hpsi-only skips
+  the batch and adjoint parts).
 uneven_batch: 24 seeded systems (grids 24..72, 8..256 bands, 2..128 atoms);
   a step is one Hpsi of every system, batched (one call) and per system.
 adjoint_48: frozen-potential ETRS at 48^3 x 64 bands under a 32-term A(t),
@@ -15,8 +20,10 @@ etrs_96: the graph-captured ETRS step at 96^3 x 512 bands (216 atoms), and
   the adjoint of 100 steps with checkpoints every 10.
 gs_64: one frozen-potential ground-state iteration (subspace diagonalisation,
   preconditioned steepest descent, orthogonalize) at 64^3 x 256 bands.
-tddft_64: one interacting ETRS step (Hartree + LDA, self-consistent second
+tddft_64: one interacting ETRS step (Hartree + PBE, self-consistent second
   half) at 64^3 x 256 bands.
+stream: the library's 16-byte streaming probe (read / write / copy) over
+  1 GiB and over a band chunk of 64^3 x 256 rows: the achievable ceiling.
 """
 import argparse
 import json
@@ -58,12 +65,16 @@ def kernel_design_bytes(N, nst, sum_k, nlm, chunks):
     }
 
 
-def cpu_baseline(s, psi, budget_s):
+def cpu_baseline(s, psi, budget_s, threads=None):
+    """The oracle's Hpsi on a bounded sample of bands of the same system, at
+    `threads` OpenMP threads (None: all of them)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
 
     osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    default = O.get_threads()
+    O.set_threads(threads if threads else 0)
     threads = O.get_threads()
     one = np.ascontiguousarray(psi[..., :1])
     t0 = time.perf_counter()
@@ -75,6 +86,7 @@ def cpu_baseline(s, psi, budget_s):
     ref = osys.hamiltonian(sample)
     dt = time.perf_counter() - t0
     log(f"cpu oracle: {nb} bands in {dt:.2f} s with {threads} threads")
+    O.set_threads(default)
     return {
         "value": nb / dt, "unit": "band-Hpsi/s", "cores": threads, "kind": "port",
         "sample": f"{nb} of {psi.shape[-1]} bands of the same 64^3 system (all 64 atoms), "
@@ -104,7 +116,11 @@ def timed(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def run_uneven_batch(args):
+def measure_uneven_batch(args, tune):
+    """Config 2: 24 seeded systems; per-system launches against one batched
+    call.  tune=False uses the committed tuning (inq_amd/tuning.json),
+    tune=True re-runs the autotuner.  Returns a dict of the figures and the
+    largest relative errors (batched vs per-system, one member vs oracle)."""
     import numpy as np
     import torch
 
@@ -122,8 +138,7 @@ def run_uneven_batch(args):
         hams.append(inq_amd.ks_hamiltonian(s, max_bands=nst))
         phis.append(torch.from_numpy(psi).to(dev))
         outs.append(torch.empty_like(phis[-1]))
-        if i % 6 == 5:
-            log(f"{i + 1} systems built ({time.perf_counter() - t0:.1f} s)")
+    log(f"uneven batch: {len(lay)} systems built in {time.perf_counter() - t0:.1f} s")
     bands = sum(nst for _, nst, _ in lay)
 
     def per_system():
@@ -133,11 +148,16 @@ def run_uneven_batch(args):
     ms_per = timed(per_system, args.steps, args.warmup)
     ref = [o.clone() for o in outs]
     bt = B.hamiltonian_batch(hams)
-    t0 = time.perf_counter()
-    best, table = B.autotune(bt, phis, outs, reps=3, chunk_mib=(128, 256, 512), ty_main=(8,),
-                             narrow=(True, False), group=(False, True), nstreams=(1, 2, 4))
-    log(f"autotune {time.perf_counter() - t0:.1f} s: best {best}; " +
-        ", ".join(f"{k}: {v:.3f} ms" for k, v in sorted(table.items(), key=lambda x: x[1])))
+    if tune:
+        t0 = time.perf_counter()
+        best, table = B.autotune(bt, phis, outs, reps=3, chunk_mib=(128, 256, 512),
+                                 ty_main=(8,), narrow=(True, False), group=(False, True),
+                                 nstreams=(1, 2, 4))
+        log(f"autotune {time.perf_counter() - t0:.1f} s: best {best}; " +
+            ", ".join(f"{k}: {v:.3f} ms" for k, v in sorted(table.items(), key=lambda x: x[1])))
+    else:
+        bt.set_tuning(**B.committed_tuning("uneven_batch_24"))
+        best = dict(bt.tuning)
     ms_bat = timed(lambda: bt(phis, outs), args.steps, args.warmup)
     torch.cuda.synchronize()
     batch_err = max(float((torch.linalg.vector_norm(o - r) / torch.linalg.vector_norm(r)).item())
@@ -164,21 +184,38 @@ def run_uneven_batch(args):
                "kind": "port",
                "sample": f"system {k} of the batch ({lay[k][0]}^3, {lay[k][1]} bands, "
                          f"{lay[k][2]} atoms), one Hpsi, C++ oracle OpenMP"}
-    cfg = {"workload": "uneven_batch: 24 seeded systems, grids 24..72 cubed, bands log-uniform "
-                       "8..256, atoms 2..128 x 4 Gaussian projectors; a step = Hpsi of every "
-                       "system in one batched call",
-           "systems": [list(x) for x in lay], "global_batch": bands, "parallelism": "single GPU"}
-    out = common_fields(args, bands / (ms_bat / 1e3), "band-Hpsi/s", ms_bat, True, cfg)
-    out.update({"per_system_ms_per_step": ms_per, "batched_speedup": ms_per / ms_bat,
-                "batched_vs_per_system_max_rel_err": batch_err, "tuning": best, "batch_info": info,
-                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                             "kernel": "batched Hpsi step (all launches of one call)",
-                             "algorithmic_bytes_per_step": total_bytes},
-                "cpu_baseline": cpu, "max_rel_err_vs_oracle": parity})
+        log(f"batch member {k} vs oracle: rel err {parity:.2e}")
     bt.close()
     for h in hams:
         h.close()
+    return {"layout": [list(x) for x in lay], "bands": bands, "per_system_ms": ms_per,
+            "batched_ms": ms_bat, "batched_speedup": ms_per / ms_bat, "tuning": best,
+            "tuning_source": "autotune in this run" if tune else "inq_amd/tuning.json",
+            "batch_info": info, "algorithmic_bytes_per_step": total_bytes,
+            "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS,
+            "batched_vs_per_system_max_rel_err": batch_err,
+            "member_vs_oracle_rel_err": parity, "oracle_member": k, "cpu_baseline": cpu}
+
+
+def run_uneven_batch(args):
+    m = measure_uneven_batch(args, tune=args.autotune)
+    cfg = {"workload": "uneven_batch: 24 seeded systems, grids 24..72 cubed, bands log-uniform "
+                       "8..256, atoms 2..128 x 4 Gaussian projectors; a step = Hpsi of every "
+                       "system in one batched call",
+           "systems": m["layout"], "global_batch": m["bands"], "parallelism": "single GPU"}
+    out = common_fields(args, m["bands"] / (m["batched_ms"] / 1e3), "band-Hpsi/s",
+                        m["batched_ms"], True, cfg)
+    out.update({"per_system_ms_per_step": m["per_system_ms"],
+                "batched_speedup": m["batched_speedup"],
+                "batched_vs_per_system_max_rel_err": m["batched_vs_per_system_max_rel_err"],
+                "tuning": m["tuning"], "tuning_source": m["tuning_source"],
+                "batch_info": m["batch_info"],
+                "roofline": {"bound": "hbm", "achieved": m["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": m["frac"], "traffic": None,
+                             "kernel": "batched Hpsi step (all launches of one call)",
+                             "algorithmic_bytes_per_step": m["algorithmic_bytes_per_step"]},
+                "cpu_baseline": m["cpu_baseline"],
+                "max_rel_err_vs_oracle": m["member_vs_oracle_rel_err"]})
     return out
 
 
@@ -207,7 +244,12 @@ def _oracle_etrs_rate(s, psi, budget_s, dt):
                                       f"(two Taylor-4 half steps), C++ oracle OpenMP"}
 
 
-def run_adjoint(args, supercell, grid, nst, nsteps, ckpt, dt, workload, fd_coefs=2):
+def measure_adjoint(args, supercell, grid, nst, nsteps, ckpt, dt, fd_coefs=2, parity_bands=4,
+                    parity_steps=5):
+    """Forward propagation and its adjoint (loss J_x(T), 32 series
+    coefficients): times, the adjoint/forward ratio, central differences of
+    the GPU loss on the largest-gradient coefficients, and a short-horizon
+    forward (parity_bands x parity_steps) against the oracle."""
     import numpy as np
     import torch
 
@@ -218,10 +260,9 @@ def run_adjoint(args, supercell, grid, nst, nsteps, ckpt, dt, workload, fd_coefs
     s = inq_amd.systems.diamond_system(supercell, grid=grid)
     psi = inq_amd.systems.random_orbitals(s, nst)
     coef = inq_amd.systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=32)
-    log(f"inputs built in {time.perf_counter() - t0:.1f} s: grid {s.n}, {nst} bands")
+    log(f"adjoint inputs built in {time.perf_counter() - t0:.1f} s: grid {s.n}, {nst} bands")
     dev = torch.device("cuda:0")
     d_psi = torch.from_numpy(psi).to(dev)
-    del psi
     ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
     prop = adjoint.propagation_adjoint(ham, nst, nsteps, dt, 32, 0.1, (1.0, 0.0, 0.0),
                                        checkpoint_every=ckpt, band_slice=min(nst, 64))
@@ -247,30 +288,65 @@ def run_adjoint(args, supercell, grid, nst, nsteps, ckpt, dt, workload, fd_coefs
     scale = float(np.abs(gc).max())
     fd_err = max(abs(a - b) for a, b in fd.values()) / scale
     log(f"FD check {fd} -> max rel err {fd_err:.2e}")
+    memory = prop.memory_bytes / 2**30
+    prop.close()
+    # the forward arithmetic against the oracle on a short horizon
+    parity = None
+    if parity_bands > 0 and not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        few = np.ascontiguousarray(psi[..., :parity_bands])
+        short = adjoint.propagation_adjoint(ham, parity_bands, parity_steps, dt, 32, 0.1,
+                                            (1.0, 0.0, 0.0), checkpoint_every=ckpt)
+        out = torch.empty_like(torch.from_numpy(few).to(dev))
+        l_gpu, _ = short.forward(coef, torch.from_numpy(few).to(dev), out)
+        short.close()
+        osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+        t0 = time.perf_counter()
+        l_ref, psi_ref = osys.td_current(few, coef, 0.1, (1.0, 0.0, 0.0), dt, parity_steps)
+        log(f"oracle short forward ({parity_bands} bands x {parity_steps} steps) "
+            f"{time.perf_counter() - t0:.1f} s")
+        got = out.cpu().numpy()
+        parity = {"bands": parity_bands, "steps": parity_steps,
+                  "psi_rel_err": float(np.linalg.norm(got - psi_ref) / np.linalg.norm(psi_ref)),
+                  "loss_rel_err": abs(l_gpu - l_ref) / max(abs(l_ref), 1e-300)}
+        log(f"short forward vs oracle: {parity}")
     N = s.npoints
     sum_k = int(np.sum(s.tables["sphere_size"]))
     step_bytes = _etrs_bytes(N, nst, sum_k)
     fwd_ms_step = t_fwd * 1e3 / nsteps
     achieved = step_bytes / (fwd_ms_step / 1e3) / 1e9
+    res = {"grid": list(s.n), "bands": nst, "atoms": s.natoms, "steps": nsteps,
+           "checkpoint_every": ckpt, "dt": dt, "forward_s": t_fwd, "backward_s": t_bwd,
+           "adjoint_over_forward": t_bwd / t_fwd, "loss": loss,
+           "fd_check": {"h": h, "coefficients": fd, "max_rel_err": fd_err},
+           "forward_vs_oracle": parity, "memory_GiB": memory,
+           "forward_step_bytes": step_bytes, "forward_achieved_GBs": achieved,
+           "forward_frac": achieved / HBM_PEAK_GBS, "sample_psi": psi, "system": s}
+    ham.close()
+    return res
+
+
+def run_adjoint(args, supercell, grid, nst, nsteps, ckpt, dt, workload, fd_coefs=2):
+    m = measure_adjoint(args, supercell, grid, nst, nsteps, ckpt, dt, fd_coefs=fd_coefs)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = _oracle_etrs_rate(s, inq_amd.systems.random_orbitals(s, min(nst, 8)),
+        import inq_amd
+        cpu = _oracle_etrs_rate(m["system"], inq_amd.systems.random_orbitals(m["system"], min(nst, 8)),
                                 args.cpu_seconds, dt)
-    cfg = {"workload": workload, "grid": list(s.n), "bands": nst, "atoms": s.natoms,
+    cfg = {"workload": workload, "grid": m["grid"], "bands": nst, "atoms": m["atoms"],
            "steps": nsteps, "checkpoint_every": ckpt, "dt": dt, "taylor_order": 4,
            "coefficients": 32, "omega": 0.1, "parallelism": "single GPU"}
-    out = common_fields(args, t_bwd / t_fwd, "adjoint time / forward time", fwd_ms_step, False,
-                        cfg)
-    out.update({"forward_s": t_fwd, "backward_s": t_bwd, "loss": loss,
-                "fd_check": {"h": h, "coefficients": fd, "max_rel_err": fd_err},
-                "memory_GiB": prop.memory_bytes / 2**30,
-                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                             "kernel": "forward ETRS step (all launches)",
-                             "algorithmic_bytes_per_step": step_bytes},
+    out = common_fields(args, m["adjoint_over_forward"], "adjoint time / forward time",
+                        m["forward_s"] * 1e3 / nsteps, False, cfg)
+    out.update({"forward_s": m["forward_s"], "backward_s": m["backward_s"], "loss": m["loss"],
+                "fd_check": m["fd_check"], "forward_vs_oracle": m["forward_vs_oracle"],
+                "memory_GiB": m["memory_GiB"],
+                "roofline": {"bound": "hbm", "achieved": m["forward_achieved_GBs"],
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": m["forward_frac"],
+                             "traffic": None, "kernel": "forward ETRS step (all launches)",
+                             "algorithmic_bytes_per_step": m["forward_step_bytes"]},
                 "cpu_baseline": cpu})
-    prop.close()
-    ham.close()
     return out
 
 
@@ -534,6 +610,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hpsi-only", action="store_true",
+                    help="hpsi_64: skip the batched and adjoint parts of the metric")
+    ap.add_argument("--autotune", action="store_true",
+                    help="uneven_batch: re-run the autotuner instead of the committed tuning")
     ap.add_argument("--xc", choices=["pbe", "lda"], default="pbe",
                     help="exchange-correlation of the tddft_64 workload (PBE is INQ's default)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -656,7 +736,9 @@ def main():
                          "frac": step_achieved / HBM_PEAK_GBS}}
 
     result_host = d_out.cpu().numpy()
-    cpu = None
+    ham.close()
+    del d_psi, d_out
+    cpu = cpu1 = None
     parity = None
     if not args.no_cpu_baseline:
         cpu, ref = cpu_baseline(s, psi, args.cpu_seconds)
@@ -664,6 +746,28 @@ def main():
         got = result_host[..., :nb]
         parity = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
         log(f"parity vs oracle on {nb} bands: rel err {parity:.3e}")
+        cpu1, ref1 = cpu_baseline(s, psi, args.cpu_seconds / 2, threads=1)
+        p1 = float(np.linalg.norm(result_host[..., :ref1.shape[-1]] - ref1) / np.linalg.norm(ref1))
+        cpu1["max_rel_err_vs_gpu"] = p1
+    extras = {}
+    if not args.hpsi_only:
+        # the rest of the metric: batched vs per-system (config 2) with the
+        # committed tuning, and the adjoint/forward cost (config 3)
+        torch.cuda.empty_cache()
+        try:
+            extras["batched"] = measure_uneven_batch(args, tune=False)
+        except Exception as e:  # the headline figure above stays valid
+            extras["batched"] = {"error": repr(e)}
+            log(f"uneven batch failed: {e!r}")
+        torch.cuda.empty_cache()
+        try:
+            m = measure_adjoint(args, 1, 48, 64, 400, 20, 0.015)
+            m.pop("sample_psi")
+            m.pop("system")
+            extras["adjoint"] = m
+        except Exception as e:
+            extras["adjoint"] = {"error": repr(e)}
+            log(f"adjoint failed: {e!r}")
 
     out = {
         "metric": METRIC, "value": value, "unit": "band-Hpsi/s", "n_gpus": 1,
@@ -677,10 +781,17 @@ def main():
                    "sum_sphere_points": sum_k, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "max_rel_err_vs_oracle": parity,
     }
+    if extras:
+        b = extras.get("batched", {})
+        a = extras.get("adjoint", {})
+        out["batched_speedup"] = b.get("batched_speedup")
+        out["adjoint_over_forward"] = a.get("adjoint_over_forward")
+        out["batched"] = b
+        out["adjoint"] = a
     print(json.dumps(out), flush=True)
-    ham.close()
 
 
 if __name__ == "__main__":

@@ -10,6 +10,8 @@ timing the batch on the device.
 """
 import ctypes
 import itertools
+import json
+import os
 
 import torch
 
@@ -78,6 +80,17 @@ class hamiltonian_batch:
             self.close()
         except Exception:
             pass
+
+
+_TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+
+def committed_tuning(name):
+    """The committed set_tuning arguments for a named batch (tuning.json)."""
+    with open(_TUNING_PATH) as f:
+        t = json.load(f)["batch"][name]
+    return {k: t[k] for k in ("chunk_bytes", "ty_main", "narrow_remainders", "group_by_grid",
+                              "nstreams")}
 
 
 def _time(fn, reps):

@@ -128,16 +128,134 @@ def load_real_time_results(dirname):
     return res
 
 
-def save_real_time_checkpoint(ham, psi, occupations, spin_density, results,
-                              root=".inq/default_checkpoint"):
-    """The checkpoint real_time::results writes every 500 steps
-    (results.hpp:53-66): the type file is removed first and written last, so
-    a partial checkpoint has none."""
+def _write_checkpoint(root, results, write_orbitals):
+    # the type file is removed first and written last (results.hpp:60-65)
     os.makedirs(root, exist_ok=True)
     tfile = os.path.join(root, "type")
     if os.path.exists(tfile):
         os.remove(tfile)
     save_real_time_results(os.path.join(root, "real-time", "observables"), **results)
-    save_electrons(os.path.join(root, "real-time", "orbitals"), ham, psi, occupations,
-                   spin_density)
+    write_orbitals(os.path.join(root, "real-time", "orbitals"))
     _write_value(tfile, "real-time")
+
+
+def save_real_time_checkpoint(ham, psi, occupations, spin_density, results,
+                              root=".inq/default_checkpoint"):
+    """The checkpoint real_time::results writes every 500 steps
+    (results.hpp:53-66): the type file is removed first and written last, so
+    a partial checkpoint has none."""
+    _write_checkpoint(root, results, lambda d: save_electrons(d, ham, psi, occupations,
+                                                              spin_density))
+
+
+def save_real_time_checkpoint_host(psi, occupations, results, root=".inq/default_checkpoint",
+                                   kpin=0):
+    """The same checkpoint from host (numpy) orbitals, without the density."""
+    def orbitals(d):
+        base = os.path.join(d, "kpin" + _num(kpin))
+        states_from_host(os.path.join(base, "states"), psi)
+        _save_raw(os.path.join(base, "occupations"), np.broadcast_to(
+            np.asarray(occupations, np.float64), (np.shape(psi)[-1],)))
+    _write_checkpoint(root, results, orbitals)
+
+
+class CheckpointError(RuntimeError):
+    """A checkpoint that cannot be resumed (the messages of run.hpp:121-151)."""
+
+
+def checkpoint_type(root=".inq/default_checkpoint"):
+    """utils::load_optional(root/type): the checkpoint type or None."""
+    p = os.path.join(root, "type")
+    if not os.path.exists(p):
+        return None
+    return open(p).read().split("\n")[0]
+
+
+def resume_real_time(root=".inq/default_checkpoint"):
+    """The checks of run::resume (interface/run.hpp:121-151) before a
+    real-time run continues from a checkpoint: the `type` file (written last,
+    so a partially written checkpoint has none) must exist and read
+    "real-time".  Returns (observables dict, orbitals directory); the
+    propagation then continues with start_step = total_steps."""
+    t = checkpoint_type(root)
+    if t is None:
+        raise CheckpointError("Cannot resume run, a checkpoint was not found.")
+    if t != "real-time":
+        raise CheckpointError("Unknown checkpoint type '%s'." % t)
+    orb = os.path.join(root, "real-time", "orbitals")
+    if not os.path.isdir(os.path.join(orb, "kpin" + _num(0), "states")):
+        raise CheckpointError("Cannot load the restart electron orbitals.\n"
+                              " The checkpoint must be corrupt")
+    return load_real_time_results(os.path.join(root, "real-time", "observables")), orb
+
+
+def states_to_host(dirname, n, nst, first_band=0, spinor=0):
+    """Read the per-band orbital files of operations::io::save into a numpy
+    array [nx, ny, nz, nst] (host side; the library's load_states does the
+    same on the device)."""
+    N = int(np.prod(n))
+    out = np.empty((N, nst), np.complex128)
+    for b in range(nst):
+        p = os.path.join(dirname, "%s_%s" % (_num(first_band + b), _num(spinor)))
+        if not os.path.exists(p):
+            raise CheckpointError("cannot open restart file " + p)
+        raw = np.fromfile(p, np.complex128)
+        if raw.size != N:
+            raise CheckpointError("restart file %s holds %d points, expected %d" % (p, raw.size, N))
+        out[:, b] = raw
+    return out.reshape(tuple(n) + (nst,))
+
+
+def states_from_host(dirname, psi, first_band=0, spinor=0):
+    """Write psi [.., nst] (numpy) as operations::io::save does: one raw
+    complex128 file per band in grid order."""
+    os.makedirs(dirname, exist_ok=True)
+    a = np.ascontiguousarray(psi, np.complex128)
+    a = a.reshape(-1, a.shape[-1])
+    for b in range(a.shape[-1]):
+        np.ascontiguousarray(a[:, b]).tofile(
+            os.path.join(dirname, "%s_%s" % (_num(first_band + b), _num(spinor))))
+
+
+class real_time_results:
+    """real_time::results (src/real_time/results.hpp:24-112): the propagation
+    callback that records time, total energy and, if asked, the dipole and
+    the current of every step, and writes the real-time checkpoint every
+    `every` steps (500 in INQ) through `save_checkpoint(viewables, results)`.
+    """
+
+    def __init__(self, dipole=False, current=False, every=500, save_checkpoint=None):
+        self.total_steps = 0
+        self.total_time = 0.0
+        self.time, self.total_energy, self.dipole, self.current = [], [], [], []
+        self.want_dipole, self.want_current = bool(dipole), bool(current)
+        self.every_steps = int(every)
+        self.save_checkpoint = save_checkpoint
+
+    @classmethod
+    def load(cls, dirname, **kw):
+        r = cls(**kw)
+        d = load_real_time_results(dirname)
+        r.total_steps, r.total_time = d["total_steps"], d["total_time"]
+        r.time, r.total_energy = list(d["time"]), list(d["total_energy"])
+        r.dipole, r.current = [tuple(v) for v in d["dipole"]], [tuple(v) for v in d["current"]]
+        return r
+
+    def as_dict(self):
+        return dict(total_steps=self.total_steps, total_time=self.total_time, time=self.time,
+                    total_energy=self.total_energy, dipole=self.dipole, current=self.current)
+
+    def __call__(self, v):
+        self.total_steps = v.iter()
+        self.total_time = v.time()
+        self.time.append(v.time())
+        self.total_energy.append(v.energy().total())
+        if self.want_dipole:
+            self.dipole.append(tuple(float(x) for x in v.dipole()))
+        if self.want_current:
+            self.current.append(tuple(float(x) for x in v.current()))
+        if self.total_steps + 1 != len(self.time):
+            raise RuntimeError("real_time_results: %d steps but %d recorded times"
+                               % (self.total_steps, len(self.time)))
+        if self.save_checkpoint is not None and v.every(self.every_steps):
+            self.save_checkpoint(v, self)
