@@ -151,6 +151,18 @@ int inqamd_projector_apply(inqamd_hamiltonian_t h, const void *sphere, void *hps
  * eps[b] = Re<hpsi_b|psi_b> / <psi_b|psi_b>.  eps is a DEVICE array [nst]. */
 int inqamd_band_energies(inqamd_hamiltonian_t h, const void *hpsi, const void *psi,
                          double *eps, int nst, int ld);
+/* The same band energies without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * partial sums live in a fixed 8 MiB device array that each GPU holds until
+ * the process ends, instead of the handle's workspace; nst is limited to
+ * 2^19 / ceil(N / 256) bands (512 at 64^3, 151 at 96^3; larger nst returns
+ * INQAMD_ERR_ARGUMENT with the limit in the message) but not by max_bands.
+ * Results equal inqamd_band_energies' bit for bit.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (two streams, two
+ * threads) race and must be serialised by the caller. */
+int inqamd_band_energies_noalloc(inqamd_hamiltonian_t h, const void *hpsi, const void *psi,
+                                 double *eps, int nst, int ld);
 /* The per-band terms of energy::calculate (energy.hpp:59-87) for one set of
  * orbitals, with the handle's current k+A:
  *   eps[b]     = Re<H psi_b|psi_b> / <psi_b|psi_b>             (:74-75)

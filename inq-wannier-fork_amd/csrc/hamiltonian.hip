@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <complex>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -594,16 +595,62 @@ void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double
   }
 }
 
-void band_energies(handle &h, const cplx *hpsi, long hld, const cplx *psi, long ld, int nst,
-                   double *eps) {
+// The two launches of the band energies with their [nchunk][nst][2] partial
+// sums in `partial` (every entry the final kernel reads is written first)
+static void band_energies_with(handle &h, const cplx *hpsi, long hld, const cplx *psi, long ld,
+                               int nst, double *eps, double *partial) {
   const int nchunk = (int)((h.npoint + EN_CHUNK - 1) / EN_CHUNK);
   dim3 grid(nchunk, (nst + EN_BT - 1) / EN_BT);
   hipLaunchKernelGGL(energies_partial_kernel, grid, dim3(256), 0, h.stream, hpsi, hld, psi, ld,
-                     nst, h.npoint, h.partial);
+                     nst, h.npoint, partial);
   INQAMD_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(energies_final_kernel, dim3((nst + 63) / 64), dim3(64), 0, h.stream,
-                     (const double *)h.partial, nchunk, nst, eps);
+                     (const double *)partial, nchunk, nst, eps);
   INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+void band_energies(handle &h, const cplx *hpsi, long hld, const cplx *psi, long ld, int nst,
+                   double *eps) {
+  band_energies_with(h, hpsi, hld, psi, ld, nst, eps, h.partial);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free band energies
+// ---------------------------------------------------------------------------
+// The partial sums of energies_partial_kernel live in this file-scope device
+// array instead of the handle's workspace: 8 MiB that each GPU the library
+// runs on holds until the process ends.  On the host the array's name is a
+// stand-in, not a device address: host code only ever uses the address that
+// hipGetSymbolAddress returns for a device (scratch_address), resolved once
+// per device when a handle is created there.
+constexpr long NOALLOC_SCRATCH_DOUBLES = 1L << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex scratch_mutex;
+static double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device `dev`, which must be current
+// the first time it is asked for (handle creation makes sure of that).
+static double *scratch_address(int dev) {
+  INQAMD_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_dev[dev]) {
+    int cur = -1;
+    INQAMD_HIP_CHECK(hipGetDevice(&cur));
+    INQAMD_REQUIRE(cur == dev, "the scratch of a device is resolved with that device current");
+    void *p = nullptr;
+    INQAMD_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    INQAMD_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the scratch");
+    scratch_dev[dev] = static_cast<double *>(p);
+  }
+  return scratch_dev[dev];
+}
+
+// Bands whose partial sums fit the scratch on a grid of npoint points
+static long noalloc_max_bands(long npoint) {
+  const long nchunk = (npoint + EN_CHUNK - 1) / EN_CHUNK;
+  return NOALLOC_SCRATCH_DOUBLES / (2 * nchunk);
 }
 
 // ---------------------------------------------------------------------------
@@ -697,6 +744,10 @@ static void create(handle &h, const inqamd_system_desc &d, hipStream_t stream) {
   {
     int dev = 0, lds = 0;
     INQAMD_HIP_CHECK(hipGetDevice(&dev));
+    h.device = dev;
+    // the allocation-free entry points' scratch on this device (its
+    // address, resolved with the device current)
+    (void)scratch_address(dev);
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess ||
         lds <= 0)
       INQAMD_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -946,6 +997,24 @@ int inqamd_band_energies(inqamd_hamiltonian_t h, const void *hpsi, const void *p
     INQAMD_REQUIRE(eps != nullptr, "eps is null");
     check_bands(*h, nst, ld, true);
     band_energies(*h, (const cplx *)hpsi, ld, (const cplx *)psi, ld, nst, eps);
+  });
+}
+
+int inqamd_band_energies_noalloc(inqamd_hamiltonian_t h, const void *hpsi, const void *psi,
+                                 double *eps, int nst, int ld) {
+  if (!h) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    check_orbital(*h, hpsi, "hpsi");
+    check_orbital(*h, psi, "psi");
+    INQAMD_REQUIRE(eps != nullptr, "eps is null");
+    check_bands(*h, nst, ld, false);
+    const long most = noalloc_max_bands(h->npoint);
+    if (nst > most)
+      throw error{ERR_ARGUMENT, "band_energies_noalloc: " + std::to_string(nst) +
+                                    " bands exceed its 8 MiB scratch, which holds at most " +
+                                    std::to_string(most) + " bands on this grid"};
+    band_energies_with(*h, (const cplx *)hpsi, ld, (const cplx *)psi, ld, nst, eps,
+                       scratch_address(h->device));
   });
 }
 

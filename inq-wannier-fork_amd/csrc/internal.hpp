@@ -84,6 +84,7 @@ struct inqamd_hamiltonian {
   double rlat[9] = {};
   double volume = 0.0;
   hipStream_t stream = nullptr;
+  int device = 0;                   // the device current at creation
   hipStream_t aux = nullptr;        // second stream: odd band chunks of Hpsi
   hipStream_t aux2 = nullptr;       // third stream (INQAMD_STREAMS=3)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr, ev_proj = nullptr;

@@ -107,6 +107,7 @@ SIGNATURES = [
     ("inqamd_projector_project", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_projector_apply", _i, [_vp, _vp, _vp, _i, _i]),
     ("inqamd_band_energies", _i, [_vp, _vp, _vp, _vp, _i, _i]),
+    ("inqamd_band_energies_noalloc", _i, [_vp, _vp, _vp, _vp, _i, _i]),
     ("inqamd_energy", _i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     ("inqamd_laplacian_add", _i, [_vp, _vp, _vp, _i, _i, _d, _c_dbl_p]),
     ("inqamd_preconditioner", _i, [_vp, _vp, _i, _i]),
