@@ -50,13 +50,32 @@ def algorithmic_bytes_per_band(N, nst, sum_k):
     return 80.0 * N + 8.0 * N / nst + 48.0 * sum_k
 
 
-def kernel_design_bytes(N, nst, sum_k, nlm, chunks):
+def vnl_from_coefficients(nlm):
+    """Whether the three-sweep Hpsi expands V_nl psi from the projector
+    coefficients in its last sweep (the library's default for nlm <= 4;
+    INQAMD_VNL=entries keeps the per-entry V_nl psi round trip)."""
+    return nlm <= 4 and os.environ.get("INQAMD_VNL", "") != "entries"
+
+
+def kernel_design_bytes(N, nst, sum_k, nlm, chunks, coefficients=True, nproj=0):
     """Bytes each kernel of one Hpsi must move by design (its own compulsory
-    reads and writes), per launch: the projector's gather of psi and write of
-    V_nl psi per sphere entry (plus its tables), each sweep's read and write of
-    the band chunk, and the last sweep's extra reads of psi (V psi), V, the
-    ownership map and the V_nl psi entries."""
+    reads and writes), per launch, counted once per launch whatever the
+    caches do.  With the coefficient form: the projector gathers psi at the
+    sphere points (once per 32-band tile its point, phase and nlm projector
+    values) and writes P; each sweep reads and writes its band chunk; the last
+    sweep also reads psi (V psi), V, the ownership word and sphere of each
+    point and, once per 8-band tile, the entry tables (nlm values, phase).
+    With per-entry V_nl psi the projector also writes 16 B per sphere point
+    and band, and the last sweep reads them instead of the tables."""
     nb = nst / chunks
+    if coefficients:
+        return {
+            "projector": 16.0 * sum_k * nst + (8.0 * nlm + 4.0 + 16.0) * sum_k * (nst / 32.0)
+                         + 16.0 * nlm * nst * nproj,
+            "z_fwd": 32.0 * N * nb,
+            "xy_kin": 32.0 * N * nb,
+            "z_bwd_epi": 48.0 * N * nb + 16.0 * N + (8.0 * 4 + 16.0 + 4.0) * sum_k * (nb / 8.0),
+        }
     return {
         "projector": 32.0 * sum_k * nst + (8.0 * nlm + 4.0 + 12.0 + 32.0) * sum_k,
         "z_fwd": 32.0 * N * nb,
@@ -711,7 +730,9 @@ def main():
             t, n = tot.get(k, (0.0, 0))
             tot[k] = (t + ms_k, n + n_k)
     chunks = tot["z_fwd"][1] // reps
-    design = kernel_design_bytes(N, nst, sum_k, int(s.tables["max_nlm"]), chunks)
+    nlm = int(s.tables["max_nlm"])
+    coefficients = vnl_from_coefficients(nlm)
+    design = kernel_design_bytes(N, nst, sum_k, nlm, chunks, coefficients, s.natoms)
     kernels = {}
     for k, (t, n) in tot.items():
         avg_ms = t / max(n, 1)
@@ -729,6 +750,8 @@ def main():
     achieved = kernels[dom]["GBs"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "vnl": "expanded from the projector coefficients in z_bwd_epi" if coefficients
+                       else "per-entry V_nl psi written by the projector, read by z_bwd_epi",
                 "avg_launch_ms": kernels[dom]["avg_launch_ms"],
                 "design_bytes_per_launch": kernels[dom]["design_bytes_per_launch"],
                 "kernels": kernels,

@@ -83,6 +83,10 @@ const char *inqamd_version(void);
  * B[k][j] = 2 + 3j - 5k into the DEVICE array out[256].  No reference
  * counterpart; a diagnostic for the projector contraction. */
 int inqamd_selftest_mfma(double *out_device, void *stream);
+/* Diagnostic: the lane layout of the 4x4x4 FP64 matrix instruction the
+ * projector coefficients use; block s of D = A B with A_s[i][k] = 1 + i + 4k +
+ * 16s and B_s[k][j] = 2 + 3j - 5k + 7s into out_device[s][i][j] (64 doubles). */
+int inqamd_selftest_mfma_4x4(double *out_device, void *stream);
 /* Streaming-bandwidth probe (diagnostic, no reference counterpart): one
  * launch of blocks_per_cu x CUs workgroups moving 16-byte complex128 elements
  * with plain vector loads/stores -- kind 0 read, 1 write, 2 copy (src -> dst,

@@ -104,6 +104,17 @@ struct fft3_args {
   const int *own;
   const cplx *Qe;        // [nent][ldQ], offset to the chunk's first band
   long ldQ;
+  // or (Pc != nullptr, Qe unused) V_nl psi expanded at the owned points
+  // from the projector coefficients: hpsi(p, b) += sum_{j < cnt}
+  // phase_ent[e] sum_{lm < nlmP} ent_mat[e][lm] Pc[(ent_sph[e] nlmP + lm) ldPc + b],
+  // e = e0 + j (Pc offset to the chunk's first band; ent_sph[e0] = own_sph[p])
+  const cplx *Pc;
+  long ldPc;
+  int nlmP;
+  const int *ent_sph;     // [nent] sphere of each ownership entry
+  const int *own_sph;     // [N] sphere of a point's first entry (0 where unowned)
+  const double *ent_mat;  // [nent][4] projector values at the entry's point
+  const cplx *phase_ent;  // [nent] e^{-i(k+A).r} of the call
   // Taylor accumulations acc += c * hpsi (p == nullptr: unused), chunk-offset
   cplx *acc1;
   long acc1_ld;

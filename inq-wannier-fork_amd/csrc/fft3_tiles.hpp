@@ -366,6 +366,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   // or 1/2 |G+k+A|^2 kin_in
   cplx pre[NE];
   int own[NE];
+  int osp[NE];  // sphere of the first ownership entry (coefficient expansion)
   [[maybe_unused]] double gx[3];
   if constexpr (GS) {
     double kc[3];
@@ -380,6 +381,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
     pre[k] = mk(0.0, 0.0);
     own[k] = -1;
+    osp[k] = 0;
     if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
       const long p = ((long)x * L + y0 + y) * L + z;
       if constexpr (GS) {
@@ -393,6 +395,41 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
         const double vp = a.V[p];
         pre[k] = mk(vp * ps.x, vp * ps.y);
         if (a.own) own[k] = a.own[p];
+        if (a.Pc) osp[k] = a.own_sph[p];
+      }
+    }
+  }
+  if constexpr (!GS) {
+    if (a.Pc) {
+      // projector_all::apply (projector_all.hpp:172-212, 225-231) expanded
+      // from the coefficients P[p][lm][b] at this thread's owned outputs,
+      // their ownership entries in fixed (sphere, point) order, before the
+      // FFT so that none of it stays live through the transform.  The first
+      // entry's sphere comes with the ownership word, so its coefficient
+      // loads need not wait for the entry tables.
+#pragma unroll
+      for (int k = 0; k < NE; k++) {
+        if (own[k] < 0) continue;
+        const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
+        const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
+        for (int j = 0; j < cnt; j++) {
+          const int e = e0 + j;
+          const int sp = j == 0 ? osp[k] : a.ent_sph[e];
+          const double4 mm = reinterpret_cast<const double4 *>(a.ent_mat)[e];
+          const double m[4] = {mm.x, mm.y, mm.z, mm.w};
+          const cplx f = a.phase_ent[e];
+          const cplx *P = a.Pc + (long)sp * a.nlmP * a.ldPc + cb;
+          cplx u = mk(0.0, 0.0);
+#pragma unroll
+          for (int lm = 0; lm < 4; lm++) {
+            if (lm < a.nlmP) {
+              const cplx c = P[(long)lm * a.ldPc];
+              u.x += m[lm] * c.x;
+              u.y += m[lm] * c.y;
+            }
+          }
+          pre[k] = cadd(pre[k], cmul(f, u));
+        }
       }
     }
   }
@@ -420,22 +457,26 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   // The first entries of all this thread's outputs are loaded back to back
   // (one memory round trip rather than one per owned output); further
   // entries (points in overlapping spheres) follow.
-  cplx q[NE];
+  if constexpr (!GS) {
+    if (!a.Pc) {
+      cplx q[NE];
 #pragma unroll
-  for (int k = 0; k < NE; k++) {
-    q[k] = mk(0.0, 0.0);
-    if (own[k] >= 0 && (own[k] & OWN_MAX_COUNT) > 0) {
-      const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
-      q[k] = a.Qe[(long)(own[k] >> OWN_COUNT_BITS) * a.ldQ + cb];
-    }
-  }
+      for (int k = 0; k < NE; k++) {
+        q[k] = mk(0.0, 0.0);
+        if (own[k] >= 0 && (own[k] & OWN_MAX_COUNT) > 0) {
+          const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
+          q[k] = a.Qe[(long)(own[k] >> OWN_COUNT_BITS) * a.ldQ + cb];
+        }
+      }
 #pragma unroll
-  for (int k = 0; k < NE; k++) {
-    if (own[k] >= 0) {
-      const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
-      const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
-      if (cnt > 0) pre[k] = cadd(pre[k], q[k]);
-      for (int j = 1; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
+      for (int k = 0; k < NE; k++) {
+        if (own[k] >= 0) {
+          const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
+          const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
+          if (cnt > 0) pre[k] = cadd(pre[k], q[k]);
+          for (int j = 1; j < cnt; j++) pre[k] = cadd(pre[k], a.Qe[(long)(e0 + j) * a.ldQ + cb]);
+        }
+      }
     }
   }
 

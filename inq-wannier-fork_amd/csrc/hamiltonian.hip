@@ -249,6 +249,28 @@ void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int ns
   fft_launch_pass(pass_args(h, 2, out, out_ld, out, out_ld, nst, sign, scale), FFT_PLAIN, h.stream);
 }
 
+// Three-sweep Hpsi: V_nl psi expanded in the last sweep from the projector
+// coefficients P (the projector writes only P, 1 MB at 64^3 x 256, on the
+// 4x4x4 FP64 MFMA), rather than from per-entry V_nl psi values that the
+// projector writes and the sweep reads back (16 B per sphere point and band
+// each way): 64^3 x 256 Hpsi 1.70 -> 1.65 ms, the 96^3 ETRS step -1.2 %.
+// Needs max_nlm <= 4.  INQAMD_VNL=entries keeps the per-entry form.
+static bool vnl_from_coefficients(const handle &h) {
+  static const bool entries = [] {
+    const char *e = getenv("INQAMD_VNL");
+    return e && std::string(e) == "entries";
+  }();
+  return !entries && h.three_sweep && h.pt.nproj > 0 && h.pt.nunique > 0 && h.pt.ent_mat;
+}
+
+// The projector step of the three-sweep Hpsi (after projector_phases)
+static void hpsi_projector(handle &h, const cplx *psi, long ld, int nst) {
+  if (vnl_from_coefficients(h))
+    projector_project_p4(h, psi, ld, nst);
+  else
+    projector_project_expand(h, psi, ld, nst, h.vnl, nst, false);
+}
+
 // The local part T + V on the bands [b0, b0 + nb) of psi (pointers already
 // offset to b0).  The three-sweep path also adds the projector apply and the
 // Taylor accumulations `acc` (offset to b0) in its last sweep; the pencil
@@ -273,7 +295,16 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
     a.tw = h.axis[0].tw;
     a.scale = 1.0 / (double)h.npoint;
     const projector_tables &t = h.pt;
-    if (t.nproj > 0 && t.nunique > 0) {
+    if (vnl_from_coefficients(h)) {
+      a.own = t.own;
+      a.Pc = h.proj + b0;
+      a.ldPc = ldP;
+      a.nlmP = t.max_nlm;
+      a.ent_sph = t.ent_sph;
+      a.own_sph = t.own_sph;
+      a.ent_mat = t.ent_mat;
+      a.phase_ent = t.phase_ent;
+    } else if (t.nproj > 0 && t.nunique > 0) {
       a.own = t.own;
       a.Qe = h.vnl + b0;
       a.ldQ = ldP;
@@ -419,7 +450,7 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
   if (overlap) fork();
   projector_phases(h, slot);
   if (h.three_sweep)
-    projector_project_expand(h, psi, ld, nst, h.vnl, nst, false);
+    hpsi_projector(h, psi, ld, nst);
   else
     projector_project_expand(h, psi, ld, nst, nullptr, 0, true);
   if (overlap) INQAMD_HIP_CHECK(hipEventRecord(h.ev_proj, h.stream));
@@ -828,6 +859,17 @@ int inqamd_selftest_mfma(double *out_device, void *stream) {
   return INQAMD_OK;
 }
 
+int inqamd_selftest_mfma_4x4(double *out_device, void *stream) {
+  if (!out_device) return INQAMD_ERR_ARGUMENT;
+  try {
+    mfma4_selftest(out_device, (hipStream_t)stream);
+  } catch (const inqamd::error &e) {
+    g_create_error = e.msg;
+    return e.code;
+  }
+  return INQAMD_OK;
+}
+
 const char *inqamd_last_error(inqamd_hamiltonian_t h) {
   if (!h) return g_create_error.c_str();
   return h->last_error.c_str();
@@ -1106,7 +1148,7 @@ int inqamd_hamiltonian_profile(inqamd_hamiltonian_t h, const void *psi, void *hp
       for (auto &e : ev) INQAMD_HIP_CHECK(hipEventCreate(&e));
       INQAMD_HIP_CHECK(hipEventRecord(ev[0], h->stream));
       projector_phases(*h, KA_CURRENT);
-      projector_project_expand(*h, (const cplx *)psi, ld, nst, h->vnl, nst, false);
+      hpsi_projector(*h, (const cplx *)psi, ld, nst);
       INQAMD_HIP_CHECK(hipEventRecord(ev[1], h->stream));
       int k = 1;
       for (int c = 0; c < nchunk; c++) {

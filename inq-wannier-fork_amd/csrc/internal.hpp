@@ -43,6 +43,11 @@ struct projector_tables {
   cplx *phase_ent = nullptr;  // [nent]
   int *ent = nullptr;         // [nproj][max_sphere] ownership entry of each sphere point
   int *own = nullptr;         // [N] (first entry << OWN_COUNT_BITS) | count, -1 if not owned
+  // per ownership entry, for expanding V_nl psi from the coefficients in the
+  // Hpsi sweep (max_nlm <= 4 only; nullptr otherwise)
+  int *ent_sph = nullptr;     // [nent] sphere
+  int *own_sph = nullptr;     // [N] sphere of the point's first entry, 0 where unowned
+  double *ent_mat = nullptr;  // [nent][4] M[sphere][lm][point], zero for lm >= max_nlm
 };
 
 struct project_expand_args {
@@ -145,6 +150,9 @@ void projector_owned_sum(handle &h, cplx *Q, int nst, int slot);
 void projector_project_expand(handle &h, const cplx *psi, long ld, int nst, cplx *Qe, long ldQ,
                               bool write_P);
 void mfma_selftest(double *out_device, hipStream_t stream);
+// P coefficients of all spheres and bands on the 4x4x4 FP64 MFMA (max_nlm <= 4)
+void projector_project_p4(handle &h, const cplx *psi, long ld, int nst);
+void mfma4_selftest(double *out_device, hipStream_t stream);
 
 // hamiltonian.hip
 void set_slot(handle &h, int slot, const double v[3]);

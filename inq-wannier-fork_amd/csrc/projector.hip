@@ -262,6 +262,21 @@ void projector_upload(handle &h, int nproj, int max_sphere, int max_nlm, const i
     own[upoint[u]] = (uoff[u] << OWN_COUNT_BITS) | cnt;
   }
   for (size_t e = 0; e < uent.size(); e++) ent[uent[e]] = (int)e;
+  if (max_nlm <= 4 && !uent.empty()) {
+    std::vector<int> esph(uent.size());
+    std::vector<double> emat(uent.size() * 4, 0.0);
+    for (size_t e = 0; e < uent.size(); e++) {
+      const int p = uent[e] / max_sphere, i = uent[e] - p * max_sphere;
+      esph[e] = p;
+      for (int lm = 0; lm < max_nlm; lm++)
+        emat[e * 4 + lm] = mat[((size_t)p * max_nlm + lm) * max_sphere + i];
+    }
+    std::vector<int> osph(h.npoint, 0);
+    for (size_t u = 0; u < upoint.size(); u++) osph[upoint[u]] = esph[uoff[u]];
+    t.ent_sph = upload(esph);
+    t.own_sph = upload(osph);
+    t.ent_mat = upload(emat);
+  }
   t.own = upload(own);
   t.ent = upload(ent);
   t.nent = (int)uent.size();
@@ -285,7 +300,8 @@ void projector_free(handle &h) {
   for (void *p : {(void *)t.point, (void *)t.pos, (void *)t.mat, (void *)t.coeff,
                   (void *)t.coeff_inq, (void *)t.size,
                   (void *)t.upoint, (void *)t.uoff, (void *)t.uent, (void *)t.owner,
-                  (void *)t.phase_sph, (void *)t.phase_ent, (void *)t.own, (void *)t.ent})
+                  (void *)t.phase_sph, (void *)t.phase_ent, (void *)t.own, (void *)t.ent,
+                  (void *)t.ent_sph, (void *)t.own_sph, (void *)t.ent_mat})
     if (p) (void)hipFree(p);
   t = projector_tables{};
 }

@@ -221,4 +221,104 @@ __device__ __forceinline__ void expand_chunk_block(const project_expand_args &a,
   gemm2_points(a, p, b0, chunk_begin(K, c, nc), chunk_begin(K, c + 1, nc), Pr);
 }
 
+// ---------------------------------------------------------------------------
+// GEMM1 alone on v_mfma_f64_4x4x4f64: P = dV D M^T (phase psi) for 16-band
+// tiles, no padding rows (projector_all.hpp:117-165)
+// ---------------------------------------------------------------------------
+// v_mfma_f64_4x4x4f64 runs four independent 4x4x4 blocks s = (l >> 2) & 3;
+// lane l holds (checked on the device by inqamd_selftest_mfma_4x4)
+//   A_s[i][k] with i = l & 3,  k = l >> 4
+//   B_s[k][j] with j = l & 3,  k = l >> 4
+//   D_s[i][j] with i = l >> 4, j = l & 3
+// With A_s = M[lm][4 points] in every block and B_s = X[4 points][bands
+// 4s..4s+3], one instruction is 4 points x 16 bands x 4 lm, and lane l
+// loads point l >> 4, band l & 15 and M[l & 3][point l >> 4], and receives
+// P[lm = l >> 4][band l & 15]: real and imaginary parts are two
+// instructions on the same A.
+constexpr int P4_WAVES = 8;
+constexpr int P4_NT = 64 * P4_WAVES;
+constexpr int P4_BANDS = 16;  // bands of one instruction pair
+constexpr int P4_UNROLL = 4;
+
+__device__ __forceinline__ double mfma4(double a, double b, double c) {
+  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+// Sums over the sphere points [ib, ie) of sphere p for bands
+// [b0, b0 + 16 NB): the waves' accumulators are added in wave order into
+// out[NB][64][2] (lane order: lm = l >> 4, band b0 + 16 q + (l & 15); real,
+// imaginary), unscaled.  Each point's M row and phase serve NB band groups.
+template <int NB>
+__device__ __forceinline__ void gemm1_p4_points(const project_expand_args &a, int p, int b0,
+                                                int ib, int ie, double (*red)[NB][64][2],
+                                                double *out) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nlm = a.max_nlm;
+  const long off = (long)p * a.max_sphere;
+  const double *M = a.mat + (long)p * nlm * a.max_sphere;
+  const int kk = lane >> 4;
+  const int lm_a = lane & 3;
+  const int band = b0 + (lane & 15);
+  double accr[NB], acci[NB];
+#pragma unroll
+  for (int q = 0; q < NB; q++) accr[q] = acci[q] = 0.0;
+  constexpr int STEP = P4_WAVES * 4;
+  for (int j0 = ib + wave * 4; j0 < ie; j0 += STEP * P4_UNROLL) {
+    int idx[P4_UNROLL];
+#pragma unroll
+    for (int u = 0; u < P4_UNROLL; u++) {
+      const int i = j0 + kk + u * STEP;
+      idx[u] = i < ie ? a.point[off + i] : -1;
+    }
+    cplx ps[P4_UNROLL][NB], ph[P4_UNROLL];
+    double av[P4_UNROLL];
+#pragma unroll
+    for (int u = 0; u < P4_UNROLL; u++) {
+      const int i = j0 + kk + u * STEP;
+#pragma unroll
+      for (int q = 0; q < NB; q++)
+        ps[u][q] = (idx[u] >= 0 && band + 16 * q < a.nst)
+                       ? a.psi[(long)idx[u] * a.ld + band + 16 * q] : mk(0.0, 0.0);
+      ph[u] = i < ie ? a.phase[off + i] : mk(0.0, 0.0);
+      av[u] = (i < ie && lm_a < nlm) ? M[(long)lm_a * a.max_sphere + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < P4_UNROLL; u++) {
+#pragma unroll
+      for (int q = 0; q < NB; q++) {
+        const cplx x = cmul(ph[u], ps[u][q]);
+        accr[q] = mfma4(av[u], x.x, accr[q]);
+        acci[q] = mfma4(av[u], x.y, acci[q]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    red[wave][q][lane][0] = accr[q];
+    red[wave][q][lane][1] = acci[q];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB * 128; t += P4_NT) {
+    const int q = t >> 7, l = (t >> 1) & 63, c = t & 1;
+    double s = red[0][q][l][c];
+    for (int w = 1; w < P4_WAVES; w++) s += red[w][q][l][c];
+    out[t] = s;
+  }
+}
+
+// P[p][lm][b] = dV D[p][lm] x the summed tile (projector_all.hpp:131-139, 160-164)
+template <int NB>
+__device__ __forceinline__ void p4_store(const project_expand_args &a, int p, int b0,
+                                         const double *sum) {
+  for (int t = threadIdx.x; t < NB * 64; t += blockDim.x) {
+    const int q = t >> 6, l = t & 63;
+    const int lm = l >> 4, b = b0 + 16 * q + (l & 15);
+    if (lm < a.max_nlm && b < a.nst) {
+      const double d = a.dV * a.coeff[(long)p * a.max_nlm + lm];
+      a.P[((long)p * a.max_nlm + lm) * a.nst + b] = mk(d * sum[2 * t], d * sum[2 * t + 1]);
+    }
+  }
+}
+
 }  // namespace inqamd

@@ -25,6 +25,56 @@ __global__ __launch_bounds__(PE_NT) void expand_chunk_kernel(const project_expan
   expand_chunk_block(b, blockIdx.y, blockIdx.x * PE_BANDS, blockIdx.z, nc, part + tile * nc * 256);
 }
 
+// P coefficients only (the Hpsi sweep expands V_nl psi itself): grid
+// (tiles of 16 NB bands, spheres)
+template <int NB>
+__global__ __launch_bounds__(P4_NT) void project_p4_kernel(const project_expand_args a) {
+  __shared__ double red[P4_WAVES][NB][64][2];
+  __shared__ double sum[NB * 128];
+  const int p = blockIdx.y, b0 = blockIdx.x * P4_BANDS * NB;
+  gemm1_p4_points<NB>(a, p, b0, 0, a.size[p], red, sum);
+  __syncthreads();
+  p4_store<NB>(a, p, b0, sum);
+}
+
+// Point-chunked form for few (sphere, tile) pairs: grid (16-band tiles,
+// spheres, chunks); chunk c of tile t of sphere p at part[((p T + t) nc + c) 128]
+__global__ __launch_bounds__(P4_NT) void project_p4_chunk_kernel(const project_expand_args a,
+                                                                 int nc, double *part) {
+  __shared__ double red[P4_WAVES][1][64][2];
+  const int p = blockIdx.y, c = blockIdx.z;
+  const long tile = (long)p * gridDim.x + blockIdx.x;
+  const int K = a.size[p];
+  gemm1_p4_points<1>(a, p, blockIdx.x * P4_BANDS, chunk_begin(K, c, nc),
+                     chunk_begin(K, c + 1, nc), red, part + (tile * nc + c) * 128);
+}
+
+// Sums a tile's chunks in chunk order and stores P: grid (tiles, spheres)
+__global__ __launch_bounds__(128) void project_p4_finish_kernel(const project_expand_args a,
+                                                                int nc, const double *part) {
+  __shared__ double sum[128];
+  const int p = blockIdx.y;
+  const long tile = (long)p * gridDim.x + blockIdx.x;
+  const double *q = part + tile * nc * 128;
+  double s = q[threadIdx.x];
+  for (int c = 1; c < nc; c++) s += q[c * 128 + threadIdx.x];
+  sum[threadIdx.x] = s;
+  __syncthreads();
+  p4_store<1>(a, p, blockIdx.x * P4_BANDS, sum);
+}
+
+// Self-test of the lane layout of v_mfma_f64_4x4x4f64: block s of D = A B with
+// A_s[i][k] = 1 + i + 4 k + 16 s and B_s[k][j] = 2 + 3 j - 5 k + 7 s, written as
+// out[s][i][j] (64 doubles).
+__global__ void mfma4_selftest_kernel(double *out) {
+  const int l = threadIdx.x;
+  const int s = (l >> 2) & 3;
+  const double av = 1.0 + (l & 3) + 4.0 * (l >> 4) + 16.0 * s;
+  const double bv = 2.0 + 3.0 * (l & 3) - 5.0 * (l >> 4) + 7.0 * s;
+  const double d = mfma4(av, bv, 0.0);
+  out[s * 16 + (l >> 4) * 4 + (l & 3)] = d;
+}
+
 // Self-test of the lane layout of projector_dev.hpp: D = A B with A[i][k] = 1 + i + 16 k and
 // B[k][j] = 2 + 3 j - 5 k, written row-major into out[16 * 16].
 __global__ void mfma_selftest_kernel(double *out) {
@@ -80,6 +130,70 @@ void projector_project_expand(handle &h, const cplx *psi, long ld, int nst, cplx
   INQAMD_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(expand_chunk_kernel, grid, dim3(PE_NT), 0, h.stream, a, nc,
                      (const double *)h.proj_part);
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+// P = dV D M^T (phase psi) for all spheres and bands on the 4x4x4 FP64 MFMA
+// (the three-sweep Hpsi with V_nl psi expanded in its last sweep)
+void projector_project_p4(handle &h, const cplx *psi, long ld, int nst) {
+  const projector_tables &t = h.pt;
+  if (t.nproj == 0) return;
+  project_expand_args a{};
+  a.psi = psi;
+  a.ld = ld;
+  a.nst = nst;
+  a.max_sphere = t.max_sphere;
+  a.max_nlm = t.max_nlm;
+  a.point = t.point;
+  a.size = t.size;
+  a.mat = t.mat;
+  a.coeff = t.coeff;
+  a.phase = t.phase_sph;
+  a.ent = t.ent;
+  a.dV = h.volume / (double)h.npoint;
+  a.P = h.proj;
+  INQAMD_REQUIRE(t.max_nlm <= 4, "the 4x4x4 projector needs max_nlm <= 4");
+  // bands per workgroup: 32 (INQAMD_P4_BANDS = 16, 32 or 64; at 64^3 x 256
+  // the projector took 62, 61 and 60 us) where that leaves 256 workgroups
+  static const int nb16 = [] {
+    const char *e = getenv("INQAMD_P4_BANDS");
+    const int b = e ? atoi(e) : 32;
+    return b >= 64 ? 4 : (b >= 32 ? 2 : 1);
+  }();
+  const int nbt = (nst + P4_BANDS - 1) / P4_BANDS;
+  const long tiles = (long)nbt * t.nproj;
+  int nc = 1;
+  if (tiles < 512 && h.proj_part) {
+    nc = (int)((1024 + tiles - 1) / tiles);
+    const int by_points = t.max_sphere / 256 > 1 ? t.max_sphere / 256 : 1;
+    if (nc > by_points) nc = by_points;
+    if (nc > PROJ_MAX_CHUNKS) nc = PROJ_MAX_CHUNKS;
+    if (tiles * nc * 128 > PROJ_PART_TILES * 256) nc = (int)(PROJ_PART_TILES * 2 / tiles);
+    if (nc < 1) nc = 1;
+  }
+  if (nc == 1) {
+    const long t32 = (long)((nst + 31) / 32) * t.nproj, t64 = (long)((nst + 63) / 64) * t.nproj;
+    if (nb16 == 4 && t64 >= 256)
+      hipLaunchKernelGGL(project_p4_kernel<4>, dim3((nst + 63) / 64, t.nproj), dim3(P4_NT), 0,
+                         h.stream, a);
+    else if (nb16 >= 2 && t32 >= 256)
+      hipLaunchKernelGGL(project_p4_kernel<2>, dim3((nst + 31) / 32, t.nproj), dim3(P4_NT), 0,
+                         h.stream, a);
+    else
+      hipLaunchKernelGGL(project_p4_kernel<1>, dim3(nbt, t.nproj), dim3(P4_NT), 0, h.stream, a);
+    INQAMD_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  hipLaunchKernelGGL(project_p4_chunk_kernel, dim3(nbt, t.nproj, nc), dim3(P4_NT), 0, h.stream,
+                     a, nc, h.proj_part);
+  INQAMD_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(project_p4_finish_kernel, dim3(nbt, t.nproj), dim3(128), 0, h.stream, a, nc,
+                     (const double *)h.proj_part);
+  INQAMD_HIP_CHECK(hipGetLastError());
+}
+
+void mfma4_selftest(double *out_device, hipStream_t stream) {
+  hipLaunchKernelGGL(mfma4_selftest_kernel, dim3(1), dim3(64), 0, stream, out_device);
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 

@@ -93,6 +93,7 @@ SIGNATURES = [
     ("inqamd_last_error", ctypes.c_char_p, [_vp]),
     ("inqamd_version", ctypes.c_char_p, []),
     ("inqamd_selftest_mfma", _i, [_vp, _vp]),
+    ("inqamd_selftest_mfma_4x4", _i, [_vp, _vp]),
     ("inqamd_stream_probe", _i, [_i, _vp, _vp, ctypes.c_long, _i, _i, _i, _i, _i, _vp,
                                  ctypes.POINTER(ctypes.c_float)]),
     ("inqamd_set_kpoint", _i, [_vp, _c_dbl_p]),

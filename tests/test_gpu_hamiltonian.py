@@ -520,6 +520,23 @@ def test_aa_mfma_f64_lane_layout(device):
     assert np.array_equal(got, A @ B)
 
 
+def test_aa_mfma_f64_4x4x4_lane_layout(device):
+    """The lane layout of the 4-block 4x4x4 FP64 matrix instruction the
+    projector coefficients of the three-sweep Hpsi use."""
+    say("mfma 4x4x4 selftest")
+    out = torch.zeros(64, dtype=torch.float64, device=device)
+    L = inq_amd._lib.lib()
+    inq_amd._lib.check(L.inqamd_selftest_mfma_4x4(
+        ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    got = out.cpu().numpy().reshape(4, 4, 4)
+    i = np.arange(4)[:, None]
+    k = np.arange(4)[None, :]
+    for s in range(4):
+        A = 1.0 + i + 4.0 * k + 16.0 * s
+        B = 2.0 + 3.0 * np.arange(4)[None, :] - 5.0 * np.arange(4)[:, None] + 7.0 * s
+        assert np.array_equal(got[s], A @ B), s
+
+
 @pytest.mark.parametrize("env", [{"INQAMD_STREAMS": "3"}, {"INQAMD_BAND_GROUP_SIZE": "16"},
                                  {"INQAMD_BAND_GROUP_SIZE": "32", "INQAMD_BAND_CHUNK": "64"}],
                          ids=["three-streams", "groups-16", "groups-32-chunk-64"])
