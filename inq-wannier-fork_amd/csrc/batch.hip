@@ -44,7 +44,7 @@ struct batch_phase_sys {
 
 struct batch_proj_sys {
   project_expand_args a;
-  int nbt;  // band tiles of PE_BANDS
+  int nbt;  // band tiles of PE_BANDS (per-entry V_nl psi) or P4_BANDS (coefficients)
   int item0;
 };
 
@@ -163,6 +163,23 @@ __global__ __launch_bounds__(PE_NT) void batch_project_kernel(
   const int local = item - sys[s].item0;
   const int nbt = sys[s].nbt;
   project_expand_block(sys[s].a, local / nbt, (local % nbt) * PE_BANDS);
+}
+
+// Projector coefficients only (the sweeps expand V_nl psi): items (sphere,
+// 16-band tile) on the 4x4x4 FP64 MFMA, one workgroup per item.
+__global__ __launch_bounds__(P4_NT) void batch_project_p4_kernel(
+    const batch_proj_sys *__restrict__ sys, int nsys, unsigned long long *, int) {
+  __shared__ double red[P4_WAVES][1][64][2];
+  __shared__ double sum[128];
+  const int item = blockIdx.x;
+  const int s = find_segment(nsys, item, [&](int i) { return sys[i].item0; });
+  const int local = item - sys[s].item0;
+  const int nbt = sys[s].nbt;
+  const project_expand_args &a = sys[s].a;
+  const int p = local / nbt, b0 = (local % nbt) * P4_BANDS;
+  gemm1_p4_points<1>(a, p, b0, 0, a.size[p], red, sum);
+  __syncthreads();
+  p4_store<1>(a, p, b0, sum);
 }
 
 __global__ __launch_bounds__(Z_NT) void batch_z_fwd_kernel(const batch_piece *__restrict__ pc, int np,
@@ -358,6 +375,9 @@ struct inqamd_batch {
   std::vector<group> groups;
   std::vector<chunk> chunks;
   int proj_grid = 0;
+  // V_nl psi expanded in the sweeps from the projector coefficients (every
+  // member with projectors allows it), else per ownership entry
+  bool coefficients = false;
   unsigned long long *counters = nullptr;
   int ncounters = 0;
   std::vector<int> fallback;  // systems run through the one-system Hpsi
@@ -414,6 +434,13 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
     int s, b0, nb, ty;
   };
   std::vector<pend> pieces;
+  b->coefficients = true;
+  for (int s = 0; s < nsys; s++) {
+    const handle &h = *b->hs[s];
+    if (h.three_sweep && batch_supported_length(h.n[0]) && h.pt.nproj > 0 &&
+        h.pt.nunique > 0 && !vnl_from_coefficients(h))
+      b->coefficients = false;
+  }
   for (int s = 0; s < nsys; s++) {
     handle &h = *b->hs[s];
     if (!h.three_sweep || !batch_supported_length(h.n[0])) {
@@ -435,7 +462,7 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       ph.push_back(q);
     }
     if (t.nproj > 0 && t.nunique > 0) {
-      batch_proj_sys q;
+      batch_proj_sys q{};
       project_expand_args &a = q.a;
       a.psi = (const cplx *)psi[s];
       a.ld = ld[s];
@@ -449,10 +476,16 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       a.phase = t.phase_sph;
       a.ent = t.ent;
       a.dV = h.volume / (double)h.npoint;
-      a.P = nullptr;
-      a.Qe = h.vnl;
-      a.ldQ = nst[s];
-      q.nbt = (nst[s] + PE_BANDS - 1) / PE_BANDS;
+      if (b->coefficients) {
+        a.P = h.proj;
+        a.Qe = nullptr;
+        q.nbt = (nst[s] + P4_BANDS - 1) / P4_BANDS;
+      } else {
+        a.P = nullptr;
+        a.Qe = h.vnl;
+        a.ldQ = nst[s];
+        q.nbt = (nst[s] + PE_BANDS - 1) / PE_BANDS;
+      }
       q.item0 = proj_items;
       proj_items += q.nbt * t.nproj;
       pj.push_back(q);
@@ -519,7 +552,16 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       for (int d = 0; d < 9; d++) a.rlat[d] = h.rlat[d];
       a.tw = h.axis[0].tw;
       a.scale = 1.0 / (double)h.npoint;
-      if (h.pt.nproj > 0 && h.pt.nunique > 0) {
+      if (h.pt.nproj > 0 && h.pt.nunique > 0 && b->coefficients) {
+        a.own = h.pt.own;
+        a.Pc = h.proj + p.b0;
+        a.ldPc = nst[p.s];
+        a.nlmP = h.pt.max_nlm;
+        a.ent_sph = h.pt.ent_sph;
+        a.own_sph = h.pt.own_sph;
+        a.ent_mat = h.pt.ent_mat;
+        a.phase_ent = h.pt.phase_ent;
+      } else if (h.pt.nproj > 0 && h.pt.nunique > 0) {
         a.own = h.pt.own;
         a.Qe = h.vnl + p.b0;
         a.ldQ = nst[p.s];
@@ -618,7 +660,11 @@ void batch_run(inqamd_batch *b) {
                        (const batch_phase_sys *)b->d_phase, b->nphase, b->phase_items);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
-  if (b->proj_items > 0) {
+  if (b->proj_items > 0 && b->coefficients) {
+    hipLaunchKernelGGL(batch_project_p4_kernel, dim3(b->proj_grid), dim3(P4_NT), 0, st,
+                       (const batch_proj_sys *)b->d_proj, b->nproj, b->counters, b->proj_items);
+    INQAMD_HIP_CHECK(hipGetLastError());
+  } else if (b->proj_items > 0) {
     hipLaunchKernelGGL(batch_project_kernel, dim3(b->proj_grid), dim3(PE_NT), 0, st,
                        (const batch_proj_sys *)b->d_proj, b->nproj, b->counters, b->proj_items);
     INQAMD_HIP_CHECK(hipGetLastError());

@@ -249,21 +249,12 @@ void fft3d(handle &h, const cplx *in, long in_ld, cplx *out, long out_ld, int ns
   fft_launch_pass(pass_args(h, 2, out, out_ld, out, out_ld, nst, sign, scale), FFT_PLAIN, h.stream);
 }
 
-// Three-sweep Hpsi: V_nl psi expanded in the last sweep from the projector
-// coefficients P (the projector writes only P, 1 MB at 64^3 x 256, on the
-// 4x4x4 FP64 MFMA), rather than from per-entry V_nl psi values that the
-// projector writes and the sweep reads back (16 B per sphere point and band
-// each way): 64^3 x 256 Hpsi 1.70 -> 1.65 ms, the 96^3 ETRS step -1.2 %.
-// Needs max_nlm <= 4.  INQAMD_VNL=entries keeps the per-entry form.
-static bool vnl_from_coefficients(const handle &h) {
-  static const bool entries = [] {
-    const char *e = getenv("INQAMD_VNL");
-    return e && std::string(e) == "entries";
-  }();
-  return !entries && h.three_sweep && h.pt.nproj > 0 && h.pt.nunique > 0 && h.pt.ent_mat;
-}
-
-// The projector step of the three-sweep Hpsi (after projector_phases)
+// The projector step of the three-sweep Hpsi (after projector_phases).  Where
+// vnl_from_coefficients holds, the last sweep expands V_nl psi from the
+// projector coefficients P (the projector writes only P, 1 MB at 64^3 x 256,
+// on the 4x4x4 FP64 MFMA), rather than from per-entry V_nl psi values that
+// the projector writes and the sweep reads back (16 B per sphere point and
+// band each way): 64^3 x 256 Hpsi 1.70 -> 1.65 ms, the 96^3 ETRS step -1.2 %.
 static void hpsi_projector(handle &h, const cplx *psi, long ld, int nst) {
   if (vnl_from_coefficients(h))
     projector_project_p4(h, psi, ld, nst);

@@ -6,6 +6,7 @@
 #include "fft.hpp"
 
 #include <complex>
+#include <cstdlib>
 #include <exception>
 #include <new>
 #include <string>
@@ -189,6 +190,17 @@ void projector_expand(handle &h, cplx *sphere, int nst);
 // hpsi(r, b) += sum_{(p,i) at r} e^{-i(k+A).r_pi} sphere[p][i][b]
 void projector_apply_sphere(handle &h, const cplx *sphere, cplx *hpsi, long ld, int nst, int slot);
 
+
+// Whether the three-sweep Hpsi of this handle expands V_nl psi in its last
+// sweep from the projector coefficients (needs max_nlm <= 4; INQAMD_VNL=entries
+// keeps V_nl psi per ownership entry instead)
+inline bool vnl_from_coefficients(const handle &h) {
+  static const bool entries = [] {
+    const char *e = getenv("INQAMD_VNL");
+    return e && std::string(e) == "entries";
+  }();
+  return !entries && h.three_sweep && h.pt.nproj > 0 && h.pt.nunique > 0 && h.pt.ent_mat;
+}
 
 // C ABI plumbing shared by the translation units that export functions
 template <typename F>
