@@ -479,6 +479,7 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       if (b->coefficients) {
         a.P = h.proj;
         a.Qe = nullptr;
+        a.nbc = nst[s];
         q.nbt = (nst[s] + P4_BANDS - 1) / P4_BANDS;
       } else {
         a.P = nullptr;

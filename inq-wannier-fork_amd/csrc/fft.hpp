@@ -128,6 +128,9 @@ struct fft3_args {
   cplx *gdump;
   // hpsi written with non-temporal stores (streamed past the caches)
   int nt_store;
+  // hpsi not written at all (the last term of a Taylor chain: only the
+  // accumulators need it)
+  int skip_store;
   // G-space Hpsi sweeps: G-space orbitals whose 1/2 |G+k+A|^2 multiple the
   // last sweep adds (fft3_fourier_hamiltonian)
   const cplx *kin_in;
@@ -147,6 +150,7 @@ bool fft3_supported(int L, size_t max_lds);
 void fft3_prepare(int L);
 // hpsi = IFFT(1/2|G+k+A|^2 FFT(psi)) / N + V psi for the chunk's bands: all
 // three sweeps (sweep < 0) or only sweep 0 (z forward), 1 (x-y) or 2 (z back).
-void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream, int sweep = -1);
+void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream, int sweep = -1,
+                            bool fwd_next = false);
 
 }  // namespace inqamd

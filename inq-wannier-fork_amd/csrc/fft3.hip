@@ -36,12 +36,12 @@ void xy_kin_kernel(const fft3_args a) {
   xy_kin_plane<R0, R1, R2, NT>(a, blockIdx.x, lds);
 }
 
-template <int R0, int R1, int R2, int TY, bool ACC>
+template <int R0, int R1, int R2, int TY, bool ACC, bool FWD = false>
 __global__ __launch_bounds__(z_threads(R0 * R1 * R2))
 __attribute__((amdgpu_waves_per_eu(z_bwd_waves(R0 * R1 * R2))))
 void z_bwd_epi_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), ACC>(a, blockIdx.x, lds);
+  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), ACC, false, FWD>(a, blockIdx.x, lds);
 }
 
 // G-space Hpsi sweeps
@@ -106,10 +106,14 @@ static void prepare3() {
   ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, false>, zl);
   ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true>, zl);
   ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true>, zl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true, true>, zl);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true, true>, zl);
 }
 
+// sweep: -1 all three, 0/1/2 one of them; fwd_next: the last sweep also
+// transforms its output forward in z into W (Taylor chains)
 template <int R0, int R1, int R2, int TY, int NT>
-static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
+static void launch3v(const fft3_args &a, hipStream_t stream, int sweep, bool fwd_next) {
   constexpr int L = R0 * R1 * R2;
   const long ztiles = z_tiles<TY>(L, a.nb);
   const size_t zlds = (size_t)L * Z_COLS * sizeof(cplx);
@@ -125,7 +129,10 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   if (sweep < 0 || sweep == 2) {
-    if (a.acc1 || a.acc2)
+    if (fwd_next)
+      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true, true>), dim3((unsigned)ztiles),
+                         dim3(z_threads(L)), zlds, stream, a);
+    else if (a.acc1 || a.acc2)
       hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true>), dim3((unsigned)ztiles),
                          dim3(z_threads(L)), zlds, stream, a);
     else
@@ -136,19 +143,19 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep) {
 }
 
 template <int R0, int R1, int R2>
-static void launch3(const fft3_args &a, hipStream_t stream, int sweep) {
+static void launch3(const fft3_args &a, hipStream_t stream, int sweep, bool fwd_next) {
   constexpr int L = R0 * R1 * R2;
   const bool wide = xy_wide_choice();
   if (z_ty_choice() == 16) {
     if (wide)
-      launch3v<R0, R1, R2, 16, wide_threads(L)>(a, stream, sweep);
+      launch3v<R0, R1, R2, 16, wide_threads(L)>(a, stream, sweep, fwd_next);
     else
-      launch3v<R0, R1, R2, 16, plane_threads(L)>(a, stream, sweep);
+      launch3v<R0, R1, R2, 16, plane_threads(L)>(a, stream, sweep, fwd_next);
   } else {
     if (wide)
-      launch3v<R0, R1, R2, 8, wide_threads(L)>(a, stream, sweep);
+      launch3v<R0, R1, R2, 8, wide_threads(L)>(a, stream, sweep, fwd_next);
     else
-      launch3v<R0, R1, R2, 8, plane_threads(L)>(a, stream, sweep);
+      launch3v<R0, R1, R2, 8, plane_threads(L)>(a, stream, sweep, fwd_next);
   }
 }
 
@@ -200,10 +207,11 @@ void fft3_prepare(int L) {
   }
 }
 
-void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream, int sweep) {
+void fft3_local_hamiltonian(const fft3_args &a, int L, hipStream_t stream, int sweep,
+                            bool fwd_next) {
   switch (L) {
 #define INQAMD_FFT3_CASE(LL, R0, R1, R2) \
-  case LL: launch3<R0, R1, R2>(a, stream, sweep); return;
+  case LL: launch3<R0, R1, R2>(a, stream, sweep, fwd_next); return;
     INQAMD_FFT3_TABLE(INQAMD_FFT3_CASE)
 #undef INQAMD_FFT3_CASE
     default: throw error{ERR_UNSUPPORTED, "three-sweep Hpsi: unsupported grid length"};

@@ -345,7 +345,11 @@ __device__ __forceinline__ void xy_fwd_plane(const fft3_args &a, long plane, cpl
 // GS (G-space Hpsi): no z transform (the first sweep made it), only the
 // transposing write, whose epilogue adds 1/2 |G+k+A|^2 kin_in instead of
 // V psi (no ownership entries, no scale).
-template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool ACC = false, bool GS = false>
+// FWD (Taylor chains): the tile then transforms its own hpsi forward in z
+// into W, in place of the next application's z_fwd sweep over the same tile
+// (the chunk's W region the tile read is the region it writes).
+template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool ACC = false, bool GS = false,
+          bool FWD = false>
 __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
@@ -487,6 +491,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     return v;
   };
   auto store = [&](long p, int cb, cplx v) {
+    if (a.skip_store) return;
     if (a.nt_store) {
       double *o = reinterpret_cast<double *>(a.hpsi + p * a.hld + cb);
       __builtin_nontemporal_store(v.x, o);
@@ -500,8 +505,11 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     for (int k = 0; k < NE; k++) {
       const int e = threadIdx.x + k * NT;
       const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-      if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb)
-        store(((long)x * L + y0 + y) * L + z, gb0 + b, result(k, b, y, z));
+      if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
+        const cplx v = result(k, b, y, z);
+        store(((long)x * L + y0 + y) * L + z, gb0 + b, v);
+        if constexpr (FWD) lds[z * Z_COLS + zswz(b * Z_TY + y)] = v;
+      }
     }
   } else {
     // Outputs in groups of G: the group's Taylor accumulators are read
@@ -532,6 +540,7 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
           const int cb = gb0 + b;
           const cplx v = result(k0 + g, b, y, z);
           store(p, cb, v);
+          if constexpr (FWD) lds[z * Z_COLS + zswz(b * Z_TY + y)] = v;
           if (a.acc1) {
             cplx o = o1[g];
             o.x += a.c1re * v.x - a.c1im * v.y;
@@ -547,6 +556,16 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
         }
       }
     }
+  }
+  if constexpr (FWD) {
+    // each thread rewrote only the slots it read; the forward plan mixes them
+    __syncthreads();
+    auto gs = [&](int i, int c, cplx v, int, int) {
+      const int b = c / Z_TY, y = c % Z_TY;
+      if (y0 + y < L && cb0 + b < a.nb)
+        a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y] = v;
+    };
+    run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, -1, true, ll, ll, ls, gs, false);
   }
 }
 

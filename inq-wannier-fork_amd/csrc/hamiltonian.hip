@@ -269,7 +269,7 @@ static void hpsi_projector(handle &h, const cplx *psi, long ld, int nst) {
 static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld, int nb,
                          int slot, int b0, long ldP, const taylor_acc *acc, int sweep = -1,
                          hipStream_t stream = nullptr, cplx *W = nullptr, int gskip = 0,
-                         int glog = 0) {
+                         int glog = 0, bool fwd_next = false, bool skip_store = false) {
   if (h.three_sweep) {
     fft3_args a{};
     a.gskip = gskip;
@@ -322,7 +322,8 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       return e && atoi(e) == 1 ? 1 : 0;
     }();
     a.nt_store = nt_store;
-    fft3_local_hamiltonian(a, h.n[0], stream ? stream : h.stream, sweep);
+    a.skip_store = skip_store ? 1 : 0;
+    fft3_local_hamiltonian(a, h.n[0], stream ? stream : h.stream, sweep, fwd_next);
     return;
   }
   cplx *w = h.work;
@@ -539,10 +540,95 @@ void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst) {
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 
+// Taylor chains on three-sweep grids (exponential.hpp:21-69): for
+// n = 1..order, H^n psi = H (H^{n-1} psi) with acc1 += c1_n H^n psi (acc1 =
+// in + c1_1 H in when init1) and acc2 += c2_n H^n psi, c_n = z^n / n!.  Run
+// band chunk by band chunk with every order of a chunk on one stream: the
+// projector takes the chunk's bands of H^{n-1} psi, and each order's last
+// sweep also transforms its output forward in z into W for the next order,
+// so no z_fwd sweep reads H^n psi back (16 B per point and band per order);
+// the last order writes only the accumulators.  Chunks alternate between two
+// streams.  The same arithmetic as one Hpsi per order: results are equal bit
+// for bit.  Needs V_nl psi from coefficients (or no projectors);
+// INQAMD_TAYLOR_CHAIN=0 runs one Hpsi per order.
+static bool taylor_chain_ok(const handle &h) {
+  static const bool enabled = [] {
+    const char *e = getenv("INQAMD_TAYLOR_CHAIN");
+    return !(e && atoi(e) == 0);
+  }();
+  return enabled && h.three_sweep &&
+         (h.pt.nproj == 0 || h.pt.nunique == 0 || vnl_from_coefficients(h));
+}
+
+static void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order, int slot,
+                         cplx *p1, long ld1, bool init1, std::complex<double> z1, cplx *p2,
+                         long ld2, std::complex<double> z2) {
+  std::vector<std::complex<double>> c1(order), c2(order);
+  {
+    std::complex<double> a = 1.0, b = 1.0;
+    for (int n = 1; n <= order; n++) {
+      a *= z1 / double(n);
+      b *= z2 / double(n);
+      c1[n - 1] = a;
+      c2[n - 1] = b;
+    }
+  }
+  const band_chunks plan = chunk_plan(h, nst, ild, true);
+  static const bool pipeline = [] {
+    const char *e = getenv("INQAMD_PIPELINE");
+    return !(e && atoi(e) == 0);
+  }();
+  const bool two = pipeline && h.aux && plan.chunk < nst && 2 * plan.chunk <= h.max_bands;
+  const bool proj = h.pt.nproj > 0 && h.pt.nunique > 0;
+  projector_phases(h, slot);
+  if (two) {
+    INQAMD_HIP_CHECK(hipEventRecord(h.ev_fork, h.stream));
+    INQAMD_HIP_CHECK(hipStreamWaitEvent(h.aux, h.ev_fork, 0));
+  }
+  cplx *buf[2] = {h.hn, h.hn2};
+  for (int c = 0; c < plan.n; c++) {
+    const int b0 = plan.b0(c), nb = plan.nb(c);
+    hipStream_t s = two && (c & 1) ? h.aux : h.stream;
+    cplx *W = two ? h.work + (size_t)(c & 1) * plan.chunk * h.npoint : h.work;
+    for (int n = 1; n <= order; n++) {
+      const cplx *src = n == 1 ? in + b0 : buf[(n - 1) & 1] + b0;
+      const long sld = n == 1 ? ild : nst;
+      cplx *dst = buf[n & 1] + b0;
+      if (proj)
+        projector_project_p4_chunk(h, src, sld, nst, b0, nb, plan.gskip, plan.glog, s,
+                                   two ? (c & 1) : 0);
+      taylor_acc acc;
+      acc.p1 = p1;
+      acc.ld1 = ld1;
+      acc.c1re = c1[n - 1].real();
+      acc.c1im = c1[n - 1].imag();
+      acc.init1 = init1 && n == 1;
+      acc.p2 = p2;
+      acc.ld2 = ld2;
+      acc.c2re = c2[n - 1].real();
+      acc.c2im = c2[n - 1].imag();
+      if (n == 1)
+        local_passes(h, src, sld, dst, nst, nb, slot, b0, nst, &acc, 0, s, W, plan.gskip,
+                     plan.glog);
+      local_passes(h, src, sld, dst, nst, nb, slot, b0, nst, &acc, 1, s, W, plan.gskip, plan.glog);
+      local_passes(h, src, sld, dst, nst, nb, slot, b0, nst, &acc, 2, s, W, plan.gskip, plan.glog,
+                   n < order, n == order);
+    }
+  }
+  if (two) {
+    INQAMD_HIP_CHECK(hipEventRecord(h.ev_join, h.aux));
+    INQAMD_HIP_CHECK(hipStreamWaitEvent(h.stream, h.ev_join, 0));
+  }
+}
+
 // exponential_in_place (exponential.hpp:21-34): the shift of each order is
 // carried by the Hpsi that produces the term.
 void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
                                  int nst, int slot) {
+  if (taylor_chain_ok(h)) {
+    taylor_chain(h, psi, ld, nst, order, slot, psi, ld, false, z, nullptr, 0, 0.0);
+    return;
+  }
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
   std::complex<double> coeff = 1.0;
@@ -567,6 +653,10 @@ void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *ps
 // their own leading dimensions
 void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, long ild,
                       cplx *out, long old, int nst, int slot) {
+  if (taylor_chain_ok(h)) {
+    taylor_chain(h, in, ild, nst, order, slot, out, old, true, z, nullptr, 0, 0.0);
+    return;
+  }
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
   std::complex<double> coeff = 1.0;
@@ -592,6 +682,10 @@ void exponential_from(handle &h, std::complex<double> z, int order, const cplx *
 void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double> z2,
                                 int order, cplx *phi, cplx *expphi, long ld, long eld, int nst,
                                 int slot) {
+  if (taylor_chain_ok(h)) {
+    taylor_chain(h, phi, ld, nst, order, slot, expphi, eld, true, z1, phi, ld, z2);
+    return;
+  }
   cplx *hn = h.hn, *hn2 = h.hn2;
   const long tl = nst;
   std::complex<double> c1 = 1.0, c2 = 1.0;

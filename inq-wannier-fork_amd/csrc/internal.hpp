@@ -64,6 +64,11 @@ struct project_expand_args {
   cplx *P;            // [nproj][max_nlm][nst] or nullptr
   cplx *Qe;           // [nent][ldQ] V_nl psi per ownership entry, or nullptr
   long ldQ;
+  // the 4x4x4 coefficient kernels only: nbc bands of a chunk whose band cb
+  // sits at psi[cb + (cb >> (3 + glog)) gskip] (psi offset to the chunk's
+  // first band) and is written to P at band pband0 + the same offset
+  int nbc, gskip, glog;
+  long pband0;
 };
 
 // psi_acc += c * (H psi) accumulations fused into the Hpsi epilogue (Taylor
@@ -153,6 +158,12 @@ void projector_project_expand(handle &h, const cplx *psi, long ld, int nst, cplx
 void mfma_selftest(double *out_device, hipStream_t stream);
 // P coefficients of all spheres and bands on the 4x4x4 FP64 MFMA (max_nlm <= 4)
 void projector_project_p4(handle &h, const cplx *psi, long ld, int nst);
+// The same for the nb bands of one band chunk (psi offset to its first band
+// b0; band groups as in fft3_args) on `stream`: P at bands b0 + offset
+// (part_slot: which of PROJ_PART_SLOTS regions of the partial-tile buffer
+// its point-chunked form uses; concurrent calls take different slots)
+void projector_project_p4_chunk(handle &h, const cplx *psi, long ld, int nst, int b0, int nb,
+                                int gskip, int glog, hipStream_t stream, int part_slot);
 void mfma4_selftest(double *out_device, hipStream_t stream);
 
 // hamiltonian.hip
@@ -183,6 +194,7 @@ constexpr int OBS_NRECORDS = 16;
 constexpr int MOM_MAX_CHUNKS = 16;     // point chunks per sphere in the moments kernel
 constexpr int PROJ_MAX_CHUNKS = 16;    // point chunks per sphere in the projector
 constexpr long PROJ_PART_TILES = 4096; // 16 x 16 GEMM1 tiles the chunk buffer holds
+constexpr int PROJ_PART_SLOTS = 4;     // regions of it for concurrent coefficient launches
 constexpr long MOM_PARTIAL_BLOCKS = 4096;  // workgroups whose partials the buffer holds
 long observables_scratch_doubles(const handle &h);
 // sphere[p][i][b] = sum_lm M[p][lm][i] P[p][lm][b]   (INQ-shaped project output)

@@ -259,7 +259,15 @@ __device__ __forceinline__ void gemm1_p4_points(const project_expand_args &a, in
   const double *M = a.mat + (long)p * nlm * a.max_sphere;
   const int kk = lane >> 4;
   const int lm_a = lane & 3;
-  const int band = b0 + (lane & 15);
+  // the lane's chunk bands b0 + (lane & 15) + 16 q and their offsets in psi
+  int gband[NB];
+  bool bok[NB];
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    const int cb = b0 + (lane & 15) + 16 * q;
+    bok[q] = cb < a.nbc;
+    gband[q] = cb + (cb >> (3 + a.glog)) * a.gskip;
+  }
   double accr[NB], acci[NB];
 #pragma unroll
   for (int q = 0; q < NB; q++) accr[q] = acci[q] = 0.0;
@@ -278,8 +286,7 @@ __device__ __forceinline__ void gemm1_p4_points(const project_expand_args &a, in
       const int i = j0 + kk + u * STEP;
 #pragma unroll
       for (int q = 0; q < NB; q++)
-        ps[u][q] = (idx[u] >= 0 && band + 16 * q < a.nst)
-                       ? a.psi[(long)idx[u] * a.ld + band + 16 * q] : mk(0.0, 0.0);
+        ps[u][q] = (idx[u] >= 0 && bok[q]) ? a.psi[(long)idx[u] * a.ld + gband[q]] : mk(0.0, 0.0);
       ph[u] = i < ie ? a.phase[off + i] : mk(0.0, 0.0);
       av[u] = (i < ie && lm_a < nlm) ? M[(long)lm_a * a.max_sphere + i] : 0.0;
     }
@@ -313,8 +320,9 @@ __device__ __forceinline__ void p4_store(const project_expand_args &a, int p, in
                                          const double *sum) {
   for (int t = threadIdx.x; t < NB * 64; t += blockDim.x) {
     const int q = t >> 6, l = t & 63;
-    const int lm = l >> 4, b = b0 + 16 * q + (l & 15);
-    if (lm < a.max_nlm && b < a.nst) {
+    const int lm = l >> 4, cb = b0 + 16 * q + (l & 15);
+    if (lm < a.max_nlm && cb < a.nbc) {
+      const long b = a.pband0 + cb + (cb >> (3 + a.glog)) * a.gskip;
       const double d = a.dV * a.coeff[(long)p * a.max_nlm + lm];
       a.P[((long)p * a.max_nlm + lm) * a.nst + b] = mk(d * sum[2 * t], d * sum[2 * t + 1]);
     }

@@ -135,7 +135,15 @@ void projector_project_expand(handle &h, const cplx *psi, long ld, int nst, cplx
 
 // P = dV D M^T (phase psi) for all spheres and bands on the 4x4x4 FP64 MFMA
 // (the three-sweep Hpsi with V_nl psi expanded in its last sweep)
+static void project_p4_launch(handle &h, const project_expand_args &a, int nst, int nst_all,
+                              hipStream_t st, double *part, long part_doubles);
+
 void projector_project_p4(handle &h, const cplx *psi, long ld, int nst) {
+  projector_project_p4_chunk(h, psi, ld, nst, 0, nst, 0, 0, h.stream, 0);
+}
+
+void projector_project_p4_chunk(handle &h, const cplx *psi, long ld, int nst, int b0, int nb,
+                                int gskip, int glog, hipStream_t stream, int part_slot) {
   const projector_tables &t = h.pt;
   if (t.nproj == 0) return;
   project_expand_args a{};
@@ -152,7 +160,24 @@ void projector_project_p4(handle &h, const cplx *psi, long ld, int nst) {
   a.ent = t.ent;
   a.dV = h.volume / (double)h.npoint;
   a.P = h.proj;
+  a.nbc = nb;
+  a.gskip = gskip;
+  a.glog = glog;
+  a.pband0 = b0;
   INQAMD_REQUIRE(t.max_nlm <= 4, "the 4x4x4 projector needs max_nlm <= 4");
+  INQAMD_REQUIRE(part_slot >= 0 && part_slot < PROJ_PART_SLOTS, "projector partial slot");
+  const long slot_doubles = PROJ_PART_TILES * 256 / PROJ_PART_SLOTS;
+  project_p4_launch(h, a, nb, nst, stream,
+                    h.proj_part ? h.proj_part + part_slot * slot_doubles : nullptr, slot_doubles);
+}
+
+// Launches of the 4x4x4 coefficient kernels over the nst (chunk) bands of `a`;
+// the point-chunked form keeps its partial tiles in part[part_doubles].  The
+// point chunking follows from all nst_all bands, so a band's coefficients are
+// the same bits whichever chunk computes them.
+static void project_p4_launch(handle &h, const project_expand_args &a, int nst, int nst_all,
+                              hipStream_t st, double *part, long part_doubles) {
+  const projector_tables &t = h.pt;
   // bands per workgroup: 32 (INQAMD_P4_BANDS = 16, 32 or 64; at 64^3 x 256
   // the projector took 62, 61 and 60 us) where that leaves 256 workgroups
   static const int nb16 = [] {
@@ -161,34 +186,34 @@ void projector_project_p4(handle &h, const cplx *psi, long ld, int nst) {
     return b >= 64 ? 4 : (b >= 32 ? 2 : 1);
   }();
   const int nbt = (nst + P4_BANDS - 1) / P4_BANDS;
-  const long tiles = (long)nbt * t.nproj;
+  const long tiles = (long)((nst_all + P4_BANDS - 1) / P4_BANDS) * t.nproj;
   int nc = 1;
-  if (tiles < 512 && h.proj_part) {
+  if (tiles < 512 && part) {
     nc = (int)((1024 + tiles - 1) / tiles);
     const int by_points = t.max_sphere / 256 > 1 ? t.max_sphere / 256 : 1;
     if (nc > by_points) nc = by_points;
     if (nc > PROJ_MAX_CHUNKS) nc = PROJ_MAX_CHUNKS;
-    if (tiles * nc * 128 > PROJ_PART_TILES * 256) nc = (int)(PROJ_PART_TILES * 2 / tiles);
+    if (tiles * nc * 128 > part_doubles) nc = (int)(part_doubles / 128 / tiles);
     if (nc < 1) nc = 1;
   }
   if (nc == 1) {
     const long t32 = (long)((nst + 31) / 32) * t.nproj, t64 = (long)((nst + 63) / 64) * t.nproj;
     if (nb16 == 4 && t64 >= 256)
       hipLaunchKernelGGL(project_p4_kernel<4>, dim3((nst + 63) / 64, t.nproj), dim3(P4_NT), 0,
-                         h.stream, a);
+                         st, a);
     else if (nb16 >= 2 && t32 >= 256)
       hipLaunchKernelGGL(project_p4_kernel<2>, dim3((nst + 31) / 32, t.nproj), dim3(P4_NT), 0,
-                         h.stream, a);
+                         st, a);
     else
-      hipLaunchKernelGGL(project_p4_kernel<1>, dim3(nbt, t.nproj), dim3(P4_NT), 0, h.stream, a);
+      hipLaunchKernelGGL(project_p4_kernel<1>, dim3(nbt, t.nproj), dim3(P4_NT), 0, st, a);
     INQAMD_HIP_CHECK(hipGetLastError());
     return;
   }
-  hipLaunchKernelGGL(project_p4_chunk_kernel, dim3(nbt, t.nproj, nc), dim3(P4_NT), 0, h.stream,
-                     a, nc, h.proj_part);
+  hipLaunchKernelGGL(project_p4_chunk_kernel, dim3(nbt, t.nproj, nc), dim3(P4_NT), 0, st,
+                     a, nc, part);
   INQAMD_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(project_p4_finish_kernel, dim3(nbt, t.nproj), dim3(128), 0, h.stream, a, nc,
-                     (const double *)h.proj_part);
+  hipLaunchKernelGGL(project_p4_finish_kernel, dim3(nbt, t.nproj), dim3(128), 0, st, a, nc,
+                     (const double *)part);
   INQAMD_HIP_CHECK(hipGetLastError());
 }
 

@@ -148,3 +148,30 @@ def test_propagate_interacting_matches_oracle(device):
         assert abs(r["xc"] - e["exc"]) <= 1e-10 * abs(e["exc"])
         assert abs(r["ext"] - e["external"]) <= 1e-10 * abs(e["external"])
     ham.close()
+
+
+def test_taylor_chain_equals_one_hpsi_per_order(device, tmp_path):
+    """The chunk-by-chunk Taylor chain (each order's last sweep transforms its
+    output forward for the next order) against one Hpsi per order
+    (INQAMD_TAYLOR_CHAIN=0), bitwise: exponential_in_place, 2-for-1, ETRS
+    steps at 48^3 and 64^3 (band groups), a captured step and the interacting
+    ETRS.  The library reads the switch once per process, so each form runs
+    in a child process (tests/taylor_chain_dump.py)."""
+    import os
+    import subprocess
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "taylor_chain_dump.py")
+    res = {}
+    for chain in ("1", "0"):
+        say("taylor chain", chain)
+        e = dict(os.environ)
+        e["INQAMD_TAYLOR_CHAIN"] = chain
+        path = str(tmp_path / f"chain{chain}.npz")
+        r = subprocess.run([sys.executable, "-u", script, path], env=e, capture_output=True,
+                           text=True, timeout=240)
+        say(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0 and "CHAIN-DUMP-OK" in r.stdout
+        res[chain] = np.load(path)
+    for k in res["0"].files:
+        same = np.array_equal(res["1"][k], res["0"][k])
+        say(k, "bitwise equal" if same else "DIFFERS")
+        assert same, k
