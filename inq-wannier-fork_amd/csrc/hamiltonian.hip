@@ -551,7 +551,7 @@ void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst) {
 // streams.  The same arithmetic as one Hpsi per order: results are equal bit
 // for bit.  Needs V_nl psi from coefficients (or no projectors);
 // INQAMD_TAYLOR_CHAIN=0 runs one Hpsi per order.
-static bool taylor_chain_ok(const handle &h) {
+bool taylor_chain_ok(const handle &h) {
   static const bool enabled = [] {
     const char *e = getenv("INQAMD_TAYLOR_CHAIN");
     return !(e && atoi(e) == 0);
@@ -560,9 +560,13 @@ static bool taylor_chain_ok(const handle &h) {
          (h.pt.nproj == 0 || h.pt.nunique == 0 || vnl_from_coefficients(h));
 }
 
-static void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order, int slot,
-                         cplx *p1, long ld1, bool init1, std::complex<double> z1, cplx *p2,
-                         long ld2, std::complex<double> z2) {
+// outs[n - 1] (ld nst) receives H^n in when given (nullptr: the handle's
+// ping-pong buffers, and for the last order nothing); dumps[n - 1] receives
+// the forward transform of order n's input (H^{n-1} in), band-major, as the
+// x-y sweep computes it (taylor_acc::gdump; contiguous band chunks then).
+void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order, int slot, cplx *p1,
+                  long ld1, bool init1, std::complex<double> z1, cplx *p2, long ld2,
+                  std::complex<double> z2, cplx *const *outs, cplx *const *dumps) {
   std::vector<std::complex<double>> c1(order), c2(order);
   {
     std::complex<double> a = 1.0, b = 1.0;
@@ -573,7 +577,7 @@ static void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order
       c2[n - 1] = b;
     }
   }
-  const band_chunks plan = chunk_plan(h, nst, ild, true);
+  const band_chunks plan = chunk_plan(h, nst, ild, dumps == nullptr);
   static const bool pipeline = [] {
     const char *e = getenv("INQAMD_PIPELINE");
     return !(e && atoi(e) == 0);
@@ -590,10 +594,11 @@ static void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order
     const int b0 = plan.b0(c), nb = plan.nb(c);
     hipStream_t s = two && (c & 1) ? h.aux : h.stream;
     cplx *W = two ? h.work + (size_t)(c & 1) * plan.chunk * h.npoint : h.work;
+    const cplx *src = in + b0;
+    long sld = ild;
     for (int n = 1; n <= order; n++) {
-      const cplx *src = n == 1 ? in + b0 : buf[(n - 1) & 1] + b0;
-      const long sld = n == 1 ? ild : nst;
-      cplx *dst = buf[n & 1] + b0;
+      cplx *dst = (outs && outs[n - 1] ? outs[n - 1] : buf[n & 1]) + b0;
+      const bool keep = n < order || (outs && outs[n - 1]);
       if (proj)
         projector_project_p4_chunk(h, src, sld, nst, b0, nb, plan.gskip, plan.glog, s,
                                    two ? (c & 1) : 0);
@@ -607,12 +612,15 @@ static void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order
       acc.ld2 = ld2;
       acc.c2re = c2[n - 1].real();
       acc.c2im = c2[n - 1].imag();
+      acc.gdump = dumps ? dumps[n - 1] : nullptr;
       if (n == 1)
         local_passes(h, src, sld, dst, nst, nb, slot, b0, nst, &acc, 0, s, W, plan.gskip,
                      plan.glog);
       local_passes(h, src, sld, dst, nst, nb, slot, b0, nst, &acc, 1, s, W, plan.gskip, plan.glog);
       local_passes(h, src, sld, dst, nst, nb, slot, b0, nst, &acc, 2, s, W, plan.gskip, plan.glog,
-                   n < order, n == order);
+                   n < order, !keep);
+      src = dst;
+      sld = nst;
     }
   }
   if (two) {
@@ -626,7 +634,8 @@ static void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order
 void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
                                  int nst, int slot) {
   if (taylor_chain_ok(h)) {
-    taylor_chain(h, psi, ld, nst, order, slot, psi, ld, false, z, nullptr, 0, 0.0);
+    taylor_chain(h, psi, ld, nst, order, slot, psi, ld, false, z, nullptr, 0, 0.0, nullptr,
+                 nullptr);
     return;
   }
   cplx *hn = h.hn, *hn2 = h.hn2;
@@ -654,7 +663,8 @@ void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *ps
 void exponential_from(handle &h, std::complex<double> z, int order, const cplx *in, long ild,
                       cplx *out, long old, int nst, int slot) {
   if (taylor_chain_ok(h)) {
-    taylor_chain(h, in, ild, nst, order, slot, out, old, true, z, nullptr, 0, 0.0);
+    taylor_chain(h, in, ild, nst, order, slot, out, old, true, z, nullptr, 0, 0.0, nullptr,
+                 nullptr);
     return;
   }
   cplx *hn = h.hn, *hn2 = h.hn2;
@@ -683,7 +693,8 @@ void exponential_2_for_1(handle &h, std::complex<double> z1, std::complex<double
                                 int order, cplx *phi, cplx *expphi, long ld, long eld, int nst,
                                 int slot) {
   if (taylor_chain_ok(h)) {
-    taylor_chain(h, phi, ld, nst, order, slot, expphi, eld, true, z1, phi, ld, z2);
+    taylor_chain(h, phi, ld, nst, order, slot, expphi, eld, true, z1, phi, ld, z2, nullptr,
+                 nullptr);
     return;
   }
   cplx *hn = h.hn, *hn2 = h.hn2;

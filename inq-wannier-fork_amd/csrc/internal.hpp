@@ -180,6 +180,14 @@ void hamiltonian_apply(handle &h, const cplx *psi, long ld, cplx *hpsi, long hld
 void shift(handle &h, std::complex<double> c, const cplx *x, long xld, cplx *y, long yld,
            int nst);
 void copy(handle &h, const cplx *x, long xld, cplx *y, long yld, int nst);
+// Taylor chains on three-sweep grids (hamiltonian.hip): H^n in for n = 1..order
+// with acc1 += z1^n/n! H^n in (acc1 = in + z1 H in when init1) and acc2 +=
+// z2^n/n! H^n in, chunk by chunk, each order's last sweep transforming its
+// output forward for the next; outs/dumps optional per order (see there)
+bool taylor_chain_ok(const handle &h);
+void taylor_chain(handle &h, const cplx *in, long ild, int nst, int order, int slot, cplx *p1,
+                  long ld1, bool init1, std::complex<double> z1, cplx *p2, long ld2,
+                  std::complex<double> z2, cplx *const *outs, cplx *const *dumps);
 void exponential_in_place(handle &h, std::complex<double> z, int order, cplx *psi, long ld,
                           int nst, int slot);
 // gs.hip

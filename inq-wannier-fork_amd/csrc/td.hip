@@ -862,6 +862,18 @@ void replay_half(inqamd_td *td, cplx *cur, int nb, cplx *const *Y, cplx *next, i
     exponential_in_place(h, z, order, cur, nb, nb, slot);
     return;
   }
+  if (taylor_chain_ok(h)) {
+    // Y[it] = H^it phi for it < order (the last power only feeds `next`);
+    // fused: Y[order + it - 1] = the transform of Y[it - 1]
+    cplx *outs[TD_MAXM], *dumps[TD_MAXM];
+    for (int it = 1; it <= order; it++) {
+      outs[it - 1] = it < order ? Y[it] : nullptr;
+      dumps[it - 1] = td->fused ? Y[order + it - 1] : nullptr;
+    }
+    taylor_chain(h, Y[0], nb, nb, order, slot, next, nb, true, z, nullptr, 0, 0.0, outs,
+                 td->fused ? dumps : nullptr);
+    return;
+  }
   std::complex<double> coeff = 1.0;
   for (int it = 1; it <= order; it++) {
     coeff *= z / double(it);
@@ -888,19 +900,31 @@ void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double 
   // first H application writes lambda' + conj(z) H lambda')
   const cplx *A[TD_MAXM];
   A[0] = td->lam;
-  std::complex<double> ck = 1.0;
-  for (int k = 1; k <= M; k++) {
-    ck *= zc / double(k);
-    taylor_acc acc;
-    acc.p1 = td->lam_in;
-    acc.ld1 = nb;
-    acc.init1 = k == 1;
-    acc.c1re = ck.real();
-    acc.c1im = ck.imag();
-    acc.gdump = td->Ah[k - 1];
-    cplx *out = k < M ? td->Ak[k] : td->hk[0];
-    hamiltonian_apply(h, A[k - 1], nb, out, nb, nb, slot, &acc);
-    if (k < M) A[k] = out;
+  if (taylor_chain_ok(h)) {
+    // A_k = H^k lambda' (k < M) into Ak[k], their transforms into Ah[k - 1]
+    cplx *outs[TD_MAXM], *dumps[TD_MAXM];
+    for (int k = 1; k <= M; k++) {
+      outs[k - 1] = k < M ? td->Ak[k] : nullptr;
+      dumps[k - 1] = td->Ah[k - 1];
+      if (k < M) A[k] = td->Ak[k];
+    }
+    taylor_chain(h, td->lam, nb, nb, M, slot, td->lam_in, nb, true, zc, nullptr, 0, 0.0, outs,
+                 dumps);
+  } else {
+    std::complex<double> ck = 1.0;
+    for (int k = 1; k <= M; k++) {
+      ck *= zc / double(k);
+      taylor_acc acc;
+      acc.p1 = td->lam_in;
+      acc.ld1 = nb;
+      acc.init1 = k == 1;
+      acc.c1re = ck.real();
+      acc.c1im = ck.imag();
+      acc.gdump = td->Ah[k - 1];
+      cplx *out = k < M ? td->Ak[k] : td->hk[0];
+      hamiltonian_apply(h, A[k - 1], nb, out, nb, nb, slot, &acc);
+      if (k < M) A[k] = out;
+    }
   }
   coef_matrix E{};
   E.M = M;
