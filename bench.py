@@ -244,10 +244,14 @@ def _etrs_bytes(N, nst, sum_k):
     return nst * (8.0 * algorithmic_bytes_per_band(N, nst, sum_k) + 256.0 * N)
 
 
-def _oracle_etrs_rate(s, psi, budget_s, dt):
+def _oracle_etrs_rate(s, psi, budget_s, dt, threads=None):
+    """The oracle's frozen-potential ETRS step on a bounded sample of bands,
+    at `threads` OpenMP threads (None: all of them)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
+    default = O.get_threads()
+    O.set_threads(threads if threads else 0)
     osys = O.System(s.n, s.lattice, s.vloc, s.tables)
     one = np.ascontiguousarray(psi[..., :1])
     t0 = time.perf_counter()
@@ -258,7 +262,10 @@ def _oracle_etrs_rate(s, psi, budget_s, dt):
     t0 = time.perf_counter()
     osys.etrs_step(sample, dt, (0, 0, 0), (0, 0, 0))
     el = time.perf_counter() - t0
-    return {"value": nb / el, "unit": "band-ETRS-steps/s", "cores": O.get_threads(),
+    cores = O.get_threads()
+    O.set_threads(default)
+    log(f"cpu oracle: {nb} bands, one ETRS step in {el:.2f} s with {cores} threads")
+    return {"value": nb / el, "unit": "band-ETRS-steps/s", "cores": cores,
             "kind": "port", "sample": f"{nb} of {psi.shape[-1]} bands, one ETRS step "
                                       f"(two Taylor-4 half steps), C++ oracle OpenMP"}
 
@@ -383,11 +390,15 @@ def run_etrs_96(args):
     log(f"inputs built in {time.perf_counter() - t0:.1f} s: grid {s.n}, {s.natoms} atoms")
     dev = torch.device("cuda:0")
     d_psi = torch.from_numpy(psi).to(dev)
+    # host bands kept for the parity check (first and last) and the CPU sample
+    pbands = np.ascontiguousarray(psi[..., [0, nst - 1]])
+    sample = np.ascontiguousarray(psi[..., :16])
     del psi
     ham = inq_amd.ks_hamiltonian(s, max_bands=nst)
     coef = inq_amd.systems.rng(0, 0, 4).uniform(-0.05, 0.05, size=32)
     pert = real_time.fourier_series_vector_potential(coef, 0.1)
     dt = 0.03
+    d0 = d_psi.clone()
     g = real_time.etrs_graph(ham, d_psi, dt, 4, pert)
     ms = timed(g.launch, args.steps, args.warmup)
     g.close()
@@ -396,7 +407,27 @@ def run_etrs_96(args):
     step_bytes = _etrs_bytes(N, nst, sum_k)
     achieved = step_bytes / (ms / 1e3) / 1e9
     log(f"ETRS step {ms:.2f} ms ({achieved:.0f} GB/s algorithmic)")
+    # one eager step under A(0), A(dt) from the same start, two bands against the oracle
+    del d_psi
+    real_time.etrs(ham, d0, 0.0, dt, pert)
+    got = d0[..., [0, nst - 1]].cpu().numpy()
+    del d0
     ham.close()
+    parity = None
+    cpu = cpu1 = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        lat = s.lattice
+        t0 = time.perf_counter()
+        ref = O.System(s.n, s.lattice, s.vloc, s.tables).etrs_step(
+            pbands, dt, lat @ pert.uniform_vector_potential(0.0),
+            lat @ pert.uniform_vector_potential(dt))
+        parity = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        log(f"step vs oracle on bands 0 and {nst - 1}: rel err {parity:.2e} "
+            f"({time.perf_counter() - t0:.1f} s)")
+        cpu = _oracle_etrs_rate(s, sample, args.cpu_seconds, dt)
+        cpu1 = _oracle_etrs_rate(s, sample, args.cpu_seconds / 2, dt, threads=1)
     cfg = {"workload": "etrs_96: 96^3 grid, 512 bands complex128, 3x3x3 diamond supercell "
                        "a=30.78 bohr, 216 atoms x 4 projectors, graph-captured ETRS step "
                        "(two Taylor-4 half steps) under a 32-term A(t), dt=0.03",
@@ -406,7 +437,9 @@ def run_etrs_96(args):
                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                        "kernel": "graph-captured ETRS step (all launches)",
                        "algorithmic_bytes_per_step": step_bytes}
-    out["cpu_baseline"] = None
+    out["cpu_baseline"] = cpu
+    out["cpu_baseline_1thread"] = cpu1
+    out["max_rel_err_vs_oracle"] = parity
     return out
 
 
