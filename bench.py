@@ -22,6 +22,8 @@ gs_64: one frozen-potential ground-state iteration (subspace diagonalisation,
   preconditioned steepest descent, orthogonalize) at 64^3 x 256 bands.
 tddft_64: one interacting ETRS step (Hartree + PBE, self-consistent second
   half) at 64^3 x 256 bands.
+autotune: the per-grid tiling table of the three-sweep Hpsi (inq_amd.tuning):
+  every knob combination timed per grid length (--grids) and band class.
 stream: the library's 16-byte streaming probe (read / write / copy) over
   1 GiB and over a band chunk of 64^3 x 256 rows: the achievable ceiling.
 """
@@ -652,11 +654,31 @@ def run_stream(args):
     return out
 
 
+def run_autotune(args):
+    """The per-grid tiling table (inq_amd.tuning): every knob combination of
+    the three-sweep Hpsi timed for each grid length of the workloads and each
+    band class; prints the table (the "hpsi" section of tuning.json)."""
+    from inq_amd import tuning
+    lengths = [int(x) for x in args.grids.split(",")]
+    t0 = time.perf_counter()
+    table = tuning.grid_table(lengths, reps=args.steps, log=log)
+    log(f"autotune of {len(lengths)} grid lengths in {time.perf_counter() - t0:.1f} s")
+    cfg = {"workload": "autotune: three-sweep Hpsi tiling per grid length and band class",
+           "grids": lengths, "parallelism": "single GPU"}
+    out = common_fields(args, len(lengths), "grid lengths tuned", None, True, cfg)
+    out["table"] = table
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="hpsi_64",
                     choices=["hpsi_64", "uneven_batch", "adjoint_48", "etrs_96", "adjoint_96",
-                             "gs_64", "tddft_64", "stream"])
+                             "gs_64", "tddft_64", "stream", "autotune"])
+    ap.add_argument("--grids", default="24,30,32,36,40,48,60,64,72,96",
+                    help="autotune: the grid lengths to tune")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -682,6 +704,9 @@ def main():
         return
     if args.workload == "stream":
         print(json.dumps(run_stream(args)), flush=True)
+        return
+    if args.workload == "autotune":
+        print(json.dumps(run_autotune(args)), flush=True)
         return
     if args.workload == "adjoint_48":
         out = run_adjoint(args, 1, 48, 64, 400, 20, 0.015,

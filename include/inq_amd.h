@@ -100,6 +100,23 @@ int inqamd_stream_probe(int kind, const void *src, void *dst, long rows, int ld,
                         int grouped, int nchunks, int blocks_per_cu, void *stream, float *ms);
 
 /* orbital_set::kpoint() (orbital_set.hpp) -- covariant k. */
+/* Per-handle tiling of the three-sweep Hpsi: the autotuner's knobs, no
+ * reference counterpart.  0 keeps the library's choice for that knob.
+ *   z_tile_y         z tiles of 8 y x 8 bands or 16 y x 4 bands
+ *   xy_threads       x-y plane workgroups of 1 or 2 x plane_threads(L)
+ *   band_chunk       bands per chunk of the sweeps (multiple of 8, <= max_bands)
+ *   projector_bands  bands per workgroup of the projector coefficients (16/32/64)
+ * z_tile_y, band_chunk and projector_bands change no result bit; xy_threads
+ * selects another instantiation of the plane kernel, whose floating-point
+ * contraction may differ in the last bits. */
+typedef struct {
+  int z_tile_y;
+  int xy_threads;
+  int band_chunk;
+  int projector_bands;
+} inqamd_tuning;
+int inqamd_set_tuning(inqamd_hamiltonian_t h, const inqamd_tuning *t);
+int inqamd_get_tuning(inqamd_hamiltonian_t h, inqamd_tuning *t);
 int inqamd_set_kpoint(inqamd_hamiltonian_t h, const double k_cov[3]);
 /* ks_hamiltonian::uniform_vector_potential() = A (ks_hamiltonian.hpp:216-221),
  * as self_consistency::update_hamiltonian sets it (self_consistency.hpp:152-156). */

@@ -131,6 +131,9 @@ struct fft3_args {
   // hpsi not written at all (the last term of a Taylor chain: only the
   // accumulators need it)
   int skip_store;
+  // launch shapes (0: the process default): z tiles of z_ty y values, x-y
+  // planes with plane_threads(L) (xy_wide 1) or twice as many (2)
+  int z_ty, xy_wide;
   // G-space Hpsi sweeps: G-space orbitals whose 1/2 |G+k+A|^2 multiple the
   // last sweep adds (fft3_fourier_hamiltonian)
   const cplx *kin_in;

@@ -145,8 +145,9 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep, bool fwd
 template <int R0, int R1, int R2>
 static void launch3(const fft3_args &a, hipStream_t stream, int sweep, bool fwd_next) {
   constexpr int L = R0 * R1 * R2;
-  const bool wide = xy_wide_choice();
-  if (z_ty_choice() == 16) {
+  const bool wide = a.xy_wide ? a.xy_wide == 2 : xy_wide_choice();
+  const int ty = a.z_ty ? a.z_ty : z_ty_choice();
+  if (ty == 16) {
     if (wide)
       launch3v<R0, R1, R2, 16, wide_threads(L)>(a, stream, sweep, fwd_next);
     else

@@ -323,6 +323,8 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
     }();
     a.nt_store = nt_store;
     a.skip_store = skip_store ? 1 : 0;
+    a.z_ty = h.tune_z_ty;
+    a.xy_wide = h.tune_xy_wide;
     fft3_local_hamiltonian(a, h.n[0], stream ? stream : h.stream, sweep, fwd_next);
     return;
   }
@@ -348,6 +350,7 @@ static int band_chunk(const handle &h, int nst) {
     const char *e = getenv("INQAMD_BAND_CHUNK");
     return e ? atoi(e) : -1;
   }();
+  if (h.tune_chunk > 0) return h.tune_chunk < nst ? h.tune_chunk : nst;
   if (forced >= 0) return forced == 0 ? nst : forced;
   // at least 192 MiB of intermediate and 24 bands per chunk: measured
   // fastest at 64^3 x 256 (48 bands; 32 and 64 were 6 % slower) and at
@@ -1006,6 +1009,36 @@ void inqamd_hamiltonian_destroy(inqamd_hamiltonian_t h) {
 int inqamd_set_stream(inqamd_hamiltonian_t h, void *stream) {
   if (!h) return INQAMD_ERR_ARGUMENT;
   h->stream = (hipStream_t)stream;
+  return INQAMD_OK;
+}
+
+int inqamd_set_tuning(inqamd_hamiltonian_t h, const inqamd_tuning *t) {
+  if (!h || !t) return INQAMD_ERR_ARGUMENT;
+  return guarded(h, [&] {
+    INQAMD_REQUIRE(t->z_tile_y == 0 || t->z_tile_y == 8 || t->z_tile_y == 16,
+                   "z_tile_y must be 0, 8 or 16");
+    INQAMD_REQUIRE(t->xy_threads >= 0 && t->xy_threads <= 2, "xy_threads must be 0, 1 or 2");
+    INQAMD_REQUIRE(t->band_chunk >= 0 && t->band_chunk % 8 == 0,
+                   "band_chunk must be 0 or a positive multiple of 8");
+    INQAMD_REQUIRE(t->projector_bands == 0 || t->projector_bands == 16 ||
+                       t->projector_bands == 32 || t->projector_bands == 64,
+                   "projector_bands must be 0, 16, 32 or 64");
+    // two band chunks of the pipeline share the handle's max_bands-wide
+    // intermediate, so a chunk may hold at most max_bands bands
+    INQAMD_REQUIRE(t->band_chunk <= h->max_bands, "band_chunk exceeds max_bands");
+    h->tune_z_ty = t->z_tile_y;
+    h->tune_xy_wide = t->xy_threads;
+    h->tune_chunk = t->band_chunk;
+    h->tune_p4 = t->projector_bands;
+  });
+}
+
+int inqamd_get_tuning(inqamd_hamiltonian_t h, inqamd_tuning *t) {
+  if (!h || !t) return INQAMD_ERR_ARGUMENT;
+  t->z_tile_y = h->tune_z_ty;
+  t->xy_threads = h->tune_xy_wide;
+  t->band_chunk = h->tune_chunk;
+  t->projector_bands = h->tune_p4;
   return INQAMD_OK;
 }
 

@@ -125,6 +125,12 @@ struct inqamd_hamiltonian {
   long mom_partial_elems = 0;
 
   inqamd::projector_tables pt;
+  // per-handle tiling of the three-sweep Hpsi (inqamd_set_tuning; 0: the
+  // process default)
+  int tune_z_ty = 0;            // z tiles of 8 or 16 y
+  int tune_xy_wide = 0;         // x-y planes: 1 = plane_threads(L), 2 = twice that
+  int tune_chunk = 0;           // bands per chunk of the sweeps
+  int tune_p4 = 0;              // bands per projector coefficient workgroup (16/32/64)
   int commutator = 0;           // INQAMD_COMMUTATOR_* of inqamd_current
   long generation = 0;          // bumped whenever V or k changes (stale-state checks)
 

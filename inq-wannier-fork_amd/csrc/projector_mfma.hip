@@ -180,11 +180,12 @@ static void project_p4_launch(handle &h, const project_expand_args &a, int nst, 
   const projector_tables &t = h.pt;
   // bands per workgroup: 32 (INQAMD_P4_BANDS = 16, 32 or 64; at 64^3 x 256
   // the projector took 62, 61 and 60 us) where that leaves 256 workgroups
-  static const int nb16 = [] {
+  static const int nb16_env = [] {
     const char *e = getenv("INQAMD_P4_BANDS");
     const int b = e ? atoi(e) : 32;
     return b >= 64 ? 4 : (b >= 32 ? 2 : 1);
   }();
+  const int nb16 = h.tune_p4 ? h.tune_p4 / 16 : nb16_env;
   const int nbt = (nst + P4_BANDS - 1) / P4_BANDS;
   const long tiles = (long)((nst_all + P4_BANDS - 1) / P4_BANDS) * t.nproj;
   int nc = 1;

@@ -66,6 +66,17 @@ class VectorPotentialSeries(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """inqamd_tuning."""
+
+    _fields_ = [
+        ("z_tile_y", ctypes.c_int),
+        ("xy_threads", ctypes.c_int),
+        ("band_chunk", ctypes.c_int),
+        ("projector_bands", ctypes.c_int),
+    ]
+
+
 class TDDesc(ctypes.Structure):
     """inqamd_td_desc."""
 
@@ -90,6 +101,8 @@ SIGNATURES = [
     ("inqamd_hamiltonian_create", _i, [ctypes.POINTER(SystemDesc), _vp, ctypes.POINTER(_vp)]),
     ("inqamd_hamiltonian_destroy", None, [_vp]),
     ("inqamd_set_stream", _i, [_vp, _vp]),
+    ("inqamd_set_tuning", _i, [_vp, ctypes.POINTER(Tuning)]),
+    ("inqamd_get_tuning", _i, [_vp, ctypes.POINTER(Tuning)]),
     ("inqamd_last_error", ctypes.c_char_p, [_vp]),
     ("inqamd_version", ctypes.c_char_p, []),
     ("inqamd_selftest_mfma", _i, [_vp, _vp]),

@@ -116,7 +116,10 @@ class projector_all:
 class ks_hamiltonian:
     """Frozen-potential Kohn-Sham Hamiltonian on one GPU."""
 
-    def __init__(self, system, max_bands, kpoint=(0.0, 0.0, 0.0)):
+    def __init__(self, system, max_bands, kpoint=(0.0, 0.0, 0.0), tuning="table"):
+        """tuning: "table" takes the committed per-grid tiling
+        (inq_amd.tuning) for this grid length and max_bands, None keeps the
+        library's defaults, a dict sets the knobs of inqamd_tuning."""
         self.system = system
         self.tables = system.tables
         self.n = tuple(system.n)
@@ -125,6 +128,27 @@ class ks_hamiltonian:
         self._k = np.zeros(3)
         self.kpoint = kpoint
         self._commutator = "inq"
+        if tuning == "table":
+            from . import tuning as _tuning
+            tuning = _tuning.lookup(self.n, max_bands)
+        if tuning:
+            self.tuning = tuning
+
+    @property
+    def tuning(self):
+        """The per-handle tiling knobs (inqamd_tuning; 0 = library default)."""
+        t = _lib.Tuning()
+        _lib.check(self._h._lib.inqamd_get_tuning(self._h.h, ctypes.byref(t)), self._h.h)
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
+    @tuning.setter
+    def tuning(self, knobs):
+        t = _lib.Tuning()
+        for f, v in dict(knobs).items():
+            if f not in dict(_lib.Tuning._fields_):
+                raise ValueError(f"unknown tuning knob {f!r}")
+            setattr(t, f, int(v))
+        _lib.check(self._h._lib.inqamd_set_tuning(self._h.h, ctypes.byref(t)), self._h.h)
 
     @property
     def handle(self):

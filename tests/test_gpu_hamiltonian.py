@@ -584,3 +584,43 @@ def test_band_groups_96_match_padded_rows(device):
     say("96^3 Hpsi vs oracle on 2 bands %.2e" % err)
     assert err < TOL
     ham.close()
+
+
+def test_tuning_knobs_do_not_change_results(device):
+    """The z-tile shape, band chunk and projector bands of inqamd_tuning leave
+    Hpsi and an ETRS step the same bit for bit; the x-y thread count is
+    another instantiation of the plane kernel, whose floating-point
+    contraction may differ (1e-14 relative); invalid values are refused."""
+    s = systems.diamond_system(1, grid=48)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=64, tuning=None)
+    ham.uniform_vector_potential = (0.02, -0.01, 0.03)
+    g = torch.Generator(device=device).manual_seed(21)
+    psi = torch.randn(tuple(s.n) + (64,), dtype=torch.complex128, device=device, generator=g)
+    ref = ham(psi)
+    step = psi.clone()
+    ham.handle.call("inqamd_etrs_step", ctypes.c_double(0.01), inq_amd._lib.dvec([0.01, 0, 0]),
+                    inq_amd._lib.dvec([0.012, 0, 0]), 4, ctypes.c_void_p(step.data_ptr()), 64, 64)
+    torch.cuda.synchronize()
+    for knobs in ({"z_tile_y": 16}, {"band_chunk": 16}, {"projector_bands": 64},
+                  {"projector_bands": 16, "band_chunk": 24, "z_tile_y": 16}, {"xy_threads": 1}):
+        say("tuning", knobs)
+        ham.tuning = dict({"z_tile_y": 0, "xy_threads": 0, "band_chunk": 0, "projector_bands": 0},
+                          **knobs)
+        ham.uniform_vector_potential = (0.02, -0.01, 0.03)  # the ETRS step leaves A(t + dt)
+        got = ham(psi)
+        other = psi.clone()
+        ham.handle.call("inqamd_etrs_step", ctypes.c_double(0.01), inq_amd._lib.dvec([0.01, 0, 0]),
+                        inq_amd._lib.dvec([0.012, 0, 0]), 4, ctypes.c_void_p(other.data_ptr()),
+                        64, 64)
+        torch.cuda.synchronize()
+        e1 = float((torch.linalg.vector_norm(got - ref) / torch.linalg.vector_norm(ref)).item())
+        e2 = float((torch.linalg.vector_norm(other - step) / torch.linalg.vector_norm(step)).item())
+        say("  Hpsi rel diff %.2e, ETRS rel diff %.2e" % (e1, e2))
+        if "xy_threads" in knobs:
+            assert e1 < 1e-14 and e2 < 1e-14, knobs
+        else:
+            assert torch.equal(got, ref) and torch.equal(other, step), knobs
+    for bad in ({"z_tile_y": 12}, {"band_chunk": 20}, {"band_chunk": 72}, {"projector_bands": 8}):
+        with pytest.raises(inq_amd._lib.InqAmdError):
+            ham.tuning = bad
+    ham.close()
