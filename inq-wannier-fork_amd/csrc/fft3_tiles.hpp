@@ -406,35 +406,73 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   if constexpr (!GS) {
     if (a.Pc) {
       // projector_all::apply (projector_all.hpp:172-212, 225-231) expanded
-      // from the coefficients P[p][lm][b] at this thread's owned outputs,
-      // their ownership entries in fixed (sphere, point) order, before the
-      // FFT so that none of it stays live through the transform.  The first
-      // entry's sphere comes with the ownership word, so its coefficient
-      // loads need not wait for the entry tables.
+      // from the coefficients P[p][lm][b] at the tile's owned outputs, each
+      // output's ownership entries in fixed (sphere, point) order, before
+      // the FFT (whose LDS tile is free until then).  About one output in
+      // five is owned, so each wave first queues its owned outputs and then
+      // expands them 64 at a time: the entry-table and coefficient loads of
+      // a round are all in flight together instead of one output position
+      // after another.  LDS slot e (16 B) of output e = thread + k NT holds
+      // its ownership word and first sphere, a queue entry in its third
+      // word, and finally the expanded value.
+      int *sl = reinterpret_cast<int *>(lds);
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      auto qslot = [&](int pos) { return (pos & 63) + 64 * wave + (pos >> 6) * NT; };
+      int T = 0;  // owned outputs of this wave (uniform)
 #pragma unroll
       for (int k = 0; k < NE; k++) {
-        if (own[k] < 0) continue;
-        const int cb = gb0 + (int)(threadIdx.x + k * NT) % Z_TB;
-        const int e0 = own[k] >> OWN_COUNT_BITS, cnt = own[k] & OWN_MAX_COUNT;
+        const int e = threadIdx.x + k * NT;
+        const bool ow = own[k] >= 0;
+        const unsigned long long m = __ballot(ow);
+        if (ow) {
+          sl[4 * e] = own[k];
+          sl[4 * e + 1] = osp[k];
+          sl[4 * qslot(T + __popcll(m & __lanemask_lt())) + 2] = e;
+        }
+        T += __popcll(m);
+      }
+      __syncthreads();
+      int item[NE];
+#pragma unroll
+      for (int r = 0; r < NE; r++) {
+        const int i = lane + 64 * r;
+        item[r] = i < T ? sl[4 * qslot(i) + 2] : -1;
+      }
+      __syncthreads();  // every queue entry is read before results overwrite slots
+#pragma unroll
+      for (int r = 0; r < NE; r++) {
+        if (64 * r >= T) break;  // uniform over the wave
+        const int e = item[r];
+        if (e < 0) continue;
+        const int w = sl[4 * e], sp0 = sl[4 * e + 1];
+        const int cb = gb0 + e % Z_TB;
+        const int e0 = w >> OWN_COUNT_BITS, cnt = w & OWN_MAX_COUNT;
+        cplx v = mk(0.0, 0.0);
         for (int j = 0; j < cnt; j++) {
-          const int e = e0 + j;
-          const int sp = j == 0 ? osp[k] : a.ent_sph[e];
-          const double4 mm = reinterpret_cast<const double4 *>(a.ent_mat)[e];
-          const double m[4] = {mm.x, mm.y, mm.z, mm.w};
-          const cplx f = a.phase_ent[e];
+          const int en = e0 + j;
+          const int sp = j == 0 ? sp0 : a.ent_sph[en];
+          const double4 mm = reinterpret_cast<const double4 *>(a.ent_mat)[en];
+          const double mv[4] = {mm.x, mm.y, mm.z, mm.w};
+          const cplx f = a.phase_ent[en];
           const cplx *P = a.Pc + (long)sp * a.nlmP * a.ldPc + cb;
           cplx u = mk(0.0, 0.0);
 #pragma unroll
           for (int lm = 0; lm < 4; lm++) {
             if (lm < a.nlmP) {
               const cplx c = P[(long)lm * a.ldPc];
-              u.x += m[lm] * c.x;
-              u.y += m[lm] * c.y;
+              u.x += mv[lm] * c.x;
+              u.y += mv[lm] * c.y;
             }
           }
-          pre[k] = cadd(pre[k], cmul(f, u));
+          v = cadd(v, cmul(f, u));
         }
+        lds[e] = v;
       }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < NE; k++)
+        if (own[k] >= 0) pre[k] = cadd(pre[k], lds[threadIdx.x + k * NT]);
+      __syncthreads();  // the FFT's first stage writes the tile
     }
   }
 
