@@ -348,6 +348,114 @@ __device__ __forceinline__ void xy_fwd_plane(const fft3_args &a, long plane, cpl
 // FWD (Taylor chains): the tile then transforms its own hpsi forward in z
 // into W, in place of the next application's z_fwd sweep over the same tile
 // (the chunk's W region the tile read is the region it writes).
+// s_waitcnt immediate that waits for every vector memory operation of the
+// wave (vmcnt 0) and leaves the export and LDS/scalar counters alone
+constexpr unsigned WAITCNT_VM0 = 0x0F70;
+
+// Position of the n-th (from 0) set bit of m, which has more than n set bits
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
+  int pos = 0;
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) {
+    const int c = __popcll(m & ((1ull << s) - 1));
+    if (n >= c) {
+      n -= c;
+      m >>= s;
+      pos += s;
+    }
+  }
+  return pos;
+}
+
+// projector_all::apply (projector_all.hpp:172-212, 225-231) expanded from
+// the coefficients P[p][lm][b] at a z tile's owned outputs (output k of this
+// thread: own[k] >= 0, its first entry's sphere in osp[k]), each output's
+// ownership entries in fixed (sphere, point) order, added to pre[k].  About
+// one output in five is owned, so each wave compacts its owned outputs with
+// ballots (queue position: outputs of slice k = 0, 1, ... in lane order) and
+// expands them 64 at a time, the entry-table and coefficient loads of a round
+// all in flight together.  Outputs and results move between lanes by
+// cross-lane permutes, which leave the LDS tile to the W rows loading into it.
+template <int NE, int Z_TB>
+__device__ __forceinline__ void vnl_expand_wave(const fft3_args &a, int gb0, const int (&own)[NE],
+                                                const int (&osp)[NE], cplx (&pre)[NE]) {
+  const int lane = threadIdx.x & 63;
+  const int wave0 = threadIdx.x & ~63;  // first thread of the wave
+  const unsigned long long lt = __lanemask_lt();
+  unsigned long long m[NE];
+  int P[NE + 1];  // queue position of slice k's first owned output (uniform)
+  P[0] = 0;
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    m[k] = __ballot(own[k] >= 0);
+    P[k + 1] = P[k] + __popcll(m[k]);
+  }
+  const int T = P[NE];
+  for (int r0 = 0; r0 < T; r0 += 64) {
+    // this lane expands queue position r0 + lane: owned output (src, ks)
+    const int pos = r0 + lane;
+    int ks = 0;
+#pragma unroll
+    for (int k = 1; k < NE; k++)
+      if (P[k] <= pos) ks = k;
+    unsigned long long mks = m[0];
+    int pks = 0;
+#pragma unroll
+    for (int k = 1; k < NE; k++)
+      if (ks == k) {
+        mks = m[k];
+        pks = P[k];
+      }
+    const int src = pos < T ? nth_set_bit(mks, pos - pks) : lane;
+    int w = 0, sp0 = 0;
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+      if (P[k] < r0 + 64 && P[k + 1] > r0) {  // slice k has positions in this round
+        const int ow = __shfl(own[k], src);
+        const int os = __shfl(osp[k], src);
+        if (ks == k) {
+          w = ow;
+          sp0 = os;
+        }
+      }
+    }
+    cplx v = mk(0.0, 0.0);
+    if (pos < T) {
+      const int cb = gb0 + (wave0 + src) % Z_TB;
+      const int e0 = w >> OWN_COUNT_BITS, cnt = w & OWN_MAX_COUNT;
+      for (int j = 0; j < cnt; j++) {
+        const int en = e0 + j;
+        const int sp = j == 0 ? sp0 : a.ent_sph[en];
+        const double4 mm = reinterpret_cast<const double4 *>(a.ent_mat)[en];
+        const double mv[4] = {mm.x, mm.y, mm.z, mm.w};
+        const cplx f = a.phase_ent[en];
+        const cplx *Pp = a.Pc + (long)sp * a.nlmP * a.ldPc + cb;
+        cplx u = mk(0.0, 0.0);
+#pragma unroll
+        for (int lm = 0; lm < 4; lm++) {
+          if (lm < a.nlmP) {
+            const cplx c = Pp[(long)lm * a.ldPc];
+            u.x += mv[lm] * c.x;
+            u.y += mv[lm] * c.y;
+          }
+        }
+        v = cadd(v, cmul(f, u));
+      }
+    }
+    // each owned output of the round takes its value from the lane that expanded it
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+      if (P[k] < r0 + 64 && P[k + 1] > r0) {
+        const int j = P[k] + __popcll(m[k] & lt) - r0;
+        const bool here = own[k] >= 0 && j >= 0 && j < 64;
+        const int from = here ? j : lane;
+        const double vx = __shfl(v.x, from), vy = __shfl(v.y, from);
+        if (here) pre[k] = cadd(pre[k], mk(vx, vy));
+      }
+    }
+  }
+}
+
 template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool ACC = false, bool GS = false,
           bool FWD = false>
 __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx *lds) {
@@ -366,114 +474,90 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   const int y0 = yt * Z_TY, cb0 = bt * Z_TB;
   const int gb0 = cb0 + (cb0 >> (3 + a.glog)) * a.gskip;  // the tile's first band in psi, hpsi
 
+  // The tile's W rows go straight from HBM into the LDS tile (16 B per lane:
+  // lane l of a wave's load fills slot l of one 64-column row, so it loads
+  // column zswz(l), zswz being its own inverse), issued first so that they
+  // are in flight with the prefetch below and the V_nl expansion, which use
+  // no LDS.  Columns outside the grid or the chunk load W[0] instead; their
+  // transforms are never stored.
+  if constexpr (!GS) {
+    const int c = zswz(threadIdx.x & 63);
+    const int b = c / Z_TY, y = c % Z_TY;
+    const bool ok = y0 + y < L && cb0 + b < a.nb;
+    const long col = ok ? (((long)(cb0 + b) * L) * L + x) * L + y0 + y : 0;
+    const long row = ok ? (long)L * L : 0;
+    for (int i = threadIdx.x >> 6; i < L; i += NT / 64)
+      __builtin_amdgcn_global_load_lds(a.W + col + i * row, lds + i * Z_COLS, 16, 0, 0);
+  }
+
   // prefetch V psi (and the ownership entries) of this thread's outputs,
-  // or 1/2 |G+k+A|^2 kin_in
+  // or 1/2 |G+k+A|^2 kin_in.  Every load of the prefetch is issued before
+  // any product waits for one: an output outside the grid or the chunk loads
+  // from point 0, band gb0 (in bounds) and is zeroed afterwards, so the loads
+  // need no branches, each of which would wait for its own loads.
   cplx pre[NE];
   int own[NE];
   int osp[NE];  // sphere of the first ownership entry (coefficient expansion)
-  [[maybe_unused]] double gx[3];
-  if constexpr (GS) {
-    double kc[3];
-    xy_kc(a, kc);
-    const int mx = symmetric_index(x, L);
-#pragma unroll
-    for (int d = 0; d < 3; d++) gx[d] = mx * a.rlat[0 * 3 + d] + kc[d];
-  }
+  int pk[NE];   // point of output k (0 outside the tile's valid outputs)
+  int bk[NE];   // band offset of output k in psi (gb0 + b, or gb0)
+  bool vk[NE];
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     const int e = threadIdx.x + k * NT;
     const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-    pre[k] = mk(0.0, 0.0);
-    own[k] = -1;
-    osp[k] = 0;
-    if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
-      const long p = ((long)x * L + y0 + y) * L + z;
-      if constexpr (GS) {
-        const int my = symmetric_index(y0 + y, L), mz = symmetric_index(z, L);
-        const double q0 = gx[0] + my * a.rlat[3] + mz * a.rlat[6];
-        const double q1 = gx[1] + my * a.rlat[4] + mz * a.rlat[7];
-        const double q2 = gx[2] + my * a.rlat[5] + mz * a.rlat[8];
-        pre[k] = cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), a.kin_in[p * a.kin_ld + gb0 + b]);
-      } else {
-        const cplx ps = a.psi[p * a.ld + gb0 + b];
-        const double vp = a.V[p];
-        pre[k] = mk(vp * ps.x, vp * ps.y);
-        if (a.own) own[k] = a.own[p];
-        if (a.Pc) osp[k] = a.own_sph[p];
-      }
+    vk[k] = e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb;
+    pk[k] = vk[k] ? (x * L + y0 + y) * L + z : 0;
+    bk[k] = gb0 + (vk[k] ? b : 0);
+  }
+  if constexpr (GS) {
+    double kc[3], gx[3];
+    xy_kc(a, kc);
+    const int mx = symmetric_index(x, L);
+#pragma unroll
+    for (int d = 0; d < 3; d++) gx[d] = mx * a.rlat[0 * 3 + d] + kc[d];
+    cplx kin[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) kin[k] = a.kin_in[(long)pk[k] * a.kin_ld + bk[k]];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+      const int e = threadIdx.x + k * NT;
+      const int y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+      const int my = symmetric_index(y0 + y, L), mz = symmetric_index(z, L);
+      const double q0 = gx[0] + my * a.rlat[3] + mz * a.rlat[6];
+      const double q1 = gx[1] + my * a.rlat[4] + mz * a.rlat[7];
+      const double q2 = gx[2] + my * a.rlat[5] + mz * a.rlat[8];
+      pre[k] = vk[k] ? cscale(0.5 * (q0 * q0 + q1 * q1 + q2 * q2), kin[k]) : mk(0.0, 0.0);
+      own[k] = -1;
+      osp[k] = 0;
+    }
+  } else {
+    cplx ps[NE];
+    double vp[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+      ps[k] = a.psi[(long)pk[k] * a.ld + bk[k]];
+      vp[k] = a.V[pk[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < NE; k++) own[k] = -1;
+    if (a.own && a.Pc) {
+#pragma unroll
+      for (int k = 0; k < NE; k++) own[k] = a.own[pk[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < NE; k++) osp[k] = 0;
+    if (a.Pc) {
+#pragma unroll
+      for (int k = 0; k < NE; k++) osp[k] = a.own_sph[pk[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+      pre[k] = vk[k] ? mk(vp[k] * ps[k].x, vp[k] * ps[k].y) : mk(0.0, 0.0);
+      if (!vk[k]) own[k] = -1;
     }
   }
   if constexpr (!GS) {
-    if (a.Pc) {
-      // projector_all::apply (projector_all.hpp:172-212, 225-231) expanded
-      // from the coefficients P[p][lm][b] at the tile's owned outputs, each
-      // output's ownership entries in fixed (sphere, point) order, before
-      // the FFT (whose LDS tile is free until then).  About one output in
-      // five is owned, so each wave first queues its owned outputs and then
-      // expands them 64 at a time: the entry-table and coefficient loads of
-      // a round are all in flight together instead of one output position
-      // after another.  LDS slot e (16 B) of output e = thread + k NT holds
-      // its ownership word and first sphere, a queue entry in its third
-      // word, and finally the expanded value.
-      int *sl = reinterpret_cast<int *>(lds);
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      auto qslot = [&](int pos) { return (pos & 63) + 64 * wave + (pos >> 6) * NT; };
-      int T = 0;  // owned outputs of this wave (uniform)
-#pragma unroll
-      for (int k = 0; k < NE; k++) {
-        const int e = threadIdx.x + k * NT;
-        const bool ow = own[k] >= 0;
-        const unsigned long long m = __ballot(ow);
-        if (ow) {
-          sl[4 * e] = own[k];
-          sl[4 * e + 1] = osp[k];
-          sl[4 * qslot(T + __popcll(m & __lanemask_lt())) + 2] = e;
-        }
-        T += __popcll(m);
-      }
-      __syncthreads();
-      int item[NE];
-#pragma unroll
-      for (int r = 0; r < NE; r++) {
-        const int i = lane + 64 * r;
-        item[r] = i < T ? sl[4 * qslot(i) + 2] : -1;
-      }
-      __syncthreads();  // every queue entry is read before results overwrite slots
-#pragma unroll
-      for (int r = 0; r < NE; r++) {
-        if (64 * r >= T) break;  // uniform over the wave
-        const int e = item[r];
-        if (e < 0) continue;
-        const int w = sl[4 * e], sp0 = sl[4 * e + 1];
-        const int cb = gb0 + e % Z_TB;
-        const int e0 = w >> OWN_COUNT_BITS, cnt = w & OWN_MAX_COUNT;
-        cplx v = mk(0.0, 0.0);
-        for (int j = 0; j < cnt; j++) {
-          const int en = e0 + j;
-          const int sp = j == 0 ? sp0 : a.ent_sph[en];
-          const double4 mm = reinterpret_cast<const double4 *>(a.ent_mat)[en];
-          const double mv[4] = {mm.x, mm.y, mm.z, mm.w};
-          const cplx f = a.phase_ent[en];
-          const cplx *P = a.Pc + (long)sp * a.nlmP * a.ldPc + cb;
-          cplx u = mk(0.0, 0.0);
-#pragma unroll
-          for (int lm = 0; lm < 4; lm++) {
-            if (lm < a.nlmP) {
-              const cplx c = P[(long)lm * a.ldPc];
-              u.x += mv[lm] * c.x;
-              u.y += mv[lm] * c.y;
-            }
-          }
-          v = cadd(v, cmul(f, u));
-        }
-        lds[e] = v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < NE; k++)
-        if (own[k] >= 0) pre[k] = cadd(pre[k], lds[threadIdx.x + k * NT]);
-      __syncthreads();  // the FFT's first stage writes the tile
-    }
+    if (a.Pc) vnl_expand_wave<NE, Z_TB>(a, gb0, own, osp, pre);
   }
 
   auto gl = [&](int i, int c) -> cplx {
@@ -491,7 +575,12 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     }
     __syncthreads();
   } else {
-    run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, +1, false, gl, ll, ls, ls, true);
+    // the W rows loaded into the tile at the start: this wave's loads are
+    // complete (vmcnt 0), the barrier makes every wave's rows visible
+    __builtin_amdgcn_s_waitcnt(WAITCNT_VM0);
+    __syncthreads();
+    // (twiddles loaded at their products: pre[] is held through this transform)
+    run_plan<P, R0, R1, R2, Z_COLS, NT, false>(a.tw, +1, true, ll, ll, ls, ls, true);
   }
 
   // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
@@ -500,7 +589,16 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   // (one memory round trip rather than one per owned output); further
   // entries (points in overlapping spheres) follow.
   if constexpr (!GS) {
-    if (!a.Pc) {
+    if (!a.Pc && a.own) {
+      // the ownership words again (not held through the transform)
+#pragma unroll
+      for (int k = 0; k < NE; k++) {
+        const int e = threadIdx.x + k * NT;
+        const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+        own[k] = e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb
+                     ? a.own[((long)x * L + y0 + y) * L + z]
+                     : -1;
+      }
       cplx q[NE];
 #pragma unroll
       for (int k = 0; k < NE; k++) {
