@@ -126,8 +126,6 @@ struct fft3_args {
   // optional copy of psi's forward transform (before the kinetic factor),
   // band-major [nb][kz][kx][ky] like W, chunk-offset (nullptr: none)
   cplx *gdump;
-  // hpsi written with non-temporal stores (streamed past the caches)
-  int nt_store;
   // hpsi not written at all (the last term of a Taylor chain: only the
   // accumulators need it)
   int skip_store;

@@ -430,13 +430,17 @@ __device__ __forceinline__ void vnl_expand_wave(const fft3_args &a, int gb0, con
         const double mv[4] = {mm.x, mm.y, mm.z, mm.w};
         const cplx f = a.phase_ent[en];
         const cplx *Pp = a.Pc + (long)sp * a.nlmP * a.ldPc + cb;
+        // the four coefficient loads together (lm past nlmP rereads lm 0), then
+        // the sum over the sphere's lm in order
+        cplx c[4];
+#pragma unroll
+        for (int lm = 0; lm < 4; lm++) c[lm] = Pp[(long)(lm < a.nlmP ? lm : 0) * a.ldPc];
         cplx u = mk(0.0, 0.0);
 #pragma unroll
         for (int lm = 0; lm < 4; lm++) {
           if (lm < a.nlmP) {
-            const cplx c = Pp[(long)lm * a.ldPc];
-            u.x += mv[lm] * c.x;
-            u.y += mv[lm] * c.y;
+            u.x += mv[lm] * c[lm].x;
+            u.y += mv[lm] * c[lm].y;
           }
         }
         v = cadd(v, cmul(f, u));
@@ -620,21 +624,13 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     }
   }
 
+  // Output k of this thread: element e = thread + k NT of the tile, band b,
+  // row y, plane z; valid inside the grid and the chunk.
   auto result = [&](int k, int b, int y, int z) {
     cplx v = cscale(a.scale, lds[z * Z_COLS + zswz(b * Z_TY + y)]);
     v.x += pre[k].x;
     v.y += pre[k].y;
     return v;
-  };
-  auto store = [&](long p, int cb, cplx v) {
-    if (a.skip_store) return;
-    if (a.nt_store) {
-      double *o = reinterpret_cast<double *>(a.hpsi + p * a.hld + cb);
-      __builtin_nontemporal_store(v.x, o);
-      __builtin_nontemporal_store(v.y, o + 1);
-    } else {
-      a.hpsi[p * a.hld + cb] = v;
-    }
   };
   if constexpr (!ACC) {
 #pragma unroll
@@ -643,53 +639,83 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
       const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
       if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
         const cplx v = result(k, b, y, z);
-        store(((long)x * L + y0 + y) * L + z, gb0 + b, v);
+        if (!a.skip_store) a.hpsi[(((long)x * L + y0 + y) * L + z) * a.hld + gb0 + b] = v;
         if constexpr (FWD) lds[z * Z_COLS + zswz(b * Z_TY + y)] = v;
       }
     }
   } else {
-    // Outputs in groups of G: the group's Taylor accumulators are read
-    // before any of its stores, so their loads overlap instead of each
-    // waiting behind the previous store (the compiler cannot know that the
-    // accumulators alias nothing else here).
-    constexpr int G = NE < 4 ? NE : 4;
+    // The Taylor accumulators of G outputs are all loaded (an invalid output
+    // loads point 0, band gb0 and stores nothing), then the outputs and the
+    // updated accumulators are formed, then stored, each kind of store in a
+    // loop of its own: no store waits for a load behind it.
+    constexpr int G = NE <= 6 ? NE : (R0 >= 12 ? 2 : (NE + 1) / 2);
+    const cplx *s1 = a.acc1_init ? a.psi : a.acc1;
+    const long l1 = a.acc1_init ? a.ld : a.acc1_ld;
 #pragma unroll
     for (int k0 = 0; k0 < NE; k0 += G) {
-      cplx o1[G], o2[G];
+      constexpr int GG = G;
+      cplx o1[GG], o2[GG], v[GG];
+      long pp[GG];
+      int cbk[GG];
+      bool ok[GG];
 #pragma unroll
-      for (int g = 0; g < G && k0 + g < NE; g++) {
+      for (int g = 0; g < GG; g++) {
         const int e = threadIdx.x + (k0 + g) * NT;
         const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-        if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
-          const long p = ((long)x * L + y0 + y) * L + z;
-          if (a.acc1)
-            o1[g] = a.acc1_init ? a.psi[p * a.ld + gb0 + b] : a.acc1[p * a.acc1_ld + gb0 + b];
-          if (a.acc2) o2[g] = a.acc2[p * a.acc2_ld + gb0 + b];
-        }
+        ok[g] = k0 + g < NE && e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb;
+        pp[g] = ok[g] ? ((long)x * L + y0 + y) * L + z : 0;
+        cbk[g] = gb0 + (ok[g] ? b : 0);
+      }
+      if (a.acc1) {
+#pragma unroll
+        for (int g = 0; g < GG; g++) o1[g] = s1[pp[g] * l1 + cbk[g]];
+      }
+      if (a.acc2) {
+#pragma unroll
+        for (int g = 0; g < GG; g++) o2[g] = a.acc2[pp[g] * a.acc2_ld + cbk[g]];
       }
 #pragma unroll
-      for (int g = 0; g < G && k0 + g < NE; g++) {
+      for (int g = 0; g < GG; g++) {
         const int e = threadIdx.x + (k0 + g) * NT;
         const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
-        if (e < L * Z_COLS && y0 + y < L && cb0 + b < a.nb) {
-          const long p = ((long)x * L + y0 + y) * L + z;
-          const int cb = gb0 + b;
-          const cplx v = result(k0 + g, b, y, z);
-          store(p, cb, v);
-          if constexpr (FWD) lds[z * Z_COLS + zswz(b * Z_TY + y)] = v;
-          if (a.acc1) {
-            cplx o = o1[g];
-            o.x += a.c1re * v.x - a.c1im * v.y;
-            o.y += a.c1re * v.y + a.c1im * v.x;
-            a.acc1[p * a.acc1_ld + cb] = o;
-          }
-          if (a.acc2) {
-            cplx o = o2[g];
-            o.x += a.c2re * v.x - a.c2im * v.y;
-            o.y += a.c2re * v.y + a.c2im * v.x;
-            a.acc2[p * a.acc2_ld + cb] = o;
-          }
+        v[g] = ok[g] ? result(k0 + g, b, y, z) : mk(0.0, 0.0);
+      }
+      if (a.acc1) {
+#pragma unroll
+        for (int g = 0; g < GG; g++) {
+          o1[g].x += a.c1re * v[g].x - a.c1im * v[g].y;
+          o1[g].y += a.c1re * v[g].y + a.c1im * v[g].x;
         }
+      }
+      if (a.acc2) {
+#pragma unroll
+        for (int g = 0; g < GG; g++) {
+          o2[g].x += a.c2re * v[g].x - a.c2im * v[g].y;
+          o2[g].y += a.c2re * v[g].y + a.c2im * v[g].x;
+        }
+      }
+      if (!a.skip_store) {
+#pragma unroll
+        for (int g = 0; g < GG; g++)
+          if (ok[g]) a.hpsi[pp[g] * a.hld + cbk[g]] = v[g];
+      }
+      if constexpr (FWD) {
+#pragma unroll
+        for (int g = 0; g < GG; g++) {
+          const int e = threadIdx.x + (k0 + g) * NT;
+          const int b = e % Z_TB, y = (e / Z_TB) % Z_TY, z = e / Z_COLS;
+          if (ok[g]) lds[z * Z_COLS + zswz(b * Z_TY + y)] = v[g];
+        }
+      }
+      if (a.acc1) {
+#pragma unroll
+        for (int g = 0; g < GG; g++)
+          if (ok[g]) a.acc1[pp[g] * a.acc1_ld + cbk[g]] = o1[g];
+      }
+      if (a.acc2) {
+#pragma unroll
+        for (int g = 0; g < GG; g++)
+          if (ok[g]) a.acc2[pp[g] * a.acc2_ld + cbk[g]] = o2[g];
       }
     }
   }

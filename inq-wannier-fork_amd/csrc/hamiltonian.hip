@@ -317,11 +317,6 @@ static void local_passes(handle &h, const cplx *psi, long ld, cplx *hpsi, long h
       if (gskip) throw error{ERR_UNSUPPORTED, "band-group chunks with a transform dump"};
       a.gdump = acc->gdump + (size_t)b0 * h.npoint;
     }
-    static const int nt_store = [] {
-      const char *e = getenv("INQAMD_NT_STORE");
-      return e && atoi(e) == 1 ? 1 : 0;
-    }();
-    a.nt_store = nt_store;
     a.skip_store = skip_store ? 1 : 0;
     a.z_ty = h.tune_z_ty;
     a.xy_wide = h.tune_xy_wide;
