@@ -55,15 +55,15 @@ def lookup(n, nst, path=_PATH):
 
 
 def chunk_candidates(L, nst):
-    """Band chunks worth timing: intermediates of 48-384 MiB per chunk (the
-    sweeps' intermediate of one chunk is 16 L^3 bytes per band), multiples
-    of 8, at most half the bands (two chunks are in flight), and 0 (the
-    library's choice)."""
+    """Band chunks worth timing: intermediates of 32-384 MiB per chunk (the
+    sweeps' intermediate of one chunk is 16 L^3 bytes per band) and 16, 32
+    or 64 bands, multiples of 8, at most half the bands (two chunks are in
+    flight), and 0 (the library's choice)."""
     per_band = 16.0 * L ** 3
     out = {0}
-    for mib in (48, 96, 192, 384):
-        b = int(mib * (1 << 20) / per_band) // 8 * 8
-        b = max(8, b)
+    sizes = [max(8, int(mib * (1 << 20) / per_band) // 8 * 8)
+             for mib in (32, 48, 64, 96, 128, 192, 384)]
+    for b in sizes + [16, 32, 64]:
         if 2 * b <= nst:
             out.add(b)
     return sorted(out)
