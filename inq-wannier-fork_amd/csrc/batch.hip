@@ -364,7 +364,7 @@ struct inqamd_batch {
     int p0, np;
     int L, ty;
     int zitems, xyitems;
-    size_t zlds, xylds;
+    size_t zlds, zlds_b, xylds;  // z forward, z backward (tile + twiddles), x-y
     int znt, xynt;
     int zgrid_f, zgrid_b, xygrid;
     group_kernels k;
@@ -578,6 +578,7 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       g.zitems += (int)zi;
       g.xyitems += (int)xi;
       g.zlds = std::max(g.zlds, (size_t)q.L * Z_COLS * sizeof(cplx));
+      g.zlds_b = std::max(g.zlds_b, z_bwd_lds(q.L));
       g.xylds = std::max(g.xylds, batch_xy_lds(q.L));
       g.xynt = std::max(g.xynt, plane_threads(q.L));
       g.np++;
@@ -594,12 +595,12 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
       g.k.xy = (const void *)batch_xy_kernel;
       g.k.zb = (const void *)batch_z_bwd_kernel;
     }
-    if (g.zlds > 65536) {
+    if (g.zlds > 65536)
       INQAMD_HIP_CHECK(hipFuncSetAttribute(g.k.zf, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)g.zlds));
+    if (g.zlds_b > 65536)
       INQAMD_HIP_CHECK(hipFuncSetAttribute(g.k.zb, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)g.zlds));
-    }
+                                           (int)g.zlds_b));
     if (g.xylds > 65536)
       INQAMD_HIP_CHECK(hipFuncSetAttribute(g.k.xy, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)g.xylds));
@@ -612,7 +613,7 @@ void batch_build(inqamd_batch *b, const void *const *psi, void *const *hpsi, con
     } else {
       g.znt = Z_NT;
       g.zgrid_f = resident_grid(g.k.zf, Z_NT, g.zlds, b->cus, g.zitems);
-      g.zgrid_b = resident_grid(g.k.zb, Z_NT, g.zlds, b->cus, g.zitems);
+      g.zgrid_b = resident_grid(g.k.zb, Z_NT, g.zlds_b, b->cus, g.zitems);
       g.xygrid = resident_grid(g.k.xy, g.xynt, g.xylds, b->cus, g.xyitems);
     }
   }
@@ -690,7 +691,7 @@ void batch_run(inqamd_batch *b) {
         const void *k = sweep == 0 ? g.k.zf : (sweep == 1 ? g.k.xy : g.k.zb);
         const int grid = sweep == 0 ? g.zgrid_f : (sweep == 1 ? g.xygrid : g.zgrid_b);
         const int nt = sweep == 1 ? g.xynt : g.znt;
-        const size_t lds = sweep == 1 ? g.xylds : g.zlds;
+        const size_t lds = sweep == 0 ? g.zlds : (sweep == 1 ? g.xylds : g.zlds_b);
         int np = g.np, total = sweep == 1 ? g.xyitems : g.zitems;
         void *args[] = {(void *)&pc, (void *)&np, (void *)&cnt, (void *)&total};
         INQAMD_HIP_CHECK(hipLaunchKernel(k, dim3(grid), dim3(nt), args, lds, cs));
@@ -754,18 +755,19 @@ int inqamd_batch_create(inqamd_hamiltonian_t const *handles, int nsys, void *str
     int dev = 0;
     INQAMD_HIP_CHECK(hipGetDevice(&dev));
     INQAMD_HIP_CHECK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
-    size_t zmax = 0, xymax = 0;
+    size_t zmax = 0, zbmax = 0, xymax = 0;
     for (auto *h : b->hs)
       if (h->three_sweep) {
         zmax = std::max(zmax, (size_t)h->n[0] * Z_COLS * sizeof(cplx));
+        zbmax = std::max(zbmax, z_bwd_lds(h->n[0]));
         xymax = std::max(xymax, (size_t)h->n[0] * (h->n[0] + 1) * sizeof(cplx));
       }
-    if (zmax > 65536) {
+    if (zmax > 65536)
       INQAMD_HIP_CHECK(hipFuncSetAttribute((const void *)batch_z_fwd_kernel,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)zmax));
+    if (zbmax > 65536)
       INQAMD_HIP_CHECK(hipFuncSetAttribute((const void *)batch_z_bwd_kernel,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)zmax));
-    }
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)zbmax));
     if (xymax > 65536)
       INQAMD_HIP_CHECK(hipFuncSetAttribute((const void *)batch_xy_kernel,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)xymax));

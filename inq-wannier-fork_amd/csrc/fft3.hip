@@ -102,12 +102,13 @@ static void prepare3() {
   ensure_lds(z_pot_kernel<R0, R1, R2, 8>, zl);
   ensure_lds(z_kin_kernel<R0, R1, R2, 8>, zl);
   ensure_lds(xy_fwd_kernel<R0, R1, R2, wide_threads(L)>, pl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, false>, zl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, false>, zl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true>, zl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true>, zl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true, true>, zl);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true, true>, zl);
+  const size_t zb = z_bwd_lds(L);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, false>, zb);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, false>, zb);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true>, zb);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true>, zb);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true, true>, zb);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true, true>, zb);
 }
 
 // sweep: -1 all three, 0/1/2 one of them; fwd_next: the last sweep also
@@ -129,15 +130,16 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep, bool fwd
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   if (sweep < 0 || sweep == 2) {
+    const size_t zb = z_bwd_lds(L);
     if (fwd_next)
       hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true, true>), dim3((unsigned)ztiles),
-                         dim3(z_threads(L)), zlds, stream, a);
+                         dim3(z_threads(L)), zb, stream, a);
     else if (a.acc1 || a.acc2)
       hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true>), dim3((unsigned)ztiles),
-                         dim3(z_threads(L)), zlds, stream, a);
+                         dim3(z_threads(L)), zb, stream, a);
     else
       hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, false>), dim3((unsigned)ztiles),
-                         dim3(z_threads(L)), zlds, stream, a);
+                         dim3(z_threads(L)), zb, stream, a);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
 }
@@ -192,8 +194,7 @@ bool fft3_supported(int L, size_t max_lds) {
 #define INQAMD_FFT3_CASE(LL, R0, R1, R2) case LL:
     INQAMD_FFT3_TABLE(INQAMD_FFT3_CASE)
 #undef INQAMD_FFT3_CASE
-      return (size_t)L * (L + 1) * sizeof(cplx) <= max_lds &&
-             (size_t)L * Z_COLS * sizeof(cplx) <= max_lds;
+      return (size_t)L * (L + 1) * sizeof(cplx) <= max_lds && z_bwd_lds(L) <= max_lds;
     default: return false;
   }
 }

@@ -484,14 +484,20 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   // are in flight with the prefetch below and the V_nl expansion, which use
   // no LDS.  Columns outside the grid or the chunk load W[0] instead; their
   // transforms are never stored.
+  // The twiddle table (L entries) goes the same way into the L slots after
+  // the tile, so that the transforms' twiddles are LDS reads.
+  cplx *ltw = lds + L * Z_COLS;
   if constexpr (!GS) {
-    const int c = zswz(threadIdx.x & 63);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = zswz(lane);
     const int b = c / Z_TY, y = c % Z_TY;
     const bool ok = y0 + y < L && cb0 + b < a.nb;
     const long col = ok ? (((long)(cb0 + b) * L) * L + x) * L + y0 + y : 0;
     const long row = ok ? (long)L * L : 0;
-    for (int i = threadIdx.x >> 6; i < L; i += NT / 64)
+    for (int i = wave; i < L; i += NT / 64)
       __builtin_amdgcn_global_load_lds(a.W + col + i * row, lds + i * Z_COLS, 16, 0, 0);
+    if (64 * wave < L && 64 * wave + lane < L)
+      __builtin_amdgcn_global_load_lds(a.tw + 64 * wave + lane, ltw + 64 * wave, 16, 0, 0);
   }
 
   // prefetch V psi (and the ownership entries) of this thread's outputs,
@@ -579,12 +585,12 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
     }
     __syncthreads();
   } else {
-    // the W rows loaded into the tile at the start: this wave's loads are
-    // complete (vmcnt 0), the barrier makes every wave's rows visible
+    // the W rows and twiddles loaded into LDS at the start: this wave's
+    // loads are complete (vmcnt 0), the barrier makes every wave's visible
     __builtin_amdgcn_s_waitcnt(WAITCNT_VM0);
     __syncthreads();
-    // (twiddles loaded at their products: pre[] is held through this transform)
-    run_plan<P, R0, R1, R2, Z_COLS, NT, false>(a.tw, +1, true, ll, ll, ls, ls, true);
+    // (twiddles read at their products: pre[] is held through this transform)
+    run_plan<P, R0, R1, R2, Z_COLS, NT, false>(ltw, +1, true, ll, ll, ls, ls, true);
   }
 
   // projector_all::apply (projector_all.hpp:225-231): V_nl psi of the owned
@@ -727,9 +733,12 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
       if (y0 + y < L && cb0 + b < a.nb)
         a.W[(((long)(cb0 + b) * L + i) * L + x) * L + y0 + y] = v;
     };
-    run_plan<P, R0, R1, R2, Z_COLS, NT>(a.tw, -1, true, ll, ll, ls, gs, false);
+    run_plan<P, R0, R1, R2, Z_COLS, NT>(ltw, -1, true, ll, ll, ls, gs, false);
   }
 }
+
+// Dynamic LDS of a z_bwd_epi tile: the tile and the twiddle table
+constexpr size_t z_bwd_lds(int L) { return ((size_t)L * Z_COLS + L) * sizeof(cplx); }
 
 // Occupancy floor of the z_bwd_epi kernels (amdgpu_waves_per_eu): at least
 // 4 waves per SIMD (two 512-thread z tiles per CU, or one of 1024) where the
