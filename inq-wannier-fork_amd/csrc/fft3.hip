@@ -36,13 +36,28 @@ void xy_kin_kernel(const fft3_args a) {
   xy_kin_plane<R0, R1, R2, NT>(a, blockIdx.x, lds);
 }
 
+// TY: y values of a 64-column tile (8 or 16); a 32-column tile takes TY / 2
+// (the same bands per tile)
+constexpr int z_bwd_ty(int L, bool acc, int ty) { return z_bwd_cols(L, acc) == Z_COLS ? ty : ty / 2; }
+
 template <int R0, int R1, int R2, int TY, bool ACC, bool FWD = false>
-__global__ __launch_bounds__(z_threads(R0 * R1 * R2))
+__global__ __launch_bounds__(z_bwd_threads(R0 * R1 * R2, ACC))
 __attribute__((amdgpu_waves_per_eu(z_bwd_waves(R0 * R1 * R2))))
 void z_bwd_epi_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), ACC, false, FWD>(a, blockIdx.x, lds);
+  constexpr int L = R0 * R1 * R2;
+  z_bwd_epi_tile<R0, R1, R2, z_bwd_ty(L, ACC, TY), z_bwd_threads(L, ACC), ACC, false, FWD,
+                 z_bwd_cols(L, ACC)>(a, blockIdx.x, lds);
 }
+
+// Launch shape of a z_bwd_epi variant (acc: the Taylor forms)
+template <int R0, int R1, int R2, int TY, bool ACC>
+struct z_bwd_shape {
+  static constexpr int L = R0 * R1 * R2, ZC = z_bwd_cols(L, ACC);
+  static long tiles(int nb) { return z_tiles<z_bwd_ty(L, ACC, TY), ZC>(L, nb); }
+  static constexpr size_t lds = z_bwd_lds(L, ZC);
+  static constexpr int threads = z_bwd_threads(L, ACC);
+};
 
 // G-space Hpsi sweeps
 template <int R0, int R1, int R2, int TY>
@@ -102,13 +117,13 @@ static void prepare3() {
   ensure_lds(z_pot_kernel<R0, R1, R2, 8>, zl);
   ensure_lds(z_kin_kernel<R0, R1, R2, 8>, zl);
   ensure_lds(xy_fwd_kernel<R0, R1, R2, wide_threads(L)>, pl);
-  const size_t zb = z_bwd_lds(L);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, false>, zb);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, false>, zb);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true>, zb);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true>, zb);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true, true>, zb);
-  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true, true>, zb);
+  const size_t zbp = z_bwd_lds(L, z_bwd_cols(L, false)), zba = z_bwd_lds(L, z_bwd_cols(L, true));
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, false>, zbp);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, false>, zbp);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true>, zba);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true>, zba);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 8, true, true>, zba);
+  ensure_lds(z_bwd_epi_kernel<R0, R1, R2, 16, true, true>, zba);
 }
 
 // sweep: -1 all three, 0/1/2 one of them; fwd_next: the last sweep also
@@ -130,16 +145,17 @@ static void launch3v(const fft3_args &a, hipStream_t stream, int sweep, bool fwd
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   if (sweep < 0 || sweep == 2) {
-    const size_t zb = z_bwd_lds(L);
+    using SA = z_bwd_shape<R0, R1, R2, TY, true>;
+    using SP = z_bwd_shape<R0, R1, R2, TY, false>;
     if (fwd_next)
-      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true, true>), dim3((unsigned)ztiles),
-                         dim3(z_threads(L)), zb, stream, a);
+      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true, true>), dim3((unsigned)SA::tiles(a.nb)),
+                         dim3(SA::threads), SA::lds, stream, a);
     else if (a.acc1 || a.acc2)
-      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true>), dim3((unsigned)ztiles),
-                         dim3(z_threads(L)), zb, stream, a);
+      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, true>), dim3((unsigned)SA::tiles(a.nb)),
+                         dim3(SA::threads), SA::lds, stream, a);
     else
-      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, false>), dim3((unsigned)ztiles),
-                         dim3(z_threads(L)), zb, stream, a);
+      hipLaunchKernelGGL((z_bwd_epi_kernel<R0, R1, R2, TY, false>), dim3((unsigned)SP::tiles(a.nb)),
+                         dim3(SP::threads), SP::lds, stream, a);
     INQAMD_HIP_CHECK(hipGetLastError());
   }
 }

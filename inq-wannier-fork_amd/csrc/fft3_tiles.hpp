@@ -29,6 +29,16 @@ constexpr int Z_NT = 512;
 // FFT stages' contiguous column accesses conflict-free.
 __device__ __forceinline__ int zswz(int c) { return c ^ (((c >> 4) & 3) << 1); }
 
+// The same for rows of ZC = 32 or 64 columns (32: the bit above a 16-column
+// block flips bit 1, so the transposed accesses of a quarter wave, 8 bands x
+// 2 y, fall in distinct bank groups; the FFT stages' accesses of 16
+// consecutive columns stay in their aligned 16-slot block)
+template <int ZC>
+__device__ __forceinline__ int zswz_n(int c) {
+  static_assert(ZC == 32 || ZC == 64, "z tile width");
+  return c ^ (((c >> 4) & (ZC / 16 - 1)) << 1);
+}
+
 constexpr int plane_threads(int L) {
   int t = (L * L / 16 + 63) / 64 * 64;
   return t < 64 ? 64 : (t > 1024 ? 1024 : t);
@@ -460,12 +470,17 @@ __device__ __forceinline__ void vnl_expand_wave(const fft3_args &a, int gb0, con
   }
 }
 
+// ZC: columns of the tile (TY y values x ZC / TY bands), 64 or 32
 template <int R0, int R1, int R2, int Z_TY, int NT = Z_NT, bool ACC = false, bool GS = false,
-          bool FWD = false>
+          bool FWD = false, int ZC = Z_COLS>
 __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx *lds) {
   using P = plan3<R0, R1, R2>;
   constexpr int L = P::LEN;
+  constexpr int Z_COLS = ZC;  // (shadows the namespace constant in this tile)
   constexpr int Z_TB = Z_COLS / Z_TY;
+  constexpr int RPI = 64 / Z_COLS;  // tile rows one wave's 16-byte DMA fills
+  static_assert(L % RPI == 0, "whole rows per DMA instruction");
+  auto zswz = [](int c) { return zswz_n<ZC>(c); };
   static_assert(NT % Z_TB == 0, "thread -> band mapping of the epilogue");
   constexpr int NE = (L * Z_COLS + NT - 1) / NT;  // output elements per thread
 
@@ -479,23 +494,23 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   const int gb0 = cb0 + (cb0 >> (3 + a.glog)) * a.gskip;  // the tile's first band in psi, hpsi
 
   // The tile's W rows go straight from HBM into the LDS tile (16 B per lane:
-  // lane l of a wave's load fills slot l of one 64-column row, so it loads
-  // column zswz(l), zswz being its own inverse), issued first so that they
-  // are in flight with the prefetch below and the V_nl expansion, which use
-  // no LDS.  Columns outside the grid or the chunk load W[0] instead; their
-  // transforms are never stored.
+  // lane l of a wave's load fills slot l of 64 / ZC consecutive rows, so it
+  // loads row l / ZC, column zswz(l % ZC), zswz being its own inverse),
+  // issued first so that they are in flight with the prefetch below and the
+  // V_nl expansion, which use no LDS.  Columns outside the grid or the chunk
+  // load W[0] instead; their transforms are never stored.
   // The twiddle table (L entries) goes the same way into the L slots after
   // the tile, so that the transforms' twiddles are LDS reads.
   cplx *ltw = lds + L * Z_COLS;
   if constexpr (!GS) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int c = zswz(lane);
+    const int lr = lane / Z_COLS, c = zswz(lane % Z_COLS);
     const int b = c / Z_TY, y = c % Z_TY;
     const bool ok = y0 + y < L && cb0 + b < a.nb;
     const long col = ok ? (((long)(cb0 + b) * L) * L + x) * L + y0 + y : 0;
     const long row = ok ? (long)L * L : 0;
-    for (int i = wave; i < L; i += NT / 64)
-      __builtin_amdgcn_global_load_lds(a.W + col + i * row, lds + i * Z_COLS, 16, 0, 0);
+    for (int i = wave * RPI; i < L; i += (NT / 64) * RPI)
+      __builtin_amdgcn_global_load_lds(a.W + col + (i + lr) * row, lds + i * Z_COLS, 16, 0, 0);
     if (64 * wave < L && 64 * wave + lane < L)
       __builtin_amdgcn_global_load_lds(a.tw + 64 * wave + lane, ltw + 64 * wave, 16, 0, 0);
   }
@@ -738,7 +753,8 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
 }
 
 // Dynamic LDS of a z_bwd_epi tile: the tile and the twiddle table
-constexpr size_t z_bwd_lds(int L) { return ((size_t)L * Z_COLS + L) * sizeof(cplx); }
+constexpr size_t z_bwd_lds(int L, int zc = Z_COLS) { return ((size_t)L * zc + L) * sizeof(cplx); }
+
 
 // Occupancy floor of the z_bwd_epi kernels (amdgpu_waves_per_eu): at least
 // 4 waves per SIMD (two 512-thread z tiles per CU, or one of 1024) where the
@@ -755,6 +771,16 @@ constexpr int z_bwd_min_waves(int L) { return L == 72 ? 1 : 4; }
 constexpr int z_threads(int L) { return L >= 80 ? 1024 : (L == 72 ? 768 : Z_NT); }
 constexpr int z_bwd_waves(int L) { return L == 72 ? 4 : z_bwd_min_waves(L); }
 
+// Columns and threads of the one-system z_bwd_epi tiles: 32 columns of 512
+// threads for the plain Hpsi at 96 (48 KB of LDS: two tiles share a CU,
+// where a 64-column tile of 96 KB held a CU alone: z_bwd_epi 6.30 -> 5.30 ms
+// per 96^3 x 512 Hpsi), else 64 columns of z_threads(L).  The Taylor forms
+// (acc) keep 64 columns: with 32 the 96^3 ETRS step ran 103 -> 113 ms (the
+// form that transforms its output forward writes W in 64-byte runs of 4 y)
+// and 103.4 -> 104.3 ms with only the last order's form at 32.
+constexpr int z_bwd_cols(int L, bool acc) { return L == 96 && !acc ? 32 : Z_COLS; }
+constexpr int z_bwd_threads(int L, bool acc) { return L == 96 && !acc ? Z_NT : z_threads(L); }
+
 // Grid lengths with three-sweep kernels: (L, radices of the plan)
 #define INQAMD_FFT3_TABLE(X) \
   X(24, 8, 3, 1)             \
@@ -769,10 +795,10 @@ constexpr int z_bwd_waves(int L) { return L == 72 ? 4 : z_bwd_min_waves(L); }
   X(80, 8, 2, 5)             \
   X(96, 8, 4, 3)
 
-// Work items of a band chunk of nb bands
-template <int Z_TY>
+// Work items of a band chunk of nb bands (tiles of ZC columns)
+template <int Z_TY, int ZC = Z_COLS>
 inline long z_tiles(int L, int nb) {
-  constexpr int Z_TB = Z_COLS / Z_TY;
+  constexpr int Z_TB = ZC / Z_TY;
   return (long)L * ((L + Z_TY - 1) / Z_TY) * ((nb + Z_TB - 1) / Z_TB);
 }
 
