@@ -335,7 +335,8 @@ def test_energy_terms_match_oracle(device, cfg0):
 
 def test_hpsi_96_matches_oracle(device):
     """The 3x3x3 supercell grid (96^3, 216 atoms): three-sweep Hpsi with the
-    1024-thread z tiles, 3 bands against the oracle."""
+    1024-thread z forward tiles and the 32-column z_bwd_epi tiles (one band
+    tile, 3 of its 8 bands valid), 3 bands against the oracle."""
     s = systems.diamond_system(3)
     psi = systems.random_orbitals(s, 3)
     say("hpsi 96^3 x 3")
@@ -555,6 +556,30 @@ def test_pipeline_variants_in_fresh_process(device, env):
                        timeout=240)
     say(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0 and "ENV-VARIANT-OK" in r.stdout
+
+
+def test_hpsi_96_z_tile_shapes(device):
+    """96^3, 20 bands (two whole 8-band tiles and a ragged one): the 32-column
+    z_bwd_epi tiles of 4 y x 8 bands (default) and 8 y x 4 bands (z_tile_y =
+    16) give the same bits, and bands 0 and 19 match the oracle."""
+    s = systems.diamond_system(3)
+    nst = 20
+    say("hpsi 96^3 x 20, z tile shapes")
+    g = torch.Generator(device=device).manual_seed(960)
+    d = torch.randn(tuple(s.n) + (nst,), dtype=torch.complex128, device=device, generator=g)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=nst, tuning=None)
+    ham.uniform_vector_potential = (0.01, 0.02, -0.03)
+    h8 = ham(d)
+    ham.tuning = {"z_tile_y": 16}
+    h16 = ham(d)
+    torch.cuda.synchronize()
+    assert torch.equal(h8, h16)
+    x = np.ascontiguousarray(d[..., [0, 19]].cpu().numpy())
+    ref = O.System(s.n, s.lattice, s.vloc, s.tables).hamiltonian(x, (0.01, 0.02, -0.03))
+    err = relerr(h8[..., [0, 19]].cpu().numpy(), ref)
+    say("96^3 Hpsi vs oracle on 2 of 20 bands %.2e" % err)
+    assert err < TOL
+    ham.close()
 
 
 def test_band_groups_96_match_padded_rows(device):
