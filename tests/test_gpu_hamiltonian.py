@@ -384,6 +384,30 @@ def test_hpsi_60_72_match_oracle(device, L):
     ham.close()
 
 
+@pytest.mark.parametrize("L", [24, 30, 36, 40, 80])
+def test_hpsi_remaining_grids_match_oracle(device, L):
+    """The other grid lengths with three-sweep kernels (24 = 8 x 3, 30 = 6 x
+    5, 36 = 6 x 6, 40 = 8 x 5, and 80 = 8 x 2 x 5 with 1024-thread z tiles):
+    Hpsi of 5 bands (one ragged band tile) with a vector potential, and a
+    Taylor-4 exponential of 3 bands, against the oracle."""
+    s = systems.uneven_batch_system(11, L, 4)
+    psi = systems.random_orbitals(s, 5, seed=5, system_id=L)
+    kA = (-0.03, 0.02, 0.01)
+    say("hpsi %d^3 x 5" % L)
+    ham = inq_amd.ks_hamiltonian(s, max_bands=5)
+    ham.uniform_vector_potential = kA
+    h = to_host(ham(to_dev(psi, device)))
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+    ref = osys.hamiltonian(psi, kA)
+    assert relerr(h, ref) < TOL
+    assert band_relerr(h, ref) < TOL
+    say("exponential %d^3 x 3" % L)
+    e = to_dev(psi[..., :3], device)
+    operations.exponential_in_place(ham, 0.025j, e, 4)
+    assert relerr(to_host(e), osys.exponential_in_place(psi[..., :3], 0.025j, 4, kA)) < TOL
+    ham.close()
+
+
 def test_band_group_chunks_match_contiguous_bitwise(device, cfg1):
     """64^3 x 256 at ld = 256 runs its chunks as interleaved 8-band groups;
     the same bands at ld = 264 run contiguous chunks.  Hpsi, G-space Hpsi,
