@@ -46,8 +46,13 @@ __attribute__((amdgpu_waves_per_eu(z_bwd_waves(R0 * R1 * R2))))
 void z_bwd_epi_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
   constexpr int L = R0 * R1 * R2;
+  // the plain Hpsi's tiles dealt per XCD (64^3 x 256 Hpsi 1.495 -> 1.480
+  // ms); the Taylor forms keep the natural order (their steps ran 1 % slower
+  // dealt per XCD)
+  constexpr int TB = z_bwd_cols(L, ACC) / z_bwd_ty(L, ACC, TY);
+  const long t = ACC ? (long)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x, (a.nb + TB - 1) / TB);
   z_bwd_epi_tile<R0, R1, R2, z_bwd_ty(L, ACC, TY), z_bwd_threads(L, ACC), ACC, false, FWD,
-                 z_bwd_cols(L, ACC)>(a, blockIdx.x, lds);
+                 z_bwd_cols(L, ACC)>(a, t, lds);
 }
 
 // Launch shape of a z_bwd_epi variant (acc: the Taylor forms)

@@ -752,6 +752,20 @@ __device__ __forceinline__ void z_bwd_epi_tile(const fft3_args &a, long t, cplx 
   }
 }
 
+// Tile of workgroup b of T when the nbt band tiles of one (x, y rows) pair
+// (consecutive in the natural order) should run on one XCD, whose L2 then
+// serves their shared V, ownership and sphere words once: workgroups go to
+// the 8 XCDs in turn, so each group of 8 pairs x nbt band tiles is dealt
+// with pair p's band tiles on workgroups b, b + 8, b + 16, ...  The tail
+// beyond whole groups keeps the natural order.
+__device__ __forceinline__ long xcd_tile(long b, long T, int nbt) {
+  constexpr int G = 8;
+  const long full = T / ((long)G * nbt) * G * nbt;
+  if (b >= full) return b;
+  const long q = b / (G * nbt), r = b % (G * nbt);
+  return (q * G + r % G) * nbt + r / G;
+}
+
 // Dynamic LDS of a z_bwd_epi tile: the tile and the twiddle table
 constexpr size_t z_bwd_lds(int L, int zc = Z_COLS) { return ((size_t)L * zc + L) * sizeof(cplx); }
 
