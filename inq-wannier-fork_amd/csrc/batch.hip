@@ -274,11 +274,17 @@ void group_xy_kernel(
 template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(z_threads(R0 * R1 * R2))
 __attribute__((amdgpu_waves_per_eu(z_bwd_waves(R0 * R1 * R2))))
-void group_z_bwd_kernel(const batch_piece *__restrict__ pc, int np, unsigned long long *, int) {
+void group_z_bwd_kernel(const batch_piece *__restrict__ pc, int np, unsigned long long *,
+                        int total) {
   extern __shared__ cplx lds[];
   const int item = blockIdx.x;
   const int i = find_segment(np, item, [&](int k) { return pc[k].z0; });
-  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2)>(pc[i].a, item - pc[i].z0, lds);
+  // the piece's tiles dealt per XCD as in the one-system kernel (workgroups
+  // b, b + 8, ... share an XCD whatever the piece's first item)
+  constexpr int TB = Z_COLS / TY;
+  const long z0 = pc[i].z0, zend = i + 1 < np ? pc[i + 1].z0 : total;
+  const long t = xcd_tile(item - z0, zend - z0, (pc[i].a.nb + TB - 1) / TB);
+  z_bwd_epi_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2)>(pc[i].a, t, lds);
 }
 
 // kernels of one launch group

@@ -68,7 +68,11 @@ struct z_bwd_shape {
 template <int R0, int R1, int R2, int TY>
 __global__ __launch_bounds__(z_threads(R0 * R1 * R2)) void z_pot_kernel(const fft3_args a) {
   extern __shared__ cplx lds[];
-  z_fwd_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), true>(a, blockIdx.x, lds);
+  // tiles dealt per XCD: one (x, y rows) pair's band tiles share V and the
+  // ownership words through one L2
+  constexpr int TB = Z_COLS / TY;
+  z_fwd_tile<R0, R1, R2, TY, z_threads(R0 * R1 * R2), true>(
+      a, xcd_tile(blockIdx.x, gridDim.x, (a.nb + TB - 1) / TB), lds);
 }
 
 template <int R0, int R1, int R2, int NT>
