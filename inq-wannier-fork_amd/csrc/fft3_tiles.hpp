@@ -3,15 +3,17 @@
 // work-queue kernels (batch.hip).
 //
 //   z_fwd_tile     psi [x][y][z][band] (INQ layout) -> W[band][z][x][y]
-//                  tile: one x, TY y, TB = 64/TY bands, all z.  A band-fastest
-//                  load into LDS, a z-FFT over the 64 (y, band) columns, and a
-//                  last stage that writes TY consecutive y per (band, z).
+//                  tile: one x, TY y, TB = 64/TY bands, all z.  The first
+//                  stage reads psi band-fastest straight into registers, a
+//                  z-FFT over the 64 (y, band) columns, and a last stage that
+//                  writes TY consecutive y per (band, z).
 //   xy_kin_plane   one (band, z) plane W[b][z][.][.] (contiguous L*L):
 //                  x-FFT, y-FFT, x 1/2|G+k+A|^2, inverse y, inverse x, in a
 //                  padded LDS plane; first/last stage touch HBM directly.
-//   z_bwd_epi_tile inverse z-FFT from W, then a transposing write
-//                  hpsi = FFT/N + V psi (+ V_nl psi of owned points, + Taylor
-//                  accumulations) in INQ layout.
+//   z_bwd_epi_tile W rows and twiddles copied into LDS by DMA while V psi and
+//                  V_nl psi of the owned points are formed in registers, the
+//                  inverse z-FFT, then a transposing write hpsi = FFT/N +
+//                  V psi + V_nl psi (+ Taylor accumulations) in INQ layout.
 #pragma once
 
 #include "fft.hpp"
