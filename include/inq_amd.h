@@ -102,7 +102,9 @@ int inqamd_stream_probe(int kind, const void *src, void *dst, long rows, int ld,
 /* orbital_set::kpoint() (orbital_set.hpp) -- covariant k. */
 /* Per-handle tiling of the three-sweep Hpsi: the autotuner's knobs, no
  * reference counterpart.  0 keeps the library's choice for that knob.
- *   z_tile_y         z tiles of 8 y x 8 bands or 16 y x 4 bands
+ *   z_tile_y         z tiles of 8 y x 8 bands or 16 y x 4 bands (the plain
+ *                    Hpsi's last sweep at 96 runs half-width tiles: 4 y x 8
+ *                    bands or 8 y x 4 bands)
  *   xy_threads       x-y plane workgroups of 1 or 2 x plane_threads(L)
  *   band_chunk       bands per chunk of the sweeps (multiple of 8, <= max_bands)
  *   projector_bands  bands per workgroup of the projector coefficients (16/32/64)
