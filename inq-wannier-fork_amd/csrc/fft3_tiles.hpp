@@ -41,6 +41,10 @@ __device__ __forceinline__ int zswz_n(int c) {
   return c ^ (((c >> 4) & (ZC / 16 - 1)) << 1);
 }
 
+// s_waitcnt immediate that waits for every vector memory operation of the
+// wave (vmcnt 0) and leaves the export and LDS/scalar counters alone
+constexpr unsigned WAITCNT_VM0 = 0x0F70;
+
 constexpr int plane_threads(int L) {
   int t = (L * L / 16 + 63) / 64 * 64;
   return t < 64 ? 64 : (t > 1024 ? 1024 : t);
@@ -130,11 +134,26 @@ constexpr bool xy_split() {
   return xy_plan<R0, R1, R2>::P::NSTAGE == 2 && 2 * L * (L + 1) * 16 > 160 * 1024 && L < 96;
 }
 
+// LDS of an x-y kinetic plane's data
+template <int R0, int R1, int R2>
+constexpr size_t xy_plane_lds() {
+  constexpr size_t L = R0 * R1 * R2;
+  return L * (L + 1) * (xy_split<R0, R1, R2>() ? sizeof(double) : sizeof(cplx));
+}
+
+// The long planes (72, 80, 96: one to three per CU) take their twiddle table
+// into LDS behind the plane.  (Shorter planes ran their Hpsi slower with it,
+// though the plane kernel alone ran faster.)
+template <int R0, int R1, int R2>
+constexpr bool xy_tw_lds() {
+  return R0 * R1 * R2 >= 72;
+}
+
 // Dynamic LDS of an x-y kinetic plane
 template <int R0, int R1, int R2>
 constexpr size_t xy_kin_lds() {
   constexpr size_t L = R0 * R1 * R2;
-  return L * (L + 1) * (xy_split<R0, R1, R2>() ? sizeof(double) : sizeof(cplx));
+  return xy_plane_lds<R0, R1, R2>() + (xy_tw_lds<R0, R1, R2>() ? L * sizeof(cplx) : 0);
 }
 
 // Occupancy floor (amdgpu_waves_per_eu) of an NT-thread x-y kinetic kernel:
@@ -323,11 +342,25 @@ __device__ __forceinline__ void xy_kin_plane_from(const fft3_args &a, long plane
 
 template <int R0, int R1, int R2, int NT = plane_threads(R0 * R1 * R2)>
 __device__ __forceinline__ void xy_kin_plane(const fft3_args &a, long plane, cplx *lds) {
+  constexpr int L = R0 * R1 * R2;
   xy_regs<R0, R1, R2, NT> in;
   xy_load<R0, R1, R2, NT>(a, plane, in);
+  const cplx *tw = a.tw;
+  if constexpr (xy_tw_lds<R0, R1, R2>()) {
+    // the twiddles go straight into LDS (16 B per lane) with the plane's
+    // loads in flight; the first stage needs none, and the barrier after it
+    // makes every wave's copy visible
+    static_assert(NT >= 64 * ((L + 63) / 64), "waves for the twiddle copy");
+    cplx *ltw = reinterpret_cast<cplx *>(reinterpret_cast<char *>(lds) + xy_plane_lds<R0, R1, R2>());
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (64 * wave < L && 64 * wave + lane < L)
+      __builtin_amdgcn_global_load_lds(a.tw + 64 * wave + lane, ltw + 64 * wave, 16, 0, 0);
+    __builtin_amdgcn_s_waitcnt(WAITCNT_VM0);
+    tw = ltw;
+  }
   double kc[3];
   xy_kc(a, kc);
-  xy_kin_plane_from<R0, R1, R2, NT>(a, plane, lds, in, a.tw, kc);
+  xy_kin_plane_from<R0, R1, R2, NT>(a, plane, lds, in, tw, kc);
 }
 
 // The forward x-y transform of one plane (G-space Hpsi), in place in W
@@ -360,10 +393,6 @@ __device__ __forceinline__ void xy_fwd_plane(const fft3_args &a, long plane, cpl
 // FWD (Taylor chains): the tile then transforms its own hpsi forward in z
 // into W, in place of the next application's z_fwd sweep over the same tile
 // (the chunk's W region the tile read is the region it writes).
-// s_waitcnt immediate that waits for every vector memory operation of the
-// wave (vmcnt 0) and leaves the export and LDS/scalar counters alone
-constexpr unsigned WAITCNT_VM0 = 0x0F70;
-
 // Position of the n-th (from 0) set bit of m, which has more than n set bits
 __device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
   int pos = 0;
