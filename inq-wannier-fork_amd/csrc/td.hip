@@ -474,26 +474,27 @@ struct vgrad_multi_args {
   double *gV;
 };
 
+// MM = E.M (the Taylor order) as a template argument: the loads of a band's
+// M powers of each field are all issued before the sums use them (with a
+// runtime M each load sat in a branch of its own and waited for itself)
+template <int MM>
 __global__ __launch_bounds__(256) void vgrad_multi_kernel(const vgrad_multi_args a,
                                                           const coef_matrix E) {
   const int lane = threadIdx.x & 63;
   const long p = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= a.npoint) return;
-  const int M = E.M;
   double s = 0.0;
   for (int b = lane; b < a.nb; b += 64) {
-    cplx y[TD_MAXM];
+    cplx y[MM], x[MM];
 #pragma unroll
-    for (int j = 0; j < TD_MAXM; j++)
-      if (j < M) y[j] = a.Y[j][p * a.ld + b];
+    for (int j = 0; j < MM; j++) y[j] = a.Y[j][p * a.ld + b];
 #pragma unroll
-    for (int k = 0; k < TD_MAXM; k++) {
-      if (k < M) {
-        const cplx x = a.A[k][p * a.ld + b];
+    for (int k = 0; k < MM; k++) x[k] = a.A[k][p * a.ld + b];
 #pragma unroll
-        for (int j = 0; j < TD_MAXM; j++)
-          if (k + j < M) s += re_e_conj_a_y(E.er[k][j], E.ei[k][j], x, y[j]);
-      }
+    for (int k = 0; k < MM; k++) {
+#pragma unroll
+      for (int j = 0; j < MM; j++)
+        if (k + j < MM) s += re_e_conj_a_y(E.er[k][j], E.ei[k][j], x[k], y[j]);
     }
   }
 #pragma unroll
@@ -509,11 +510,12 @@ struct gpair_args {
   double bd[3], kad;
 };
 
-// per-block partial sums of Re sum_kj E[k][j] w(G) conj(A_k^) Y_j^
+// per-block partial sums of Re sum_kj E[k][j] w(G) conj(A_k^) Y_j^ (MM = E.M,
+// as in vgrad_multi_kernel)
+template <int MM>
 __global__ __launch_bounds__(256) void gpair_kernel(const gpair_args a, const coef_matrix E,
                                                     double *partial) {
   __shared__ double red[256];
-  const int M = E.M;
   const int L = a.L;
   const long L2 = (long)L * L, L3 = L2 * L;
   double s = 0.0;
@@ -522,19 +524,17 @@ __global__ __launch_bounds__(256) void gpair_kernel(const gpair_args a, const co
     const int kz = (int)(r / L2), kx = (int)((r / L) % L), ky = (int)(r % L);
     const double w = sym_index(kx, L) * a.bd[0] + sym_index(ky, L) * a.bd[1] +
                      sym_index(kz, L) * a.bd[2] + a.kad;
-    cplx y[TD_MAXM];
+    cplx y[MM], x[MM];
 #pragma unroll
-    for (int j = 0; j < TD_MAXM; j++)
-      if (j < M) y[j] = a.Y[j][e];
+    for (int j = 0; j < MM; j++) y[j] = a.Y[j][e];
+#pragma unroll
+    for (int k = 0; k < MM; k++) x[k] = a.A[k][e];
     double t = 0.0;
 #pragma unroll
-    for (int k = 0; k < TD_MAXM; k++) {
-      if (k < M) {
-        const cplx x = a.A[k][e];
+    for (int k = 0; k < MM; k++) {
 #pragma unroll
-        for (int j = 0; j < TD_MAXM; j++)
-          if (k + j < M) t += re_e_conj_a_y(E.er[k][j], E.ei[k][j], x, y[j]);
-      }
+      for (int j = 0; j < MM; j++)
+        if (k + j < MM) t += re_e_conj_a_y(E.er[k][j], E.ei[k][j], x[k], y[j]);
     }
     s += w * t;
   }
@@ -949,8 +949,17 @@ void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double 
     a.nb = nb;
     a.npoint = h.npoint;
     a.gV = td->gV;
-    hipLaunchKernelGGL(vgrad_multi_kernel, dim3((unsigned)((h.npoint + 3) / 4)), dim3(256), 0,
-                       h.stream, a, E);
+    const dim3 grid((unsigned)((h.npoint + 3) / 4));
+    switch (M) {
+#define INQAMD_VGRAD(m)                                                                   \
+  case m:                                                                                \
+    hipLaunchKernelGGL(vgrad_multi_kernel<m>, grid, dim3(256), 0, h.stream, a, E);        \
+    break;
+      INQAMD_VGRAD(1) INQAMD_VGRAD(2) INQAMD_VGRAD(3) INQAMD_VGRAD(4)
+      INQAMD_VGRAD(5) INQAMD_VGRAD(6) INQAMD_VGRAD(7) INQAMD_VGRAD(8)
+#undef INQAMD_VGRAD
+      default: throw error{ERR_ARGUMENT, "Taylor order outside 1..8"};
+    }
     INQAMD_HIP_CHECK(hipGetLastError());
   }
   double *rec = obs_record(h, 0);
@@ -967,7 +976,17 @@ void adjoint_half_fused(inqamd_td *td, int nb, cplx *const *Y, int slot, double 
     a.total = h.npoint * nb;
     for (int e = 0; e < 3; e++) a.bd[e] = w.bd[e];
     a.kad = w.kad;
-    hipLaunchKernelGGL(gpair_kernel, dim3(OBS_GS_BLOCKS), dim3(256), 0, h.stream, a, E, part);
+    switch (M) {
+#define INQAMD_GPAIR(m)                                                                   \
+  case m:                                                                                \
+    hipLaunchKernelGGL(gpair_kernel<m>, dim3(OBS_GS_BLOCKS), dim3(256), 0, h.stream, a, E, \
+                       part);                                                            \
+    break;
+      INQAMD_GPAIR(1) INQAMD_GPAIR(2) INQAMD_GPAIR(3) INQAMD_GPAIR(4)
+      INQAMD_GPAIR(5) INQAMD_GPAIR(6) INQAMD_GPAIR(7) INQAMD_GPAIR(8)
+#undef INQAMD_GPAIR
+      default: throw error{ERR_ARGUMENT, "Taylor order outside 1..8"};
+    }
     INQAMD_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(SUM_ROWS_NT), 0, h.stream,
                        (const double *)part, OBS_GS_BLOCKS, 1, rec);
