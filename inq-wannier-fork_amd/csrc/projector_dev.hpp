@@ -244,62 +244,92 @@ __device__ __forceinline__ double mfma4(double a, double b, double c) {
   return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
 }
 
-// Sums over the sphere points [ib, ie) of sphere p for bands
-// [b0, b0 + 16 NB): the waves' accumulators are added in wave order into
-// out[NB][64][2] (lane order: lm = l >> 4, band b0 + 16 q + (l & 15); real,
-// imaginary), unscaled.  Each point's M row and phase serve NB band groups.
-template <int NB>
-__device__ __forceinline__ void gemm1_p4_points(const project_expand_args &a, int p, int b0,
-                                                int ib, int ie, double (*red)[NB][64][2],
-                                                double *out) {
+// One wave's sums over the points j0 + kk + u STEP (j0 = j, j + STEP U, ...
+// below ie; kk = lane >> 4 picks one of 4 points of a step) of sphere p for
+// bands [b0, b0 + 16 NB) into accr/acci (lane order: lm = l >> 4, band
+// b0 + 16 q + (l & 15)).  Each point's M row and phase serve NB band groups.
+template <int NB, int STEP>
+__device__ __forceinline__ void p4_wave_points(const project_expand_args &a, int p, int b0,
+                                               int j, int ie, double *accr, double *acci) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   const int nlm = a.max_nlm;
   const long off = (long)p * a.max_sphere;
   const double *M = a.mat + (long)p * nlm * a.max_sphere;
   const int kk = lane >> 4;
   const int lm_a = lane & 3;
   // the lane's chunk bands b0 + (lane & 15) + 16 q and their offsets in psi
+  // (band 0 of the row for a band past the chunk)
   int gband[NB];
   bool bok[NB];
 #pragma unroll
   for (int q = 0; q < NB; q++) {
     const int cb = b0 + (lane & 15) + 16 * q;
     bok[q] = cb < a.nbc;
-    gband[q] = cb + (cb >> (3 + a.glog)) * a.gskip;
+    gband[q] = bok[q] ? cb + (cb >> (3 + a.glog)) * a.gskip : 0;
   }
-  double accr[NB], acci[NB];
+  const long mrow = (long)(lm_a < nlm ? lm_a : 0) * a.max_sphere;
 #pragma unroll
   for (int q = 0; q < NB; q++) accr[q] = acci[q] = 0.0;
-  constexpr int STEP = P4_WAVES * 4;
-  for (int j0 = ib + wave * 4; j0 < ie; j0 += STEP * P4_UNROLL) {
-    int idx[P4_UNROLL];
+  // Every load reads an address in bounds (a lane past the range reads the
+  // sphere's point 0) and its value is selected after it arrives, so no load
+  // sits in a branch of its own.  The point indices of a step are loaded one
+  // step ahead: the gathers of psi wait for no index load, and the MFMAs
+  // wait for their own operands only.
+  int idx[P4_UNROLL];
 #pragma unroll
-    for (int u = 0; u < P4_UNROLL; u++) {
-      const int i = j0 + kk + u * STEP;
-      idx[u] = i < ie ? a.point[off + i] : -1;
-    }
+  for (int u = 0; u < P4_UNROLL; u++) {
+    const int i = j + kk + u * STEP;
+    idx[u] = a.point[off + (i < ie ? i : 0)];
+  }
+  for (int j0 = j; j0 < ie; j0 += STEP * P4_UNROLL) {
     cplx ps[P4_UNROLL][NB], ph[P4_UNROLL];
     double av[P4_UNROLL];
 #pragma unroll
     for (int u = 0; u < P4_UNROLL; u++) {
       const int i = j0 + kk + u * STEP;
+      const int ic = i < ie ? i : 0;
 #pragma unroll
-      for (int q = 0; q < NB; q++)
-        ps[u][q] = (idx[u] >= 0 && bok[q]) ? a.psi[(long)idx[u] * a.ld + gband[q]] : mk(0.0, 0.0);
-      ph[u] = i < ie ? a.phase[off + i] : mk(0.0, 0.0);
-      av[u] = (i < ie && lm_a < nlm) ? M[(long)lm_a * a.max_sphere + i] : 0.0;
+      for (int q = 0; q < NB; q++) ps[u][q] = a.psi[(long)idx[u] * a.ld + gband[q]];
+      ph[u] = a.phase[off + ic];
+      av[u] = M[mrow + ic];
     }
+    // the next indices are issued after this step's operands (the scheduler
+    // would otherwise mix them, and the MFMAs would wait for every load)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < P4_UNROLL; u++) {
+      const int i = j0 + STEP * P4_UNROLL + kk + u * STEP;
+      idx[u] = a.point[off + (i < ie ? i : 0)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < P4_UNROLL; u++) {
+      const bool in = j0 + kk + u * STEP < ie;
+      const cplx f = in ? ph[u] : mk(0.0, 0.0);
+      const double m = (in && lm_a < nlm) ? av[u] : 0.0;
 #pragma unroll
       for (int q = 0; q < NB; q++) {
-        const cplx x = cmul(ph[u], ps[u][q]);
-        accr[q] = mfma4(av[u], x.x, accr[q]);
-        acci[q] = mfma4(av[u], x.y, acci[q]);
+        const cplx x = cmul(f, (in && bok[q]) ? ps[u][q] : mk(0.0, 0.0));
+        accr[q] = mfma4(m, x.x, accr[q]);
+        acci[q] = mfma4(m, x.y, acci[q]);
       }
     }
   }
+}
+
+// Sums over the sphere points [ib, ie) of sphere p for bands
+// [b0, b0 + 16 NB): wave w takes the steps of 4 points at ib + 4 w,
+// ib + 4 w + 4 P4_WAVES, ...; the waves' accumulators are added in wave
+// order into out[NB][64][2] (lane order of p4_wave_points; real,
+// imaginary), unscaled.
+template <int NB>
+__device__ __forceinline__ void gemm1_p4_points(const project_expand_args &a, int p, int b0,
+                                                int ib, int ie, double (*red)[NB][64][2],
+                                                double *out) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  double accr[NB], acci[NB];
+  p4_wave_points<NB, P4_WAVES * 4>(a, p, b0, ib + wave * 4, ie, accr, acci);
 #pragma unroll
   for (int q = 0; q < NB; q++) {
     red[wave][q][lane][0] = accr[q];
