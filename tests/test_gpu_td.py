@@ -283,3 +283,44 @@ def test_autograd_second_forward_before_backward_raises(device, small):
         prop.backward(1.0)
     prop.close()
     ham.close()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 5, 6, 7, 8])
+def test_adjoint_taylor_orders(device, order):
+    """The fused half-step kernels (dL/dV, the G-space pair sums) take the
+    Taylor order as a template argument: at each order from 1 to 8 other
+    than the 4 of the tests above, the 24^3 case's dL/dc matches central
+    differences of the oracle's loss (h = 1e-5, 1e-6 of the largest), and
+    dL/dc and dL/dV match the unfused form (store_powers=False, which keeps
+    no powers and runs none of those kernels) to 1e-12."""
+    say("adjoint Taylor order", order)
+    s, psi = _small_case("cubic")
+    ham = hamiltonian.ks_hamiltonian(s, 3)
+    out = []
+    for store in (True, False):
+        prop = adjoint.propagation_adjoint(
+            ham, 3, PROP["nsteps"], PROP["dt"], len(COEF), PROP["omega"], PROP["direction"],
+            order, PROP["occ"], checkpoint_every=2, band_slice=0, store_powers=store,
+            commutator="inq")
+        prop.forward(COEF, to_dev(psi, device))
+        gc, gV, _ = prop.backward(1.0)
+        torch.cuda.synchronize()
+        out.append((gc.copy(), gV.cpu().numpy()))
+        prop.close()
+    ham.close()
+    (gc, gV), (gc_u, gV_u) = out
+    assert np.abs(gc - gc_u).max() <= 1e-12 * np.abs(gc_u).max()
+    assert np.abs(gV - gV_u).max() <= 1e-12 * np.abs(gV_u).max()
+    osys = O.System(s.n, s.lattice, s.vloc, s.tables)
+
+    def loss(c):
+        v, _ = osys.td_current(psi, c, PROP["omega"], PROP["direction"], PROP["dt"],
+                               PROP["nsteps"], order, PROP["occ"], commutator="inq")
+        return v
+
+    h = 1e-5
+    fd = np.array([(loss(COEF + h * e) - loss(COEF - h * e)) / (2 * h)
+                   for e in np.eye(COEF.size)])
+    scale = np.abs(fd).max()
+    assert scale > 1e-6
+    assert np.abs(gc - fd).max() <= 1e-6 * scale, (order, gc, fd)
