@@ -324,21 +324,39 @@ __device__ double tpa_k(double x) {
   return num / den;
 }
 
-// pass 3: phi(G) *= K(1/2 |G|^2 / kinetic_b) (preconditioner.hpp:45-52)
+// pass 3: phi(G) *= K(1/2 |G|^2 / kinetic_b) (preconditioner.hpp:45-52).
+// A wave takes one grid point at a time, its lanes the bands: the point's
+// G vector is worked out once per point (the index divisions of a flat
+// per-element loop cost more than the memory traffic).
+constexpr int PRECOND_POINT_WAVES = 65536;  // waves of the grid-stride loop at most
+
 __global__ __launch_bounds__(256) void precondition_kernel(gspace geo, cplx *__restrict__ phi,
                                                            long ld, int nst, long npoint,
                                                            const double *__restrict__ band) {
-  const long total = npoint * nst;
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < total; i += (long)gridDim.x * 256) {
-    const long p = i / nst;
-    const int b = (int)(i % nst);
+  const int lane = threadIdx.x & 63;
+  const long nwaves = (long)gridDim.x * 4;
+  for (long p = ((long)blockIdx.x * 256 + threadIdx.x) >> 6; p < npoint; p += nwaves) {
     int m0, m1, m2;
     geo.m_of(p, m0, m1, m2);
-    const double k = tpa_k(-0.5 * (-geo.g2(m0, m1, m2)) / band[b * 16 + 5]);
-    cplx f = phi[p * ld + b];
-    f.x *= k;
-    f.y *= k;
-    phi[p * ld + b] = f;
+    const double g2 = geo.g2(m0, m1, m2);
+    // four bands per lane in flight (a band past nst reads the lane's first
+    // band and is not stored)
+    for (int b0 = lane; b0 < nst; b0 += 256) {
+      cplx f[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int b = b0 + 64 * j;
+        f[j] = phi[p * ld + (b < nst ? b : b0)];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int b = b0 + 64 * j;
+        if (b < nst) {
+          const double k = tpa_k(-0.5 * (-g2) / band[b * 16 + 5]);
+          phi[p * ld + b] = mk(f[j].x * k, f[j].y * k);
+        }
+      }
+    }
   }
 }
 
@@ -430,8 +448,9 @@ void precondition_apply(handle &h, cplx *phi, long ld, int nst) {
   hipLaunchKernelGGL(kinetic_final, dim3(nst), dim3(256), 0, h.stream,
                      (const double *)g.part, nchunks(h), nst, g.band);
   check_launch();
-  hipLaunchKernelGGL(precondition_kernel, dim3(grid1(h.npoint * nst)), dim3(256), 0, h.stream,
-                     geometry(h), phi, ld, nst, h.npoint, (const double *)g.band);
+  const long waves = h.npoint < PRECOND_POINT_WAVES ? h.npoint : PRECOND_POINT_WAVES;
+  hipLaunchKernelGGL(precondition_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0,
+                     h.stream, geometry(h), phi, ld, nst, h.npoint, (const double *)g.band);
   check_launch();
 }
 
