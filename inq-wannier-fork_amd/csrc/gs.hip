@@ -14,7 +14,7 @@
 //   pass 3  residual *= K(1/2|G|^2 / kinetic)
 //   H residual (G-space Hpsi)
 //   pass 4  <r|r>, <phi|r>, <r|Hr>, <phi|Hr> and the step lambda per band
-//   pass 5  phi += lambda r, hphi += lambda Hr
+//   pass 5  phi += lambda r, hphi += lambda Hr (with the next step's pass-1 sums)
 // orthogonalize forms S = dV Phi^H Phi (a split-K strided-batched ZGEMM),
 // factors it on the host (S = R R^H, R lower) and applies Phi <- Phi R^-H
 // with ZGEMM; the subspace diagonalisation diagonalises dV Phi^H H Phi with
@@ -388,6 +388,34 @@ __global__ __launch_bounds__(256) void sd_update_kernel(cplx *__restrict__ phi, 
   }
 }
 
+// pass 5 of a step fused with pass 1 of the next (the partial sums of
+// eig_norm_partial over the updated phi and hphi, in its order), for steps
+// that update hphi
+__global__ __launch_bounds__(256) void sd_update_eig_norm(cplx *__restrict__ phi, long ld,
+                                                          const cplx *__restrict__ res,
+                                                          cplx *__restrict__ hphi,
+                                                          const cplx *__restrict__ hres, long rld,
+                                                          int nst, long npoint,
+                                                          const double *__restrict__ band,
+                                                          double *part) {
+  band_partials<3>(npoint, nst, part, [&](long p, int b, double *a) {
+    const double lam = band[b * 16 + 6];
+    cplx f = phi[p * ld + b];
+    const cplx r = res[p * rld + b];
+    cplx h = hphi[p * rld + b];
+    const cplx hr = hres[p * rld + b];
+    f.x += lam * r.x;
+    f.y += lam * r.y;
+    h.x += lam * hr.x;
+    h.y += lam * hr.y;
+    phi[p * ld + b] = f;
+    hphi[p * rld + b] = h;
+    a[0] += h.x * f.x + h.y * f.y;
+    a[1] += h.x * f.y - h.y * f.x;
+    a[2] += f.x * f.x + f.y * f.y;
+  });
+}
+
 dim3 red_grid(const handle &h, int nst) {
   return dim3((unsigned)nchunks(h), (unsigned)((nst + GS_BT - 1) / GS_BT));
 }
@@ -565,9 +593,12 @@ void steepest_descent(handle &h, cplx *phi, long ld, int nst, bool precond) {
   hamiltonian_fourier(h, phi, ld, g.hphi, rl, nst);
   const int num_steps = 5;
   for (int istep = 0; istep < num_steps; istep++) {
-    hipLaunchKernelGGL(eig_norm_partial, rg, dim3(256), 0, h.stream, (const cplx *)g.hphi, rl,
-                       (const cplx *)phi, ld, nst, h.npoint, g.part);
-    check_launch();
+    // pass 1 (after the first step, the previous step's update made its sums)
+    if (istep == 0) {
+      hipLaunchKernelGGL(eig_norm_partial, rg, dim3(256), 0, h.stream, (const cplx *)g.hphi,
+                         rl, (const cplx *)phi, ld, nst, h.npoint, g.part);
+      check_launch();
+    }
     hipLaunchKernelGGL(eig_norm_final, fg, dim3(256), 0, h.stream, (const double *)g.part, nchunk,
                        nst, dV, g.band);
     check_launch();
@@ -583,9 +614,14 @@ void steepest_descent(handle &h, cplx *phi, long ld, int nst, bool precond) {
     hipLaunchKernelGGL(lambda_final, fg, dim3(256), 0, h.stream, (const double *)g.part, nchunk,
                        nst, dV, g.band);
     check_launch();
-    hipLaunchKernelGGL(sd_update_kernel, dim3(eg), dim3(256), 0, h.stream, phi, ld,
-                       (const cplx *)g.res, g.hphi, (const cplx *)g.hres, rl, nst, h.npoint,
-                       (const double *)g.band, istep != num_steps - 1 ? 1 : 0);
+    if (istep != num_steps - 1)
+      hipLaunchKernelGGL(sd_update_eig_norm, rg, dim3(256), 0, h.stream, phi, ld,
+                         (const cplx *)g.res, g.hphi, (const cplx *)g.hres, rl, nst, h.npoint,
+                         (const double *)g.band, g.part);
+    else
+      hipLaunchKernelGGL(sd_update_kernel, dim3(eg), dim3(256), 0, h.stream, phi, ld,
+                         (const cplx *)g.res, g.hphi, (const cplx *)g.hres, rl, nst, h.npoint,
+                         (const double *)g.band, 0);
     check_launch();
   }
   orthogonalize(h, phi, ld, nst, dV);
