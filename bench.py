@@ -450,10 +450,13 @@ def _gs_bytes(N, nst, sum_k):
     diagonalisation, to_fourier, steepest descent, to_real) on nst bands:
     seven Hpsi at B_H (one in the subspace step, six G-space ones in steepest
     descent, each of those with two more transforms at 32N), the orbitals'
-    two transforms, the five line-search passes (256N: 16 B per array read or
-    written) and the two rotations (read + write, 32N each)."""
+    two transforms, the line-search passes of five line minimisations (256N:
+    16 B per array read or written; 224N in the last four, whose first pass
+    comes with the previous step's update) and the two rotations (read +
+    write, 32N each)."""
     b_h = algorithmic_bytes_per_band(N, nst, sum_k)
-    return nst * (7.0 * b_h + 6 * 64.0 * N + 64.0 * N + 5 * 256.0 * N + 2 * 32.0 * N)
+    return nst * (7.0 * b_h + 6 * 64.0 * N + 64.0 * N + (256.0 + 4 * 224.0) * N +
+                  2 * 32.0 * N)
 
 
 def run_gs_64(args):
