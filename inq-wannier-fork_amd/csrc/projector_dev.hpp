@@ -251,6 +251,10 @@ __device__ __forceinline__ double mfma4(double a, double b, double c) {
 template <int NB, int STEP>
 __device__ __forceinline__ void p4_wave_points(const project_expand_args &a, int p, int b0,
                                                int j, int ie, double *accr, double *acci) {
+  // point steps in flight: 8 for one band group (the small per-chunk
+  // launches, which have few workgroups), 4 for more (registers); a wave
+  // adds its points in increasing order either way
+  constexpr int U = NB == 1 ? 8 : P4_UNROLL;
   const int lane = threadIdx.x & 63;
   const int nlm = a.max_nlm;
   const long off = (long)p * a.max_sphere;
@@ -275,17 +279,17 @@ __device__ __forceinline__ void p4_wave_points(const project_expand_args &a, int
   // sits in a branch of its own.  The point indices of a step are loaded one
   // step ahead: the gathers of psi wait for no index load, and the MFMAs
   // wait for their own operands only.
-  int idx[P4_UNROLL];
+  int idx[U];
 #pragma unroll
-  for (int u = 0; u < P4_UNROLL; u++) {
+  for (int u = 0; u < U; u++) {
     const int i = j + kk + u * STEP;
     idx[u] = a.point[off + (i < ie ? i : 0)];
   }
-  for (int j0 = j; j0 < ie; j0 += STEP * P4_UNROLL) {
-    cplx ps[P4_UNROLL][NB], ph[P4_UNROLL];
-    double av[P4_UNROLL];
+  for (int j0 = j; j0 < ie; j0 += STEP * U) {
+    cplx ps[U][NB], ph[U];
+    double av[U];
 #pragma unroll
-    for (int u = 0; u < P4_UNROLL; u++) {
+    for (int u = 0; u < U; u++) {
       const int i = j0 + kk + u * STEP;
       const int ic = i < ie ? i : 0;
 #pragma unroll
@@ -297,13 +301,13 @@ __device__ __forceinline__ void p4_wave_points(const project_expand_args &a, int
     // would otherwise mix them, and the MFMAs would wait for every load)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < P4_UNROLL; u++) {
-      const int i = j0 + STEP * P4_UNROLL + kk + u * STEP;
+    for (int u = 0; u < U; u++) {
+      const int i = j0 + STEP * U + kk + u * STEP;
       idx[u] = a.point[off + (i < ie ? i : 0)];
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < P4_UNROLL; u++) {
+    for (int u = 0; u < U; u++) {
       const bool in = j0 + kk + u * STEP < ie;
       const cplx f = in ? ph[u] : mk(0.0, 0.0);
       const double m = (in && lm_a < nlm) ? av[u] : 0.0;
