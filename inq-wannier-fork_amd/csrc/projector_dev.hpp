@@ -275,8 +275,9 @@ __device__ __forceinline__ void p4_wave_points(const project_expand_args &a, int
 #pragma unroll
   for (int q = 0; q < NB; q++) accr[q] = acci[q] = 0.0;
   // Every load reads an address in bounds (a lane past the range reads the
-  // sphere's point 0) and its value is selected after it arrives, so no load
-  // sits in a branch of its own.  The point indices of a step are loaded one
+  // sphere's point 0, a grid index whenever the range is non-empty; table
+  // padding is -1, but for an empty range no psi is gathered) and its value
+  // is selected after it arrives, so no load sits in a branch of its own.  The point indices of a step are loaded one
   // step ahead: the gathers of psi wait for no index load, and the MFMAs
   // wait for their own operands only.
   int idx[U];
