@@ -109,6 +109,42 @@ def test_projector_project_apply_and_non_local(device, cfg0):
     ham.close()
 
 
+def test_hpsi_locally_empty_projectors(device, cfg0):
+    """projector::locally_empty (projector.hpp:89-91): sphere 2 with no points
+    (its table rows stay padding, -1) and sphere 5 with nlm = 0.  Hpsi (k+A
+    != 0) against the oracle on the same tables (1e-12), and against the
+    library's Hpsi of the system without those two spheres (1e-14); the
+    oracle's result for the emptied tables equals its result without the two
+    spheres exactly, and differs from the full system's by 1.5e-4."""
+    import copy
+    s, psi, _ = cfg0
+    kA = (0.05, -0.02, 0.03)
+    say("locally empty projectors")
+    t = dict(s.tables)
+    t["sphere_size"] = t["sphere_size"].copy()
+    t["sphere_size"][2] = 0
+    t["nlm"] = t["nlm"].copy()
+    t["nlm"][5] = 0
+    se = copy.copy(s)
+    se.tables = t
+    keep = [i for i in range(t["nproj"]) if i not in (2, 5)]
+    td = {k: (v[keep] if hasattr(v, "shape") and v.shape[:1] == (t["nproj"],) else v)
+          for k, v in s.tables.items()}
+    td["nproj"] = len(keep)
+    sd = copy.copy(s)
+    sd.tables = td
+    out = []
+    for sys_ in (se, sd):
+        ham = inq_amd.ks_hamiltonian(sys_, max_bands=16)
+        ham.uniform_vector_potential = kA
+        out.append(to_host(ham(to_dev(psi, device))))
+        ham.close()
+    ref = O.System(se.n, se.lattice, se.vloc, se.tables).hamiltonian(psi, kA)
+    assert relerr(out[0], ref) < TOL
+    assert band_relerr(out[0], ref) < TOL
+    assert relerr(out[0], out[1]) < 1e-14
+
+
 def test_hpsi_bitwise_deterministic(device, cfg0):
     s, psi, _ = cfg0
     say("determinism")
