@@ -61,7 +61,6 @@ template <int R, int NS, int LEN, int B, int NT>
 __device__ __forceinline__ void ptw(stage_tw<R, LEN, B, NT> &t, const cplx *__restrict__ tw,
                                     int sign) {
   if constexpr (NS > 1) {
-    constexpr int NB = LEN / R;
     constexpr int NBF = stage_regs<R, LEN, B, NT>::NBF;
     constexpr int TSTRIDE = LEN / (NS * R);
 #pragma unroll

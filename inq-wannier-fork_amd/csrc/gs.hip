@@ -360,31 +360,26 @@ __global__ __launch_bounds__(256) void precondition_kernel(gspace geo, cplx *__r
   }
 }
 
-// pass 5 (steepest_descent.hpp:70-77)
-__global__ __launch_bounds__(256) void sd_update_kernel(cplx *__restrict__ phi, long ld,
-                                                        const cplx *__restrict__ res,
-                                                        cplx *__restrict__ hphi,
-                                                        const cplx *__restrict__ hres, long rld,
-                                                        int nst, long npoint,
-                                                        const double *__restrict__ band,
-                                                        int update_hphi) {
-  const long total = npoint * nst;
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < total; i += (long)gridDim.x * 256) {
-    const long p = i / nst;
-    const int b = (int)(i % nst);
-    const double lam = band[b * 16 + 6];
+// pass 5 (steepest_descent.hpp:70-77) of the last step, which leaves hphi:
+// phi += lambda r over the grid of the partial-sum passes (point chunk x
+// 64-band block; the lanes of a point chunk stride its points)
+__global__ __launch_bounds__(256) void sd_update_phi(cplx *__restrict__ phi, long ld,
+                                                     const cplx *__restrict__ res, long rld,
+                                                     int nst, long npoint,
+                                                     const double *__restrict__ band) {
+  const int bb = threadIdx.x % GS_BT;
+  const int lane = threadIdx.x / GS_BT;
+  const int b = blockIdx.y * GS_BT + bb;
+  if (b >= nst) return;
+  const long p0 = (long)blockIdx.x * GS_CHUNK;
+  const long p1 = p0 + GS_CHUNK < npoint ? p0 + GS_CHUNK : npoint;
+  const double lam = band[b * 16 + 6];
+  for (long p = p0 + lane; p < p1; p += GS_LANES) {
     cplx f = phi[p * ld + b];
     const cplx r = res[p * rld + b];
     f.x += lam * r.x;
     f.y += lam * r.y;
     phi[p * ld + b] = f;
-    if (update_hphi) {
-      cplx h = hphi[p * rld + b];
-      const cplx hr = hres[p * rld + b];
-      h.x += lam * hr.x;
-      h.y += lam * hr.y;
-      hphi[p * rld + b] = h;
-    }
   }
 }
 
@@ -589,7 +584,6 @@ void steepest_descent(handle &h, cplx *phi, long ld, int nst, bool precond) {
   // fourier_space::volume_element (fourier_space.hpp:41-44)
   const double dV = h.volume / ((double)h.npoint * (double)h.npoint);
   const dim3 rg = red_grid(h, nst), fg(nst);
-  const unsigned eg = grid1(h.npoint * nst);
   hamiltonian_fourier(h, phi, ld, g.hphi, rl, nst);
   const int num_steps = 5;
   for (int istep = 0; istep < num_steps; istep++) {
@@ -619,9 +613,8 @@ void steepest_descent(handle &h, cplx *phi, long ld, int nst, bool precond) {
                          (const cplx *)g.res, g.hphi, (const cplx *)g.hres, rl, nst, h.npoint,
                          (const double *)g.band, g.part);
     else
-      hipLaunchKernelGGL(sd_update_kernel, dim3(eg), dim3(256), 0, h.stream, phi, ld,
-                         (const cplx *)g.res, g.hphi, (const cplx *)g.hres, rl, nst, h.npoint,
-                         (const double *)g.band, 0);
+      hipLaunchKernelGGL(sd_update_phi, rg, dim3(256), 0, h.stream, phi, ld,
+                         (const cplx *)g.res, rl, nst, h.npoint, (const double *)g.band);
     check_launch();
   }
   orthogonalize(h, phi, ld, nst, dV);
